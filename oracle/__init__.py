@@ -1,0 +1,466 @@
+"""CPU oracle for ct-2's reconstruction hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker.  The product
+(``ct-2_amd/ct2``) never imports it.
+
+Two backends expose the reference's f2py call signatures (hidden
+``intent(out)`` arguments are returned, dimensions passed explicitly):
+
+* ``oracle.restated(prec)``: the C restatement in ``ct2_oracle.c`` /
+  ``ora_body.h``.  ``prec="f32"`` is Fortran-faithful FP32, ``prec="f64"``
+  the same algorithm in FP64.
+* ``oracle.reference()``: the reference Fortran itself
+  (``emil_code/{view_proj3D,view_proj3D_tomo,ray_proj3D,totalvar}.f``)
+  compiled by ``oracle/Makefile`` into ``oracle/_ref/``, called through the
+  Fortran ABI (every argument by reference, arrays column-major).  Arrays are
+  handed over as F-ordered copies of the same logical arrays, and outputs are
+  zero-filled first, which is what f2py does (numpy/f2py fortranobject.c).
+
+FDK (``emil_code/FDK.py``) is pure numpy in the reference; its restatement is
+``oracle.fdk``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from types import SimpleNamespace
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LINEAR, NEAREST = 0, 1
+
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_i8p = C.POINTER(C.c_int8)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is not built; run `make -C oracle` (or __graft_entry__.build())")
+    return C.CDLL(path)
+
+
+_ORA = None
+
+
+def _ora():
+    global _ORA
+    if _ORA is None:
+        _ORA = _load(os.path.join(HERE, "liboracle.so"))
+    return _ORA
+
+
+def ref_available() -> bool:
+    return all(os.path.exists(os.path.join(HERE, "_ref", f"lib{m}.so"))
+               for m in ("view_proj3D", "view_proj3D_tomo", "ray_proj3D", "totalvar"))
+
+
+# --------------------------------------------------------------------------
+# C restatement
+# --------------------------------------------------------------------------
+class _Restated:
+    """f2py-signature facade over ct2_oracle.c in one precision."""
+
+    def __init__(self, prec: str):
+        assert prec in ("f32", "f64")
+        self.prec = prec
+        self.dt = np.float32 if prec == "f32" else np.float64
+        self.cr = C.c_float if prec == "f32" else C.c_double
+        self.lib = _ora()
+
+    def _fn(self, name, restype=None):
+        f = getattr(self.lib, f"{name}_{self.prec}")
+        f.restype = restype
+        return f
+
+    def _a(self, x):
+        return np.ascontiguousarray(x, dtype=self.dt)
+
+    @staticmethod
+    def _ind(ind, shape):
+        if ind is None:
+            return np.ones(shape, np.int8)
+        return np.ascontiguousarray(ind, dtype=np.int8).reshape(shape)
+
+    def _geo(self, *vals):
+        return [self.cr(v) for v in vals]
+
+    # ---- view-indexed operators (view_proj3D.f / view_proj3D_tomo.f) ----
+    def sinoproj(self, interp, zonly, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+                 smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((ns, nu, nv), self.dt)
+        ind = self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32)
+        vol = self._a(smat).reshape(nx, ny, nz)
+        self._fn("ora_sinoproj")(C.c_int(interp), C.c_int(zonly), _ptr(out), _ptr(ind), _ptr(fv),
+                                 C.c_int(ns), C.c_int(nu), C.c_int(nv), *self._geo(du, dv, u0, v0),
+                                 _ptr(vol), *self._geo(dx, dy, dz, x0, y0, z0),
+                                 C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return out
+
+    def sinobackproject(self, interp, zonly, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+                        dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nx, ny, nz), self.dt)
+        ind = self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32)
+        s = self._a(sinomat).reshape(ns, nu, nv)
+        self._fn("ora_sinobackproject")(C.c_int(interp), C.c_int(zonly), _ptr(out), _ptr(s),
+                                        _ptr(ind), _ptr(fv), C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                                        *self._geo(du, dv, u0, v0, dx, dy, dz, x0, y0, z0),
+                                        C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return out
+
+    def pd_backproj(self, interp, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+                    dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nx, ny, nz), self.dt)
+        fv = np.ascontiguousarray(fv, np.float32)
+        s = self._a(sinomat).reshape(ns, nu, nv)
+        self._fn("ora_pd_backproj")(C.c_int(interp), _ptr(out), _ptr(s), _ptr(fv),
+                                    C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                                    *self._geo(du, dv, u0, v0, dx, dy, dz, x0, y0, z0),
+                                    C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return out
+
+    def artstep(self, interp, zonly, reg_fac, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+                smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._a(smat).reshape(nx, ny, nz).copy()
+        ind = self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32)
+        s = self._a(sinomat).reshape(ns, nu, nv)
+        self._fn("ora_artstep")(C.c_int(interp), C.c_int(zonly), self.cr(reg_fac), _ptr(s),
+                                _ptr(ind), _ptr(fv), C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                                *self._geo(du, dv, u0, v0), _ptr(vol),
+                                *self._geo(dx, dy, dz, x0, y0, z0),
+                                C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return vol
+
+    def artstepz2vol(self, reg_fac, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+                     smat, dx, dy, dz, x0, y0, z0, nx, ny, nz,
+                     smata, dxa, dya, dza, x0a, y0a, z0a, nxa, nya, nza):
+        vol = self._a(smat).reshape(nx, ny, nz).copy()
+        vola = self._a(smata).reshape(nxa, nya, nza).copy()
+        ind = self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32)
+        s = self._a(sinomat).reshape(ns, nu, nv)
+        self._fn("ora_artstepz2vol")(self.cr(reg_fac), _ptr(s), _ptr(ind), _ptr(fv),
+                                     C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                                     *self._geo(du, dv, u0, v0), _ptr(vol),
+                                     *self._geo(dx, dy, dz, x0, y0, z0),
+                                     C.c_int(nx), C.c_int(ny), C.c_int(nz), _ptr(vola),
+                                     *self._geo(dxa, dya, dza, x0a, y0a, z0a),
+                                     C.c_int(nxa), C.c_int(nya), C.c_int(nza))
+        return vol, vola
+
+    def emstep(self, interp, zonly, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+               smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._a(smat).reshape(nx, ny, nz).copy()
+        ind = self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32)
+        s = self._a(sinomat).reshape(ns, nu, nv)
+        self._fn("ora_emstep")(C.c_int(interp), C.c_int(zonly), _ptr(s), _ptr(ind), _ptr(fv),
+                               C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                               *self._geo(du, dv, u0, v0), _ptr(vol),
+                               *self._geo(dx, dy, dz, x0, y0, z0),
+                               C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return vol
+
+    # ---- ray-list operators (ray_proj3D.f) ----
+    def ray_sinoproj(self, interp, indsino, rv, nr, smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nr,), self.dt)
+        ind = self._ind(indsino, (nr,))
+        rv = np.ascontiguousarray(rv, np.float32)
+        vol = self._a(smat).reshape(nx, ny, nz)
+        self._fn("ora_ray_sinoproj")(C.c_int(interp), _ptr(out), _ptr(ind), _ptr(rv), C.c_int(nr),
+                                     _ptr(vol), *self._geo(dx, dy, dz, x0, y0, z0),
+                                     C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return out
+
+    def ray_sinobackproject(self, interp, strict, sinomat, indsino, rv, nr,
+                            dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nx, ny, nz), self.dt)
+        ind = self._ind(indsino, (nr,))
+        rv = np.ascontiguousarray(rv, np.float32)
+        s = self._a(sinomat).reshape(nr)
+        self._fn("ora_ray_sinobackproject")(C.c_int(interp), C.c_int(strict), _ptr(out), _ptr(s),
+                                            _ptr(ind), _ptr(rv), C.c_int(nr),
+                                            *self._geo(dx, dy, dz, x0, y0, z0),
+                                            C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return out
+
+    def ray_artstep(self, interp, strict, reg_fac, sinomat, indsino, rv, nr, smat,
+                    dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._a(smat).reshape(nx, ny, nz).copy()
+        ind = self._ind(indsino, (nr,))
+        rv = np.ascontiguousarray(rv, np.float32)
+        s = self._a(sinomat).reshape(nr)
+        self._fn("ora_ray_artstep")(C.c_int(interp), C.c_int(strict), self.cr(reg_fac), _ptr(s),
+                                    _ptr(ind), _ptr(rv), C.c_int(nr), _ptr(vol),
+                                    *self._geo(dx, dy, dz, x0, y0, z0),
+                                    C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return vol
+
+    def ray_emstep(self, interp, strict, sinomat, indsino, rv, nr, smat,
+                   dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._a(smat).reshape(nx, ny, nz).copy()
+        ind = self._ind(indsino, (nr,))
+        rv = np.ascontiguousarray(rv, np.float32)
+        s = self._a(sinomat).reshape(nr)
+        self._fn("ora_ray_emstep")(C.c_int(interp), C.c_int(strict), _ptr(s), _ptr(ind),
+                                   _ptr(rv), C.c_int(nr), _ptr(vol),
+                                   *self._geo(dx, dy, dz, x0, y0, z0),
+                                   C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return vol
+
+    # ---- TV (totalvar.f) ----
+    def total_var3d(self, ambdala, smat, nx, ny, nz, eps=1e-9):
+        vol = self._a(smat).reshape(nx, ny, nz)
+        return float(self._fn("ora_total_var3d", C.c_double)(
+            self.cr(ambdala), _ptr(vol), C.c_int(nx), C.c_int(ny), C.c_int(nz), self.cr(eps)))
+
+    def total_var_grad3d(self, ambdala, smat, nx, ny, nz, normalizeq, eps=1e-9):
+        vol = self._a(smat).reshape(nx, ny, nz)
+        g = np.zeros((nx, ny, nz), self.dt)
+        bobo = self._fn("ora_total_var_grad3d", C.c_double)(
+            self.cr(ambdala), _ptr(g), _ptr(vol), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+            C.c_int(normalizeq), self.cr(eps))
+        return g, float(bobo)
+
+    # ---- f2py-named facades (reference module/function names) ----
+    def modules(self):
+        o = self
+        vp = SimpleNamespace(
+            sinoproj_linear=lambda *a: o.sinoproj(LINEAR, 0, *a),
+            sinoproj_nearest=lambda *a: o.sinoproj(NEAREST, 0, *a),
+            sinobackproject_linear=lambda *a: o.sinobackproject(LINEAR, 0, *a),
+            sinobackproject_nearest=lambda *a: o.sinobackproject(NEAREST, 0, *a),
+            pd_backproj_linear=lambda *a: o.pd_backproj(LINEAR, *a),
+            pd_backproj_nearest=lambda *a: o.pd_backproj(NEAREST, *a),
+            artstep_linear=lambda *a: o.artstep(LINEAR, 0, *a),
+            artstep_nearest=lambda *a: o.artstep(NEAREST, 0, *a),
+            emstep_linear=lambda *a: o.emstep(LINEAR, 0, *a),
+            emstep_nearest=lambda *a: o.emstep(NEAREST, 0, *a),
+        )
+        vpt = SimpleNamespace(
+            sinoprojz=lambda *a: o.sinoproj(LINEAR, 1, *a),
+            artstepz=lambda *a: o.artstep(LINEAR, 1, *a),
+            artstepz2vol=o.artstepz2vol,
+            emstep_linearz=lambda *a: o.emstep(LINEAR, 1, *a),
+        )
+        rp = SimpleNamespace(
+            sinoproj_linear=lambda *a: o.ray_sinoproj(LINEAR, *a),
+            sinoproj_nearest=lambda *a: o.ray_sinoproj(NEAREST, *a),
+            sinobackproject_nearest=lambda *a: o.ray_sinobackproject(NEAREST, 1, *a),
+            artstep_linear=lambda *a: o.ray_artstep(LINEAR, 0, *a),
+            artstep_nearest=lambda *a: o.ray_artstep(NEAREST, 1, *a),
+            emstep_nearest=lambda *a: o.ray_emstep(NEAREST, 1, *a),
+        )
+        tv = SimpleNamespace(
+            total_var3d=lambda lam, s, nx, ny, nz: o.total_var3d(lam, s, nx, ny, nz),
+            total_var_grad3d=lambda lam, s, nx, ny, nz, nq: o.total_var_grad3d(lam, s, nx, ny, nz, nq),
+        )
+        return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv)
+
+
+def restated(prec: str = "f32") -> SimpleNamespace:
+    """Reference-named modules backed by the C restatement."""
+    return _Restated(prec).modules()
+
+
+def restated_raw(prec: str = "f32") -> _Restated:
+    return _Restated(prec)
+
+
+# --------------------------------------------------------------------------
+# compiled reference Fortran (oracle/_ref), called the way f2py calls it
+# --------------------------------------------------------------------------
+class _Ref:
+    def __init__(self):
+        d = os.path.join(HERE, "_ref")
+        self.libs = {m: _load(os.path.join(d, f"lib{m}.so"))
+                     for m in ("view_proj3D", "view_proj3D_tomo", "ray_proj3D", "totalvar")}
+
+    @staticmethod
+    def _F(x, dt=np.float32):
+        return np.asfortranarray(x, dtype=dt)
+
+    @staticmethod
+    def _i(v):
+        return C.byref(C.c_int(int(v)))
+
+    @staticmethod
+    def _r(v):
+        return C.byref(C.c_float(float(v)))
+
+    def _call(self, mod, name, *args):
+        fn = getattr(self.libs[mod], name + "_")
+        fn.restype = None
+        fn(*args)
+
+    def _ind(self, ind, shape):
+        if ind is None:
+            ind = np.ones(shape, np.int32)
+        return self._F(np.asarray(ind).reshape(shape), np.int32)
+
+    # generic view-geometry forward: (sinomat, indsino, fv, ns,nu,nv,du,dv,u0,v0, smat, dx..z0, nx,ny,nz)
+    def _fwd(self, mod, name, indsino, fv, ns, nu, nv, du, dv, u0, v0, smat,
+             dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = self._F(np.zeros((ns, nu, nv)))
+        ind = self._ind(indsino, (ns, nu, nv))
+        f = self._F(fv)
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        i, r = self._i, self._r
+        self._call(mod, name, _ptr(out), _ptr(ind), _ptr(f), i(ns), i(nu), i(nv), r(du), r(dv),
+                   r(u0), r(v0), _ptr(s), r(dx), r(dy), r(dz), r(x0), r(y0), r(z0),
+                   i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(out)
+
+    def _bp(self, mod, name, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+            dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = self._F(np.zeros((nx, ny, nz)))
+        ind = self._ind(indsino, (ns, nu, nv))
+        f = self._F(fv)
+        sm = self._F(np.asarray(sinomat).reshape(ns, nu, nv))
+        i, r = self._i, self._r
+        self._call(mod, name, _ptr(sm), _ptr(ind), _ptr(f), i(ns), i(nu), i(nv), r(du), r(dv),
+                   r(u0), r(v0), _ptr(out), r(dx), r(dy), r(dz), r(x0), r(y0), r(z0),
+                   i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(out)
+
+    def _art(self, mod, name, reg_fac, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+             smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._F(np.asarray(smat).reshape(nx, ny, nz)).copy(order="F")
+        ind = self._ind(indsino, (ns, nu, nv))
+        f = self._F(fv)
+        sm = self._F(np.asarray(sinomat).reshape(ns, nu, nv))
+        i, r = self._i, self._r
+        self._call(mod, name, r(reg_fac), _ptr(sm), _ptr(ind), _ptr(f), i(ns), i(nu), i(nv),
+                   r(du), r(dv), r(u0), r(v0), _ptr(vol), r(dx), r(dy), r(dz), r(x0), r(y0),
+                   r(z0), i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(vol)
+
+    def _em(self, mod, name, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+            smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._F(np.asarray(smat).reshape(nx, ny, nz)).copy(order="F")
+        ind = self._ind(indsino, (ns, nu, nv))
+        f = self._F(fv)
+        sm = self._F(np.asarray(sinomat).reshape(ns, nu, nv))
+        i, r = self._i, self._r
+        self._call(mod, name, _ptr(sm), _ptr(ind), _ptr(f), i(ns), i(nu), i(nv),
+                   r(du), r(dv), r(u0), r(v0), _ptr(vol), r(dx), r(dy), r(dz), r(x0), r(y0),
+                   r(z0), i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(vol)
+
+    def _artz2(self, reg_fac, sinomat, indsino, fv, ns, nu, nv, du, dv, u0, v0,
+               smat, dx, dy, dz, x0, y0, z0, nx, ny, nz,
+               smata, dxa, dya, dza, x0a, y0a, z0a, nxa, nya, nza):
+        vol = self._F(np.asarray(smat).reshape(nx, ny, nz)).copy(order="F")
+        vola = self._F(np.asarray(smata).reshape(nxa, nya, nza)).copy(order="F")
+        ind = self._ind(indsino, (ns, nu, nv))
+        f = self._F(fv)
+        sm = self._F(np.asarray(sinomat).reshape(ns, nu, nv))
+        i, r = self._i, self._r
+        self._call("view_proj3D_tomo", "artstepz2vol", r(reg_fac), _ptr(sm), _ptr(ind), _ptr(f),
+                   i(ns), i(nu), i(nv), r(du), r(dv), r(u0), r(v0),
+                   _ptr(vol), r(dx), r(dy), r(dz), r(x0), r(y0), r(z0), i(nx), i(ny), i(nz),
+                   _ptr(vola), r(dxa), r(dya), r(dza), r(x0a), r(y0a), r(z0a),
+                   i(nxa), i(nya), i(nza))
+        return np.ascontiguousarray(vol), np.ascontiguousarray(vola)
+
+    # ray lists: (sinomat, indsino, ray_vectors, nr, smat, dx..z0, nx,ny,nz)
+    def _rfwd(self, name, indsino, rv, nr, smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nr,), np.float32)
+        ind = self._ind(indsino, (nr,))
+        f = self._F(rv)
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        i, r = self._i, self._r
+        self._call("ray_proj3D", name, _ptr(out), _ptr(ind), _ptr(f), i(nr), _ptr(s),
+                   r(dx), r(dy), r(dz), r(x0), r(y0), r(z0), i(nx), i(ny), i(nz))
+        return out
+
+    def _rbp(self, name, sinomat, indsino, rv, nr, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        out = self._F(np.zeros((nx, ny, nz)))
+        ind = self._ind(indsino, (nr,))
+        f = self._F(rv)
+        sm = np.ascontiguousarray(sinomat, np.float32).reshape(nr)
+        i, r = self._i, self._r
+        self._call("ray_proj3D", name, _ptr(sm), _ptr(ind), _ptr(f), i(nr), _ptr(out),
+                   r(dx), r(dy), r(dz), r(x0), r(y0), r(z0), i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(out)
+
+    def _rart(self, name, reg_fac, sinomat, indsino, rv, nr, smat, dx, dy, dz, x0, y0, z0,
+              nx, ny, nz):
+        vol = self._F(np.asarray(smat).reshape(nx, ny, nz)).copy(order="F")
+        ind = self._ind(indsino, (nr,))
+        f = self._F(rv)
+        sm = np.ascontiguousarray(sinomat, np.float32).reshape(nr)
+        i, r = self._i, self._r
+        self._call("ray_proj3D", name, r(reg_fac), _ptr(sm), _ptr(ind), _ptr(f), i(nr), _ptr(vol),
+                   r(dx), r(dy), r(dz), r(x0), r(y0), r(z0), i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(vol)
+
+    def _rem(self, name, sinomat, indsino, rv, nr, smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        vol = self._F(np.asarray(smat).reshape(nx, ny, nz)).copy(order="F")
+        ind = self._ind(indsino, (nr,))
+        f = self._F(rv)
+        sm = np.ascontiguousarray(sinomat, np.float32).reshape(nr)
+        i, r = self._i, self._r
+        self._call("ray_proj3D", name, _ptr(sm), _ptr(ind), _ptr(f), i(nr), _ptr(vol),
+                   r(dx), r(dy), r(dz), r(x0), r(y0), r(z0), i(nx), i(ny), i(nz))
+        return np.ascontiguousarray(vol)
+
+    def _tv(self, ambdala, smat, nx, ny, nz):
+        fn = self.libs["totalvar"].total_var3d_
+        fn.restype = C.c_float
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        return float(fn(self._r(ambdala), _ptr(s), self._i(nx), self._i(ny), self._i(nz)))
+
+    def _tvg(self, ambdala, smat, nx, ny, nz, normalizeq):
+        g = self._F(np.zeros((nx, ny, nz)))
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        bobo = C.c_float(0.0)
+        fn = self.libs["totalvar"].total_var_grad3d_
+        fn.restype = None
+        fn(self._r(ambdala), _ptr(g), _ptr(s), self._i(nx), self._i(ny), self._i(nz),
+           self._i(normalizeq), C.byref(bobo))
+        return np.ascontiguousarray(g), float(bobo.value)
+
+    def modules(self):
+        o = self
+        vp = SimpleNamespace(**{
+            n: (lambda n: (lambda *a: o._fwd("view_proj3D", n, *a)))(n)
+            for n in ("sinoproj_linear", "sinoproj_nearest")})
+        for n in ("sinobackproject_linear", "sinobackproject_nearest",
+                  "pd_backproj_linear", "pd_backproj_nearest"):
+            setattr(vp, n, (lambda n: (lambda *a: o._bp("view_proj3D", n, *a)))(n))
+        for n in ("artstep_linear", "artstep_nearest"):
+            setattr(vp, n, (lambda n: (lambda *a: o._art("view_proj3D", n, *a)))(n))
+        for n in ("emstep_linear", "emstep_nearest"):
+            setattr(vp, n, (lambda n: (lambda *a: o._em("view_proj3D", n, *a)))(n))
+        vpt = SimpleNamespace(
+            sinoprojz=lambda *a: o._fwd("view_proj3D_tomo", "sinoprojz", *a),
+            artstepz=lambda *a: o._art("view_proj3D_tomo", "artstepz", *a),
+            artstepz2vol=o._artz2,
+            emstep_linearz=lambda *a: o._em("view_proj3D_tomo", "emstep_linearz", *a),
+        )
+        rp = SimpleNamespace(
+            sinoproj_linear=lambda *a: o._rfwd("sinoproj_linear", *a),
+            sinoproj_nearest=lambda *a: o._rfwd("sinoproj_nearest", *a),
+            sinobackproject_nearest=lambda *a: o._rbp("sinobackproject_nearest", *a),
+            artstep_linear=lambda *a: o._rart("artstep_linear", *a),
+            artstep_nearest=lambda *a: o._rart("artstep_nearest", *a),
+            emstep_nearest=lambda *a: o._rem("emstep_nearest", *a),
+        )
+        tv = SimpleNamespace(total_var3d=o._tv, total_var_grad3d=o._tvg)
+        return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv)
+
+
+def reference() -> SimpleNamespace:
+    """Reference-named modules backed by the compiled reference Fortran."""
+    return _Ref().modules()

@@ -1,0 +1,653 @@
+/*
+ * ora_body.h -- body of the CPU oracle, included twice by ct2_oracle.c:
+ * once with REAL=float (Fortran-faithful FP32: the reference's own loop order
+ * and operation order, no FMA contraction) and once with REAL=double (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY. This restatement is the checker for the HIP
+ * kernels; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it. The product path never calls it.
+ *
+ * Array conventions (logical indices equal the reference's 1-based Fortran
+ * indices minus one; f2py hands the Fortran an F-ordered copy of the same
+ * logical array):
+ *   vol  [nx][ny][nz]   C order, element (i,j,k) at (i*ny+j)*nz+k
+ *   sino [ns][nu][nv]   C order, element (p,j,k) at (p*nu+j)*nv+k
+ *   ind  [ns][nu][nv]   int8, a ray is used iff ind==1
+ *   fv   [ns][18]       frame vectors S, tangent, D, e_u, e_v, e_w
+ *   rv   [nr][6]        ray list: source (3), bin (3)
+ */
+
+#define R(c) ((REAL)(c))
+
+typedef struct {
+    REAL s[3];      /* source */
+    REAL b[3];      /* detector bin */
+} FN(ray_t);
+
+typedef struct {
+    REAL d[3], o[3];
+    int n[3];
+} FN(grid_t);
+
+/* Per-ray Joseph setup: driving axis choice and slopes.
+ * view_proj3D.f:802-815 (x), :843-847 (y), :873-876 (z);
+ * z-forced variant view_proj3D_tomo.f:111-113.  Axis x wins only when
+ * strictly larger than both others; y when strictly larger than z; else z. */
+typedef struct {
+    int a, b, c;    /* driving axis and the two transverse axes */
+    REAL r1, r2;    /* slopes d(b)/d(a), d(c)/d(a) */
+    REAL trav;      /* d_a * sqrt(1 + r1^2 + r2^2) */
+} FN(setup_t);
+
+static void FN(ray_setup)(const FN(ray_t) *ry, const FN(grid_t) *g, int zonly,
+                          FN(setup_t) *st)
+{
+    REAL diff[3];
+    diff[0] = ry->b[0] - ry->s[0];
+    diff[1] = ry->b[1] - ry->s[1];
+    diff[2] = ry->b[2] - ry->s[2];
+    int a;
+    if (zonly) {
+        a = 2;
+    } else {
+        REAL xad = ABS(diff[0]) / g->d[0];
+        REAL yad = ABS(diff[1]) / g->d[1];
+        REAL zad = ABS(diff[2]) / g->d[2];
+        if ((xad > yad) && (xad > zad)) a = 0;
+        else if (yad > zad) a = 1;
+        else a = 2;
+    }
+    int b = (a == 0) ? 1 : 0;
+    int c = (a == 2) ? 1 : 2;
+    st->a = a; st->b = b; st->c = c;
+    st->r1 = diff[b] / diff[a];
+    st->r2 = diff[c] / diff[a];
+    st->trav = g->d[a] * SQRT(R(1.0) + st->r1 * st->r1 + st->r2 * st->r2);
+}
+
+/* Sample of a ray on plane l of its driving axis.
+ * Linear (view_proj3D.f:816-826): cell index floor((p-o)/d - 0.5), valid
+ * 0 <= i < n-1, weights w2 = (p - (d*(i+0.5)+o))/d, w1 = 1-w2.
+ * Nearest (view_proj3D.f:1028-1035): index floor((p-o)/d), valid
+ * 0 <= i <= n-1 (or < n-1 for the ray-list bp/ART/EM, ray_proj3D.f:895). */
+typedef struct {
+    int ib, ic;
+    REAL wb1, wb2, wc1, wc2;
+} FN(sample_t);
+
+static int FN(sample_linear)(const FN(ray_t) *ry, const FN(grid_t) *g,
+                             const FN(setup_t) *st, int l, FN(sample_t) *sm)
+{
+    int a = st->a, b = st->b, c = st->c;
+    REAL t = g->o[a] + g->d[a] * ((REAL)l + R(0.5));
+    REAL pb = ry->s[b] + st->r1 * (t - ry->s[a]);
+    REAL pc = ry->s[c] + st->r2 * (t - ry->s[a]);
+    int ib = (int)FLOOR((pb - g->o[b]) / g->d[b] - R(0.5));
+    int ic = (int)FLOOR((pc - g->o[c]) / g->d[c] - R(0.5));
+    if (!((ib >= 0) && (ib < g->n[b] - 1) && (ic >= 0) && (ic < g->n[c] - 1)))
+        return 0;
+    sm->ib = ib; sm->ic = ic;
+    sm->wb2 = (pb - (g->d[b] * ((REAL)ib + R(0.5)) + g->o[b])) / g->d[b];
+    sm->wb1 = R(1.0) - sm->wb2;
+    sm->wc2 = (pc - (g->d[c] * ((REAL)ic + R(0.5)) + g->o[c])) / g->d[c];
+    sm->wc1 = R(1.0) - sm->wc2;
+    return 1;
+}
+
+static int FN(sample_nearest)(const FN(ray_t) *ry, const FN(grid_t) *g,
+                              const FN(setup_t) *st, int l, int strict,
+                              FN(sample_t) *sm)
+{
+    int a = st->a, b = st->b, c = st->c;
+    REAL t = g->o[a] + g->d[a] * ((REAL)l + R(0.5));
+    REAL pb = ry->s[b] + st->r1 * (t - ry->s[a]);
+    REAL pc = ry->s[c] + st->r2 * (t - ry->s[a]);
+    int ib = (int)FLOOR((pb - g->o[b]) / g->d[b]);
+    int ic = (int)FLOOR((pc - g->o[c]) / g->d[c]);
+    int hb = strict ? g->n[b] - 2 : g->n[b] - 1;
+    int hc = strict ? g->n[c] - 2 : g->n[c] - 1;
+    if (!((ib >= 0) && (ib <= hb) && (ic >= 0) && (ic <= hc)))
+        return 0;
+    sm->ib = ib; sm->ic = ic;
+    return 1;
+}
+
+static inline long FN(vidx)(const FN(grid_t) *g, int a, int b, int c,
+                            int l, int ib, int ic)
+{
+    int q[3];
+    q[a] = l; q[b] = ib; q[c] = ic;
+    return ((long)q[0] * g->n[1] + q[1]) * g->n[2] + q[2];
+}
+
+/* Bin position of view ray (p,j,k): view_proj3D.f:784-797. */
+static void FN(view_ray)(const float *fv, int p, int j, int k,
+                         REAL du, REAL dv, REAL u0, REAL v0, FN(ray_t) *ry)
+{
+    const float *f = fv + (long)p * 18;
+    REAL u = u0 + ((REAL)j + R(0.5)) * du;
+    REAL v = v0 + ((REAL)k + R(0.5)) * dv;
+    for (int q = 0; q < 3; ++q) {
+        ry->s[q] = (REAL)f[q];
+        ry->b[q] = (REAL)f[6 + q] + (REAL)f[9 + q] * u + (REAL)f[12 + q] * v;
+    }
+}
+
+static void FN(list_ray)(const float *rv, int r, FN(ray_t) *ry)
+{
+    for (int q = 0; q < 3; ++q) {
+        ry->s[q] = (REAL)rv[(long)r * 6 + q];
+        ry->b[q] = (REAL)rv[(long)r * 6 + 3 + q];
+    }
+}
+
+/* Weighted corner list of one ray, in the reference's order: per plane the
+ * corners (b,c) = (0,0),(1,0),(0,1),(1,1) with weight trav*wb*wc
+ * (view_proj3D.f:1228-1251); nearest: one entry of weight trav. */
+typedef struct {
+    long n, cap;
+    long *idx;
+    REAL *w;
+} FN(list_t);
+
+static void FN(list_push)(FN(list_t) *L, long idx, REAL w)
+{
+    if (L->n == L->cap) {
+        L->cap = L->cap ? 2 * L->cap : 1024;
+        L->idx = (long *)realloc(L->idx, L->cap * sizeof(long));
+        L->w = (REAL *)realloc(L->w, L->cap * sizeof(REAL));
+    }
+    L->idx[L->n] = idx;
+    L->w[L->n] = w;
+    L->n++;
+}
+
+/* March one ray. Returns the ray sum of vol (if vol != NULL), fills the
+ * corner list (if L != NULL) and the ART normaliser sum of squared weights
+ * (*tot; linear: sum of raypix^2 per corner, view_proj3D.f:2086-2115;
+ * nearest: sum of trav^2, view_proj3D.f:2407). */
+static REAL FN(march)(const FN(ray_t) *ry, const FN(grid_t) *g, int zonly,
+                      int interp, int strict, const REAL *vol,
+                      FN(list_t) *L, REAL *tot)
+{
+    FN(setup_t) st;
+    FN(ray_setup)(ry, g, zonly, &st);
+    REAL total = R(0.0), sq = R(0.0);
+    int a = st.a, b = st.b, c = st.c;
+    for (int l = 0; l < g->n[a]; ++l) {
+        FN(sample_t) sm;
+        if (interp == ORA_LINEAR) {
+            if (!FN(sample_linear)(ry, g, &st, l, &sm)) continue;
+            long i00 = FN(vidx)(g, a, b, c, l, sm.ib, sm.ic);
+            long i10 = FN(vidx)(g, a, b, c, l, sm.ib + 1, sm.ic);
+            long i01 = FN(vidx)(g, a, b, c, l, sm.ib, sm.ic + 1);
+            long i11 = FN(vidx)(g, a, b, c, l, sm.ib + 1, sm.ic + 1);
+            if (vol) {
+                REAL vi1 = sm.wb1 * vol[i00] + sm.wb2 * vol[i10];
+                REAL vi2 = sm.wb1 * vol[i01] + sm.wb2 * vol[i11];
+                REAL val = sm.wc1 * vi1 + sm.wc2 * vi2;
+                total = total + st.trav * val;
+            }
+            if (L) {
+                REAL w;
+                w = st.trav * sm.wb1 * sm.wc1; FN(list_push)(L, i00, w); sq = sq + w * w;
+                w = st.trav * sm.wb2 * sm.wc1; FN(list_push)(L, i10, w); sq = sq + w * w;
+                w = st.trav * sm.wb1 * sm.wc2; FN(list_push)(L, i01, w); sq = sq + w * w;
+                w = st.trav * sm.wb2 * sm.wc2; FN(list_push)(L, i11, w); sq = sq + w * w;
+            }
+        } else {
+            if (!FN(sample_nearest)(ry, g, &st, l, strict, &sm)) continue;
+            long i00 = FN(vidx)(g, a, b, c, l, sm.ib, sm.ic);
+            if (vol) total = total + st.trav * vol[i00];
+            if (L) { FN(list_push)(L, i00, st.trav); sq = sq + st.trav * st.trav; }
+        }
+    }
+    if (tot) *tot = sq;
+    return total;
+}
+
+static void FN(grid_init)(FN(grid_t) *g, REAL dx, REAL dy, REAL dz,
+                          REAL x0, REAL y0, REAL z0, int nx, int ny, int nz)
+{
+    g->d[0] = dx; g->d[1] = dy; g->d[2] = dz;
+    g->o[0] = x0; g->o[1] = y0; g->o[2] = z0;
+    g->n[0] = nx; g->n[1] = ny; g->n[2] = nz;
+}
+
+/* ---- views: forward (sinoproj_linear view_proj3D.f:703-910,
+ *      sinoproj_nearest :915-1092, sinoprojz view_proj3D_tomo.f:4-148) ---- */
+EXPORT void FN(ora_sinoproj)(int interp, int zonly, REAL *sino, const int8_t *ind,
+                             const float *fv, int ns, int nu, int nv,
+                             REAL du, REAL dv, REAL u0, REAL v0, const REAL *vol,
+                             REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                             int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    long nray = (long)ns * nu * nv;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long r = 0; r < nray; ++r) {
+        int p = (int)(r / ((long)nu * nv));
+        int j = (int)((r / nv) % nu);
+        int k = (int)(r % nv);
+        if (ind && ind[r] != 1) { sino[r] = R(0.0); continue; }
+        FN(ray_t) ry;
+        FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+        sino[r] = FN(march)(&ry, &g, zonly, interp, 0, vol, NULL, NULL);
+    }
+}
+
+/* ---- views: exact-adjoint backprojection (sinobackproject_linear
+ *      view_proj3D.f:1096-1370, _nearest :1374-1595). Output zero-filled
+ *      first (f2py intent(out) zero-fill for the linear one, explicit loop
+ *      view_proj3D.f:1411-1417 for the nearest one). Scatter order = ray
+ *      order (p, j, k), corners in list order. zonly gives the transpose of
+ *      sinoprojz (no reference counterpart). ---- */
+EXPORT void FN(ora_sinobackproject)(int interp, int zonly, REAL *vol, const REAL *sino,
+                                    const int8_t *ind, const float *fv,
+                                    int ns, int nu, int nv, REAL du, REAL dv,
+                                    REAL u0, REAL v0,
+                                    REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                                    int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    memset(vol, 0, sizeof(REAL) * (size_t)nx * ny * nz);
+    FN(list_t) L = {0, 0, NULL, NULL};
+    long nray = (long)ns * nu * nv;
+    for (long r = 0; r < nray; ++r) {
+        if (ind && ind[r] != 1) continue;
+        int p = (int)(r / ((long)nu * nv));
+        int j = (int)((r / nv) % nu);
+        int k = (int)(r % nv);
+        FN(ray_t) ry;
+        FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+        L.n = 0;
+        FN(march)(&ry, &g, zonly, interp, 0, NULL, &L, NULL);
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e)
+            vol[L.idx[e]] = vol[L.idx[e]] + dataval * L.w[e];
+    }
+    free(L.idx); free(L.w);
+}
+
+/* ---- views: voxel-driven backprojection, not an adjoint
+ *      (pd_backproj_linear view_proj3D.f:1769-1944,
+ *       pd_backproj_nearest :1598-1766) ---- */
+EXPORT void FN(ora_pd_backproj)(int interp, REAL *vol, const REAL *sino,
+                                const float *fv, int ns, int nu, int nv,
+                                REAL du, REAL dv, REAL u0, REAL v0,
+                                REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                                int nx, int ny, int nz)
+{
+    REAL *pv = (REAL *)malloc(sizeof(REAL) * 16 * (size_t)(ns > 0 ? ns : 1));
+    /* per-view: S(3) D(3) eu(3) ev(3) ew(3) sd_dist (view_proj3D.f:1699-1714) */
+    for (int p = 0; p < ns; ++p) {
+        const float *f = fv + (long)p * 18;
+        REAL S[3] = {f[0], f[1], f[2]}, D[3] = {f[6], f[7], f[8]};
+        REAL eu[3] = {f[9], f[10], f[11]}, ev[3] = {f[12], f[13], f[14]};
+        REAL ew[3] = {f[15], f[16], f[17]};
+        REAL udr = (D[0] - S[0]) * eu[0] + (D[1] - S[1]) * eu[1] + (D[2] - S[2]) * eu[2];
+        REAL vdr = (D[0] - S[0]) * ev[0] + (D[1] - S[1]) * ev[1] + (D[2] - S[2]) * ev[2];
+        REAL tc[3];
+        for (int q = 0; q < 3; ++q) tc[q] = D[q] - eu[q] * udr - ev[q] * vdr;
+        REAL sdd = SQRT((S[0] - tc[0]) * (S[0] - tc[0]) + (S[1] - tc[1]) * (S[1] - tc[1])
+                        + (S[2] - tc[2]) * (S[2] - tc[2]));
+        REAL *o = pv + 16 * p;
+        for (int q = 0; q < 3; ++q) {
+            o[q] = S[q]; o[3 + q] = D[q]; o[6 + q] = eu[q]; o[9 + q] = ev[q]; o[12 + q] = ew[q];
+        }
+        o[15] = sdd;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int ix = 0; ix < nx; ++ix) {
+        for (int iy = 0; iy < ny; ++iy) {
+            REAL x = x0 + ((REAL)ix + R(0.5)) * dx;
+            REAL y = y0 + ((REAL)iy + R(0.5)) * dy;
+            for (int iz = 0; iz < nz; ++iz) {
+                REAL z = z0 + ((REAL)iz + R(0.5)) * dz;
+                REAL total = R(0.0);
+                for (int p = 0; p < ns; ++p) {
+                    const REAL *o = pv + 16 * p;
+                    REAL xd = x - o[0], yd = y - o[1], zd = z - o[2];
+                    REAL zoom = ABS((xd * o[12] + yd * o[13] + zd * o[14]) / o[15]);
+                    REAL xb = xd / zoom + o[0], yb = yd / zoom + o[1], zb = zd / zoom + o[2];
+                    REAL xdt = xb - o[3], ydt = yb - o[4], zdt = zb - o[5];
+                    REAL u = (xdt * o[6]) + (ydt * o[7]) + (zdt * o[8]);
+                    REAL v = (xdt * o[9]) + (ydt * o[10]) + (zdt * o[11]);
+                    const REAL *sp = sino + (long)p * nu * nv;
+                    if (interp == ORA_LINEAR) {
+                        /* iu is the 1-based left bin, valid 1 <= iu < nu */
+                        int iu = (int)FLOOR((u - u0) / du + R(0.5));
+                        int iv = (int)FLOOR((v - v0) / dv + R(0.5));
+                        if ((iu >= 1) && (iu < nu) && (iv >= 1) && (iv < nv)) {
+                            REAL wu2 = (u - u0) / du - ((REAL)iu - R(0.5));
+                            REAL wu1 = R(1.0) - wu2;
+                            REAL wv2 = (v - v0) / dv - ((REAL)iv - R(0.5));
+                            REAL wv1 = R(1.0) - wv2;
+                            long a00 = (long)(iu - 1) * nv + (iv - 1);
+                            REAL vi1 = wu1 * sp[a00] + wu2 * sp[a00 + nv];
+                            REAL vi2 = wu1 * sp[a00 + 1] + wu2 * sp[a00 + nv + 1];
+                            REAL val = wv1 * vi1 + wv2 * vi2;
+                            total = total + val;
+                        }
+                    } else {
+                        int iu = (int)FLOOR((u - u0) / du);
+                        int iv = (int)FLOOR((v - v0) / dv);
+                        if ((iu >= 0) && (iu <= nu - 1) && (iv >= 0) && (iv <= nv - 1))
+                            total = total + sp[(long)iu * nv + iv];
+                    }
+                }
+                vol[((long)ix * ny + iy) * nz + iz] = total;
+            }
+        }
+    }
+    free(pv);
+}
+
+/* ---- views: sequential Kaczmarz ART (artstep_linear view_proj3D.f:1950-2271,
+ *      artstep_nearest :2275-2503, artstepz view_proj3D_tomo.f:154-347).
+ *      x += reg_fac*(b - a.x)*w/total, ray after ray in (p,j,k) order. ---- */
+EXPORT void FN(ora_artstep)(int interp, int zonly, REAL reg_fac, const REAL *sino,
+                            const int8_t *ind, const float *fv, int ns, int nu, int nv,
+                            REAL du, REAL dv, REAL u0, REAL v0, REAL *vol,
+                            REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                            int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    FN(list_t) L = {0, 0, NULL, NULL};
+    long nray = (long)ns * nu * nv;
+    for (long r = 0; r < nray; ++r) {
+        if (ind && ind[r] != 1) continue;
+        int p = (int)(r / ((long)nu * nv));
+        int j = (int)((r / nv) % nu);
+        int k = (int)(r % nv);
+        FN(ray_t) ry;
+        FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+        L.n = 0;
+        REAL total;
+        REAL raysum = FN(march)(&ry, &g, zonly, interp, 0, vol, &L, &total);
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e)
+            vol[L.idx[e]] = vol[L.idx[e]] + reg_fac * (dataval - raysum) * L.w[e] / total;
+    }
+    free(L.idx); free(L.w);
+}
+
+/* ---- views: two-volume z-driven ART (artstepz2vol view_proj3D_tomo.f:352-612):
+ *      one raysum / total over the main then the auxiliary grid, then both
+ *      grids updated with the same residual. ---- */
+EXPORT void FN(ora_artstepz2vol)(REAL reg_fac, const REAL *sino, const int8_t *ind,
+                                 const float *fv, int ns, int nu, int nv,
+                                 REAL du, REAL dv, REAL u0, REAL v0,
+                                 REAL *vol, REAL dx, REAL dy, REAL dz,
+                                 REAL x0, REAL y0, REAL z0, int nx, int ny, int nz,
+                                 REAL *vola, REAL dxa, REAL dya, REAL dza,
+                                 REAL x0a, REAL y0a, REAL z0a, int nxa, int nya, int nza)
+{
+    FN(grid_t) g, ga;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    FN(grid_init)(&ga, dxa, dya, dza, x0a, y0a, z0a, nxa, nya, nza);
+    FN(list_t) L = {0, 0, NULL, NULL}, La = {0, 0, NULL, NULL};
+    long nray = (long)ns * nu * nv;
+    for (long r = 0; r < nray; ++r) {
+        if (ind && ind[r] != 1) continue;
+        int p = (int)(r / ((long)nu * nv));
+        int j = (int)((r / nv) % nu);
+        int k = (int)(r % nv);
+        FN(ray_t) ry;
+        FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+        /* The reference accumulates raysum and total across both grids in
+         * one running sum: main grid first, auxiliary second. */
+        FN(setup_t) st, sta;
+        FN(ray_setup)(&ry, &g, 1, &st);
+        FN(ray_setup)(&ry, &ga, 1, &sta);
+        REAL raysum = R(0.0), total = R(0.0);
+        L.n = 0; La.n = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            const FN(grid_t) *gg = pass ? &ga : &g;
+            const FN(setup_t) *ss = pass ? &sta : &st;
+            const REAL *vv = pass ? vola : vol;
+            FN(list_t) *LL = pass ? &La : &L;
+            for (int l = 0; l < gg->n[2]; ++l) {
+                FN(sample_t) sm;
+                if (!FN(sample_linear)(&ry, gg, ss, l, &sm)) continue;
+                long i00 = FN(vidx)(gg, 2, 0, 1, l, sm.ib, sm.ic);
+                long i10 = FN(vidx)(gg, 2, 0, 1, l, sm.ib + 1, sm.ic);
+                long i01 = FN(vidx)(gg, 2, 0, 1, l, sm.ib, sm.ic + 1);
+                long i11 = FN(vidx)(gg, 2, 0, 1, l, sm.ib + 1, sm.ic + 1);
+                REAL vi1 = sm.wb1 * vv[i00] + sm.wb2 * vv[i10];
+                REAL vi2 = sm.wb1 * vv[i01] + sm.wb2 * vv[i11];
+                REAL val = sm.wc1 * vi1 + sm.wc2 * vi2;
+                raysum = raysum + ss->trav * val;
+                REAL w;
+                w = ss->trav * sm.wb1 * sm.wc1; FN(list_push)(LL, i00, w); total = total + w * w;
+                w = ss->trav * sm.wb2 * sm.wc1; FN(list_push)(LL, i10, w); total = total + w * w;
+                w = ss->trav * sm.wb1 * sm.wc2; FN(list_push)(LL, i01, w); total = total + w * w;
+                w = ss->trav * sm.wb2 * sm.wc2; FN(list_push)(LL, i11, w); total = total + w * w;
+            }
+        }
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e)
+            vol[L.idx[e]] = vol[L.idx[e]] + reg_fac * (dataval - raysum) * L.w[e] / total;
+        for (long e = 0; e < La.n; ++e)
+            vola[La.idx[e]] = vola[La.idx[e]] + reg_fac * (dataval - raysum) * La.w[e] / total;
+    }
+    free(L.idx); free(L.w); free(La.idx); free(La.w);
+}
+
+/* ---- views: MLEM (emstep_linear view_proj3D.f:2507-2831,
+ *      emstep_nearest :2834-3075, emstep_linearz view_proj3D_tomo.f:615-822).
+ *      raysum < small -> 1; W += (b/raysum)*w; O += w; x = x*W/O with
+ *      O < small -> 1.  small = 1e-5 (linear), 1e-8 (nearest). ---- */
+EXPORT void FN(ora_emstep)(int interp, int zonly, const REAL *sino, const int8_t *ind,
+                           const float *fv, int ns, int nu, int nv,
+                           REAL du, REAL dv, REAL u0, REAL v0, REAL *vol,
+                           REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                           int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    size_t nvox = (size_t)nx * ny * nz;
+    REAL *ws = (REAL *)calloc(nvox, sizeof(REAL));
+    REAL *one = (REAL *)calloc(nvox, sizeof(REAL));
+    REAL small = (interp == ORA_LINEAR) ? (REAL)0.00001f : (REAL)0.00000001f;
+    FN(list_t) L = {0, 0, NULL, NULL};
+    long nray = (long)ns * nu * nv;
+    for (long r = 0; r < nray; ++r) {
+        if (ind && ind[r] != 1) continue;
+        int p = (int)(r / ((long)nu * nv));
+        int j = (int)((r / nv) % nu);
+        int k = (int)(r % nv);
+        FN(ray_t) ry;
+        FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+        L.n = 0;
+        REAL raysum = FN(march)(&ry, &g, zonly, interp, 0, vol, &L, NULL);
+        if (raysum < small) raysum = R(1.0);
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e) {
+            ws[L.idx[e]] = ws[L.idx[e]] + (dataval / raysum) * L.w[e];
+            one[L.idx[e]] = one[L.idx[e]] + L.w[e];
+        }
+    }
+    for (size_t q = 0; q < nvox; ++q) {
+        if (one[q] < small) one[q] = R(1.0);
+        vol[q] = vol[q] * ws[q] / one[q];
+    }
+    free(ws); free(one); free(L.idx); free(L.w);
+}
+
+/* ---- ray list ("linogram"): ray_proj3D.f.  Forward linear :514-671,
+ *      forward nearest :677-804 (bounds <= n-1); backprojection nearest
+ *      :809-980, ART linear :987-1267, ART nearest :1272-1449, EM nearest
+ *      :1454-1651 -- the nearest ones there use bounds < n-1. ---- */
+EXPORT void FN(ora_ray_sinoproj)(int interp, REAL *sino, const int8_t *ind,
+                                 const float *rv, int nr, const REAL *vol,
+                                 REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                                 int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int r = 0; r < nr; ++r) {
+        if (ind && ind[r] != 1) { sino[r] = R(0.0); continue; }
+        FN(ray_t) ry;
+        FN(list_ray)(rv, r, &ry);
+        sino[r] = FN(march)(&ry, &g, 0, interp, 0, vol, NULL, NULL);
+    }
+}
+
+EXPORT void FN(ora_ray_sinobackproject)(int interp, int strict, REAL *vol, const REAL *sino,
+                                        const int8_t *ind, const float *rv, int nr,
+                                        REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                                        int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    memset(vol, 0, sizeof(REAL) * (size_t)nx * ny * nz);
+    FN(list_t) L = {0, 0, NULL, NULL};
+    for (int r = 0; r < nr; ++r) {
+        if (ind && ind[r] != 1) continue;
+        FN(ray_t) ry;
+        FN(list_ray)(rv, r, &ry);
+        L.n = 0;
+        FN(march)(&ry, &g, 0, interp, strict, NULL, &L, NULL);
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e)
+            vol[L.idx[e]] = vol[L.idx[e]] + dataval * L.w[e];
+    }
+    free(L.idx); free(L.w);
+}
+
+EXPORT void FN(ora_ray_artstep)(int interp, int strict, REAL reg_fac, const REAL *sino,
+                                const int8_t *ind, const float *rv, int nr, REAL *vol,
+                                REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                                int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    FN(list_t) L = {0, 0, NULL, NULL};
+    for (int r = 0; r < nr; ++r) {
+        if (ind && ind[r] != 1) continue;
+        FN(ray_t) ry;
+        FN(list_ray)(rv, r, &ry);
+        L.n = 0;
+        REAL total;
+        REAL raysum = FN(march)(&ry, &g, 0, interp, strict, vol, &L, &total);
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e)
+            vol[L.idx[e]] = vol[L.idx[e]] + reg_fac * (dataval - raysum) * L.w[e] / total;
+    }
+    free(L.idx); free(L.w);
+}
+
+EXPORT void FN(ora_ray_emstep)(int interp, int strict, const REAL *sino, const int8_t *ind,
+                               const float *rv, int nr, REAL *vol,
+                               REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                               int nx, int ny, int nz)
+{
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    size_t nvox = (size_t)nx * ny * nz;
+    REAL *ws = (REAL *)calloc(nvox, sizeof(REAL));
+    REAL *one = (REAL *)calloc(nvox, sizeof(REAL));
+    REAL small = (interp == ORA_LINEAR) ? (REAL)0.00001f : (REAL)0.00000001f;
+    FN(list_t) L = {0, 0, NULL, NULL};
+    for (int r = 0; r < nr; ++r) {
+        if (ind && ind[r] != 1) continue;
+        FN(ray_t) ry;
+        FN(list_ray)(rv, r, &ry);
+        L.n = 0;
+        REAL raysum = FN(march)(&ry, &g, 0, interp, strict, vol, &L, NULL);
+        if (raysum < small) raysum = R(1.0);
+        REAL dataval = sino[r];
+        for (long e = 0; e < L.n; ++e) {
+            ws[L.idx[e]] = ws[L.idx[e]] + (dataval / raysum) * L.w[e];
+            one[L.idx[e]] = one[L.idx[e]] + L.w[e];
+        }
+    }
+    for (size_t q = 0; q < nvox; ++q) {
+        if (one[q] < small) one[q] = R(1.0);
+        vol[q] = vol[q] * ws[q] / one[q];
+    }
+    free(ws); free(one); free(L.idx); free(L.w);
+}
+
+/* ---- TV (totalvar.f).  total_var3d :35-62: Real*8 sum over i,j,k >= 1 of
+ *      sqrt(eps + dx^2 + dy^2 + dz^2) + lambda*|x|, minus the sqrt(eps)
+ *      shift.  total_var_grad3d :476-602: main term + three one-ahead
+ *      terms, Real*8 norm, |norm|<1e-5 -> 1, optional normalisation.
+ *      eps is a parameter (the reference hard-codes 1e-9). ---- */
+EXPORT double FN(ora_total_var3d)(REAL lam, const REAL *s, int nx, int ny, int nz, REAL eps)
+{
+    double tot = 0.0;
+#define V(i, j, k) s[((long)(i) * ny + (j)) * nz + (k)]
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL c = V(i, j, k);
+                REAL ax = c - V(i - 1, j, k), ay = c - V(i, j - 1, k), az = c - V(i, j, k - 1);
+                /* Fortran: (tot + sqrt(..)) + lambda*|x|, Real*4 terms promoted */
+                tot = tot + (double)SQRT(eps + ax * ax + ay * ay + az * az);
+                tot = tot + (double)(lam * ABS(c));
+            }
+    REAL shift = (REAL)(nx - 1) * (REAL)(ny - 1) * (REAL)(nz - 1) * SQRT(eps);
+    return (double)(REAL)(tot - (double)shift);   /* Real*4 function result */
+}
+
+EXPORT double FN(ora_total_var_grad3d)(REAL lam, REAL *g, const REAL *s, int nx, int ny,
+                                       int nz, int normalizeq, REAL eps)
+{
+    size_t nvox = (size_t)nx * ny * nz;
+#define G(i, j, k) g[((long)(i) * ny + (j)) * nz + (k)]
+    memset(g, 0, sizeof(REAL) * nvox);
+    /* main TV term and l1 term (:505-521) */
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w5 = V(i, j, k - 1), w11 = V(i, j - 1, k), w13 = V(i - 1, j, k), w14 = V(i, j, k);
+                REAL term1 = (-w11 - w13 + R(3) * w14 - w5)
+                           / SQRT((w11 - w14) * (w11 - w14) + (w13 - w14) * (w13 - w14)
+                                  + (-w14 + w5) * (-w14 + w5) + eps);
+                REAL term2 = lam * w14 / (eps + ABS(w14));
+                G(i, j, k) = term1 + term2;
+            }
+    /* one-ahead in x (:524-539) */
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w6 = V(i + 1, j, k - 1), w12 = V(i + 1, j - 1, k), w14 = V(i, j, k), w15 = V(i + 1, j, k);
+                REAL t = (w14 - w15) / SQRT((w12 - w15) * (w12 - w15) + (w14 - w15) * (w14 - w15)
+                                            + (-w15 + w6) * (-w15 + w6) + eps);
+                G(i, j, k) = G(i, j, k) + t;
+            }
+    /* one-ahead in y (:542-556) */
+    for (int i = 1; i < nx; ++i)
+        for (int j = 0; j < ny - 1; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w8 = V(i, j + 1, k - 1), w14 = V(i, j, k), w16 = V(i - 1, j + 1, k), w17 = V(i, j + 1, k);
+                REAL t = (w14 - w17) / SQRT((w14 - w17) * (w14 - w17) + (w16 - w17) * (w16 - w17)
+                                            + (-w17 + w8) * (-w17 + w8) + eps);
+                G(i, j, k) = G(i, j, k) + t;
+            }
+    /* one-ahead in z (:559-574) */
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 0; k < nz - 1; ++k) {
+                REAL w14 = V(i, j, k), w20 = V(i, j - 1, k + 1), w22 = V(i - 1, j, k + 1), w23 = V(i, j, k + 1);
+                REAL t = (w14 - w23) / SQRT((w14 - w23) * (w14 - w23) + (w20 - w23) * (w20 - w23)
+                                            + (w22 - w23) * (w22 - w23) + eps);
+                G(i, j, k) = G(i, j, k) + t;
+            }
+#undef V
+#undef G
+    /* Real*8 norm (:576-586); the reference stores it in a Real*4 */
+    double bb = 0.0;
+    for (size_t q = 0; q < nvox; ++q) bb = bb + (double)(g[q] * g[q]);   /* Real*4 square */
+    REAL bobo = (REAL)sqrt(bb);
+    if (ABS(bobo) < R(0.00001)) bobo = R(1.0);
+    if (normalizeq == 1)
+        for (size_t q = 0; q < nvox; ++q) g[q] = g[q] / bobo;
+    return (double)bobo;
+}

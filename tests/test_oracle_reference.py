@@ -1,0 +1,109 @@
+"""Pins the FP32 oracle to the reference Fortran compiled from its own
+sources (oracle/_ref): every function must agree bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+from cases import cone_case, ray_case, tomo_case
+
+pytestmark = pytest.mark.skipif(not oracle.ref_available(),
+                                reason="oracle/_ref not built (reference sources absent)")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    return oracle.reference()
+
+
+@pytest.fixture(scope="module")
+def o32():
+    return oracle.restated("f32")
+
+
+@pytest.mark.parametrize("name", ["sinoproj_linear", "sinoproj_nearest"])
+def test_view_forward(ref, o32, name):
+    ind, fv, geo, vol, g2 = cone_case()
+    a = getattr(ref.view_proj3D, name)(ind, fv, *geo, vol, *g2)
+    b = getattr(o32.view_proj3D, name)(ind, fv, *geo, vol, *g2)
+    assert np.abs(a).max() > 1.0
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["sinobackproject_linear", "sinobackproject_nearest",
+                                  "pd_backproj_linear", "pd_backproj_nearest"])
+def test_view_backprojections(ref, o32, name):
+    ind, fv, geo, vol, g2 = cone_case()
+    sino = o32.view_proj3D.sinoproj_linear(ind, fv, *geo, vol, *g2)
+    a = getattr(ref.view_proj3D, name)(sino, ind, fv, *geo, *g2)
+    b = getattr(o32.view_proj3D, name)(sino, ind, fv, *geo, *g2)
+    assert np.abs(a).max() > 1.0
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["artstep_linear", "artstep_nearest"])
+def test_view_art(ref, o32, name):
+    ind, fv, geo, vol, g2 = cone_case(ns=6)
+    sino = o32.view_proj3D.sinoproj_linear(ind, fv, *geo, vol, *g2)
+    x0 = np.zeros_like(vol)
+    a = getattr(ref.view_proj3D, name)(0.5, sino, ind, fv, *geo, x0, *g2)
+    b = getattr(o32.view_proj3D, name)(0.5, sino, ind, fv, *geo, x0, *g2)
+    assert np.abs(a).max() > 0.1
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["emstep_linear", "emstep_nearest"])
+def test_view_em(ref, o32, name):
+    ind, fv, geo, vol, g2 = cone_case(ns=6)
+    sino = o32.view_proj3D.sinoproj_linear(ind, fv, *geo, vol, *g2)
+    x0 = np.ones_like(vol)
+    a = getattr(ref.view_proj3D, name)(sino, ind, fv, *geo, x0, *g2)
+    b = getattr(o32.view_proj3D, name)(sino, ind, fv, *geo, x0, *g2)
+    assert np.array_equal(a, b)
+
+
+def test_tomo(ref, o32):
+    ind, fv, geo, vol, g2 = tomo_case()
+    a = ref.view_proj3D_tomo.sinoprojz(ind, fv, *geo, vol, *g2)
+    b = o32.view_proj3D_tomo.sinoprojz(ind, fv, *geo, vol, *g2)
+    assert np.abs(a).max() > 1.0 and np.array_equal(a, b)
+    x0 = np.zeros_like(vol)
+    assert np.array_equal(ref.view_proj3D_tomo.artstepz(0.7, b, ind, fv, *geo, x0, *g2),
+                          o32.view_proj3D_tomo.artstepz(0.7, b, ind, fv, *geo, x0, *g2))
+    x1 = np.ones_like(vol)
+    assert np.array_equal(ref.view_proj3D_tomo.emstep_linearz(b, ind, fv, *geo, x1, *g2),
+                          o32.view_proj3D_tomo.emstep_linearz(b, ind, fv, *geo, x1, *g2))
+    rng = np.random.default_rng(3)
+    va = rng.random((20, 16, 4)).astype(np.float32)
+    ga = (1.6, 1.6, 3.0, -16.0, -12.8, 16.0, 20, 16, 4)
+    ra = ref.view_proj3D_tomo.artstepz2vol(0.7, b, ind, fv, *geo, vol, *g2, va, *ga)
+    oa = o32.view_proj3D_tomo.artstepz2vol(0.7, b, ind, fv, *geo, vol, *g2, va, *ga)
+    assert np.array_equal(ra[0], oa[0]) and np.array_equal(ra[1], oa[1])
+
+
+def test_ray_list(ref, o32):
+    ind, rv, nr, vol, g3 = ray_case()
+    for name in ("sinoproj_linear", "sinoproj_nearest"):
+        a = getattr(ref.ray_proj3D, name)(ind, rv, nr, vol, *g3)
+        b = getattr(o32.ray_proj3D, name)(ind, rv, nr, vol, *g3)
+        assert np.abs(a).max() > 1.0 and np.array_equal(a, b), name
+    s = b
+    assert np.array_equal(ref.ray_proj3D.sinobackproject_nearest(s, ind, rv, nr, *g3),
+                          o32.ray_proj3D.sinobackproject_nearest(s, ind, rv, nr, *g3))
+    for name in ("artstep_linear", "artstep_nearest"):
+        a = getattr(ref.ray_proj3D, name)(0.5, s, ind, rv, nr, vol * 0, *g3)
+        b = getattr(o32.ray_proj3D, name)(0.5, s, ind, rv, nr, vol * 0, *g3)
+        assert np.array_equal(a, b), name
+    assert np.array_equal(ref.ray_proj3D.emstep_nearest(s, ind, rv, nr, vol + 1, *g3),
+                          o32.ray_proj3D.emstep_nearest(s, ind, rv, nr, vol + 1, *g3))
+
+
+def test_totalvar(ref, o32):
+    rng = np.random.default_rng(5)
+    vol = rng.random((9, 11, 13)).astype(np.float32)
+    for lam in (0.0, 0.01):
+        assert ref.totalvar.total_var3d(lam, vol, 9, 11, 13) == \
+            pytest.approx(o32.totalvar.total_var3d(lam, vol, 9, 11, 13), rel=0, abs=0)
+        for nq in (0, 1):
+            ga, ba = ref.totalvar.total_var_grad3d(lam, vol, 9, 11, 13, nq)
+            gb, bb = o32.totalvar.total_var_grad3d(lam, vol, 9, 11, 13, nq)
+            assert np.array_equal(ga, gb) and ba == bb
