@@ -1,0 +1,325 @@
+// ct2_backproject.hip -- exact adjoint of the Joseph projector, gathered per
+// voxel brick with no global atomics.
+//
+// The reference scatters every ray's (weight, voxel) list into the volume
+// (emil_code/view_proj3D.f:1228-1251, :1355-1360).  Here a workgroup owns a
+// brick of voxels held in LDS.  For every view it projects the brick, grown
+// by one voxel, onto the detector; only rays through that footprint can
+// touch the brick.  Each candidate ray is set up and sampled with the same
+// device functions as the forward kernel, so the matrix entries are the
+// forward's own, and every sample's corner weights that fall inside the
+// brick are added to LDS (ds_add_f32).  The brick is written to HBM once.
+//
+// Linear weights per corner are (trav*wb)*wc and the contribution is
+// datum*weight, the reference's order; only the summation order differs.
+#include "ct2_internal.h"
+
+namespace ct2 {
+
+struct Brick {
+    int lo[3];
+    int ext[3];
+};
+
+// Detector footprint of an axis-aligned box for view p: rays from S through
+// every corner hit the detector plane D + u e_u + v e_v; returns the bin
+// range [jlo,jhi] x [klo,khi] (inclusive, clamped) or the full detector when
+// the box is not entirely in front of the source.  FP64, once per view per
+// workgroup.
+__device__ void project_corner(const float* __restrict__ f, const double P[3], double& u,
+                               double& v, int& ok) {
+    double S[3] = {f[0], f[1], f[2]};
+    double D[3] = {f[6], f[7], f[8]};
+    double eu[3] = {f[9], f[10], f[11]};
+    double ev[3] = {f[12], f[13], f[14]};
+    double n[3] = {eu[1] * ev[2] - eu[2] * ev[1], eu[2] * ev[0] - eu[0] * ev[2],
+                   eu[0] * ev[1] - eu[1] * ev[0]};
+    double dp[3] = {P[0] - S[0], P[1] - S[1], P[2] - S[2]};
+    double ds[3] = {D[0] - S[0], D[1] - S[1], D[2] - S[2]};
+    double den = dp[0] * n[0] + dp[1] * n[1] + dp[2] * n[2];
+    double num = ds[0] * n[0] + ds[1] * n[1] + ds[2] * n[2];
+    if (fabs(den) < 1e-12 * (fabs(num) + 1e-30)) { ok = 0; return; }
+    double t = num / den;
+    if (!(t > 0.0)) { ok = 0; return; }
+    double X[3] = {S[0] + t * dp[0] - D[0], S[1] + t * dp[1] - D[1], S[2] + t * dp[2] - D[2]};
+    double guu = eu[0] * eu[0] + eu[1] * eu[1] + eu[2] * eu[2];
+    double gvv = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
+    double guv = eu[0] * ev[0] + eu[1] * ev[1] + eu[2] * ev[2];
+    double bu = X[0] * eu[0] + X[1] * eu[1] + X[2] * eu[2];
+    double bv = X[0] * ev[0] + X[1] * ev[1] + X[2] * ev[2];
+    double det = guu * gvv - guv * guv;
+    if (fabs(det) < 1e-30) { ok = 0; return; }
+    u = (bu * gvv - bv * guv) / det;
+    v = (bv * guu - bu * guv) / det;
+    ok = 1;
+}
+
+template <int INTERP, typename OT>
+__global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int view0,
+                                                           int nviews, int zonly, int3 bdim,
+                                                           int3 nbr, OT* __restrict__ vol,
+                                                           const float* __restrict__ sino,
+                                                           const int8_t* __restrict__ ind,
+                                                           const float* __restrict__ fv) {
+    extern __shared__ float acc[];
+    __shared__ int s_range[4];
+    __shared__ double s_fp[8][2];
+    __shared__ int s_ok[8];
+
+    // brick coordinates: x-major over bricks so neighbouring workgroups share z
+    int bid = blockIdx.x;
+    int bz = bid % nbr.z;
+    int by = (bid / nbr.z) % nbr.y;
+    int bx = bid / (nbr.z * nbr.y);
+    Brick br;
+    br.lo[0] = bx * bdim.x; br.lo[1] = by * bdim.y; br.lo[2] = bz * bdim.z;
+    br.ext[0] = min(bdim.x, g.n[0] - br.lo[0]);
+    br.ext[1] = min(bdim.y, g.n[1] - br.lo[1]);
+    br.ext[2] = min(bdim.z, g.n[2] - br.lo[2]);
+    const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
+    for (int q = threadIdx.x; q < nloc; q += blockDim.x) acc[q] = 0.0f;
+
+    // box containing every sample point that can touch the brick: centres of
+    // voxels lo-1 .. lo+ext (one voxel margin on both sides)
+    double box_lo[3], box_hi[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        box_lo[q] = (double)g.o[q] + ((double)(br.lo[q] - 1) + 0.5) * g.d[q];
+        box_hi[q] = (double)g.o[q] + ((double)(br.lo[q] + br.ext[q]) + 0.5) * g.d[q];
+    }
+    // LDS strides of x, y, z inside the brick
+    const int ls[3] = {br.ext[1] * br.ext[2], br.ext[2], 1};
+
+    for (int pv = 0; pv < nviews; ++pv) {
+        const int p = view0 + pv;
+        const float* f = fv + (size_t)p * 18;
+        __syncthreads();   // previous view's readers of s_range are done
+        if (threadIdx.x < 8) {
+            int c = threadIdx.x;
+            double P[3] = {(c & 1) ? box_hi[0] : box_lo[0], (c & 2) ? box_hi[1] : box_lo[1],
+                           (c & 4) ? box_hi[2] : box_lo[2]};
+            double u = 0, v = 0;
+            int ok = 0;
+            project_corner(f, P, u, v, ok);
+            s_fp[c][0] = u;
+            s_fp[c][1] = v;
+            s_ok[c] = ok;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int allok = 1;
+            double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+            for (int c = 0; c < 8; ++c) {
+                allok &= s_ok[c];
+                umin = fmin(umin, s_fp[c][0]); umax = fmax(umax, s_fp[c][0]);
+                vmin = fmin(vmin, s_fp[c][1]); vmax = fmax(vmax, s_fp[c][1]);
+            }
+            int jlo = 0, jhi = dt.nu - 1, klo = 0, khi = dt.nv - 1;
+            if (allok) {
+                // bin j has centre u0 + (j+0.5) du; one bin of margin each side
+                double jl = (umin - dt.u0) / dt.du - 0.5, jh = (umax - dt.u0) / dt.du - 0.5;
+                double kl = (vmin - dt.v0) / dt.dv - 0.5, kh = (vmax - dt.v0) / dt.dv - 0.5;
+                if (dt.du < 0) { double t = jl; jl = jh; jh = t; }
+                if (dt.dv < 0) { double t = kl; kl = kh; kh = t; }
+                jl = fmax(jl, -2.0); jh = fmin(jh, (double)dt.nu + 1.0);
+                kl = fmax(kl, -2.0); kh = fmin(kh, (double)dt.nv + 1.0);
+                jlo = max(0, (int)floor(jl) - 1); jhi = min(dt.nu - 1, (int)ceil(jh) + 1);
+                klo = max(0, (int)floor(kl) - 1); khi = min(dt.nv - 1, (int)ceil(kh) + 1);
+            }
+            s_range[0] = jlo; s_range[1] = jhi; s_range[2] = klo; s_range[3] = khi;
+        }
+        __syncthreads();
+        const int jlo = s_range[0], jhi = s_range[1], klo = s_range[2], khi = s_range[3];
+        if (jhi < jlo || khi < klo) continue;
+        const int nk = khi - klo + 1;
+        const int ncand = (jhi - jlo + 1) * nk;
+        for (int cidx = threadIdx.x; cidx < ncand; cidx += blockDim.x) {
+            const int j = jlo + cidx / nk;
+            const int k = klo + cidx % nk;
+            const size_t r = ((size_t)p * dt.nu + j) * dt.nv + k;
+            if (ind != nullptr && ind[r] != 1) continue;
+            const float datum = sino[r];
+            if (datum == 0.0f) continue;
+            float S[3], B[3];
+            view_ray(fv, p, j, k, dt, S, B);
+            AxisSel A;
+            setup_exact(g, S, B, zonly != 0, A);
+            const int a = A.a, b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+            // select per-axis brick scalars without indexing a local array
+            const int la0 = sel3i(a, br.lo[0], br.lo[1], br.lo[2]);
+            const int la1 = la0 + sel3i(a, br.ext[0], br.ext[1], br.ext[2]);
+            const int lob = sel3i(b, br.lo[0], br.lo[1], br.lo[2]);
+            const int lobext = sel3i(b, br.ext[0], br.ext[1], br.ext[2]);
+            const int loc = sel3i(c, br.lo[0], br.lo[1], br.lo[2]);
+            const int locext = sel3i(c, br.ext[0], br.ext[1], br.ext[2]);
+            const int lsa = sel3i(a, ls[0], ls[1], ls[2]);
+            const int lsb = sel3i(b, ls[0], ls[1], ls[2]);
+            const int lsc = sel3i(c, ls[0], ls[1], ls[2]);
+            for (int l = la0; l < la1; ++l) {
+                Sample sm;
+                if (INTERP == CT2_LINEAR) {
+                    if (!sample_linear_exact(A, l, sm)) continue;
+                    const int qb = sm.ib - lob, qc = sm.ic - loc;
+                    // cell corners qb, qb+1 and qc, qc+1 must meet the brick
+                    if (qb < -1 || qb >= lobext || qc < -1 || qc >= locext) continue;
+                    const int base = (l - la0) * lsa;
+                    float w;
+                    {
+                        if (qb >= 0 && qc >= 0) {
+                            w = A.trav * sm.wb1 * sm.wc1;
+                            atomicAdd(&acc[base + qb * lsb + qc * lsc], datum * w);
+                        }
+                        if (qb + 1 < lobext && qc >= 0) {
+                            w = A.trav * sm.wb2 * sm.wc1;
+                            atomicAdd(&acc[base + (qb + 1) * lsb + qc * lsc], datum * w);
+                        }
+                        if (qb >= 0 && qc + 1 < locext) {
+                            w = A.trav * sm.wb1 * sm.wc2;
+                            atomicAdd(&acc[base + qb * lsb + (qc + 1) * lsc], datum * w);
+                        }
+                        if (qb + 1 < lobext && qc + 1 < locext) {
+                            w = A.trav * sm.wb2 * sm.wc2;
+                            atomicAdd(&acc[base + (qb + 1) * lsb + (qc + 1) * lsc], datum * w);
+                        }
+                    }
+                } else {
+                    if (!sample_nearest_exact(A, l, false, sm)) continue;
+                    const int qb = sm.ib - lob, qc = sm.ic - loc;
+                    if (qb < 0 || qb >= lobext || qc < 0 || qc >= locext) continue;
+                    atomicAdd(&acc[(l - la0) * lsa + qb * lsb + qc * lsc], datum * A.trav);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // write the brick: z-rows are contiguous in both LDS and HBM
+    for (int q = threadIdx.x; q < nloc; q += blockDim.x) {
+        int lz = q % br.ext[2];
+        int ly = (q / br.ext[2]) % br.ext[1];
+        int lx = q / (br.ext[2] * br.ext[1]);
+        size_t gi = (size_t)(br.lo[0] + lx) * g.s[0] + (size_t)(br.lo[1] + ly) * g.s[1] +
+                    (size_t)(br.lo[2] + lz) * g.s[2];
+        Store<OT>::st(vol, gi, acc[q]);
+    }
+}
+
+// Ray-list adjoint: no detector structure to bound a footprint, so rays
+// scatter straight into the zero-filled volume with float atomics.
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_backproject_rays(Grid g, int nr, int strict,
+                                                          float* __restrict__ vol,
+                                                          const float* __restrict__ sino,
+                                                          const int8_t* __restrict__ ind,
+                                                          const float* __restrict__ rv) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nr) return;
+    if (ind != nullptr && ind[r] != 1) return;
+    const float datum = sino[r];
+    float S[3], B[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        S[q] = rv[(size_t)r * 6 + q];
+        B[q] = rv[(size_t)r * 6 + 3 + q];
+    }
+    AxisSel A;
+    setup_exact(g, S, B, false, A);
+    for (int l = 0; l < A.na; ++l) {
+        Sample sm;
+        if (INTERP == CT2_LINEAR) {
+            if (!sample_linear_exact(A, l, sm)) continue;
+            int i00 = l * A.sa + sm.ib * A.sb + sm.ic * A.sc;
+            {
+                atomicAdd(&vol[i00], datum * (A.trav * sm.wb1 * sm.wc1));
+                atomicAdd(&vol[i00 + A.sb], datum * (A.trav * sm.wb2 * sm.wc1));
+                atomicAdd(&vol[i00 + A.sc], datum * (A.trav * sm.wb1 * sm.wc2));
+                atomicAdd(&vol[i00 + A.sb + A.sc], datum * (A.trav * sm.wb2 * sm.wc2));
+            }
+        } else {
+            if (!sample_nearest_exact(A, l, strict != 0, sm)) continue;
+            atomicAdd(&vol[l * A.sa + sm.ib * A.sb + sm.ic * A.sc], datum * A.trav);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host ----
+static int3 default_brick(const ct2_desc* d) {
+    int3 b;
+    b.x = d->brick[0] > 0 ? d->brick[0] : 16;
+    b.y = d->brick[1] > 0 ? d->brick[1] : 16;
+    b.z = d->brick[2] > 0 ? d->brick[2] : 32;
+    return b;
+}
+
+template <int INTERP, typename OT>
+static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+                        void* vol, const float* sino, const int8_t* ind, const float* fv,
+                        hipStream_t st) {
+    int3 b = default_brick(d);
+    b.x = min(b.x, g.n[0]); b.y = min(b.y, g.n[1]); b.z = min(b.z, g.n[2]);
+    size_t lds = (size_t)b.x * b.y * b.z * sizeof(float);
+    if (lds > 64 * 1024) {
+        set_error("ct2_backproject: brick %dx%dx%d needs %zu B of LDS (max 65536)", b.x, b.y,
+                  b.z, lds);
+        return CT2_EINVAL;
+    }
+    int3 nb = make_int3((g.n[0] + b.x - 1) / b.x, (g.n[1] + b.y - 1) / b.y,
+                        (g.n[2] + b.z - 1) / b.z);
+    dim3 grid(nb.x * nb.y * nb.z);
+    hipLaunchKernelGGL((k_backproject_brick<INTERP, OT>), grid, dim3(256), lds, st, g, dt, view0,
+                       nviews, d->axis == CT2_AXIS_Z ? 1 : 0, b, nb, (OT*)vol, sino, ind, fv);
+    return CT2_OK;
+}
+
+template <int INTERP>
+static int dispatch_out(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+                        void* vol, const float* sino, const int8_t* ind, const float* fv,
+                        hipStream_t st) {
+    switch (d->vol_dtype) {
+        case CT2_F32: return launch_brick<INTERP, float>(d, g, dt, view0, nviews, vol, sino, ind, fv, st);
+        case CT2_F16: return launch_brick<INTERP, __half>(d, g, dt, view0, nviews, vol, sino, ind, fv, st);
+        case CT2_BF16: return launch_brick<INTERP, hip_bfloat16>(d, g, dt, view0, nviews, vol, sino, ind, fv, st);
+    }
+    set_error("ct2_backproject: unknown vol_dtype %d", d->vol_dtype);
+    return CT2_EINVAL;
+}
+
+int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int8_t* ind,
+                       const float* fv, hipStream_t st) {
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
+    int rc;
+    if ((rc = check_grid(g)) || (rc = check_det(dt))) return rc;
+    int view0 = d->view0, nviews = d->nviews ? d->nviews : d->ns - d->view0;
+    if (view0 < 0 || nviews <= 0 || view0 + nviews > d->ns) {
+        set_error("ct2_backproject: bad view range [%d, %d) of %d", view0, view0 + nviews, d->ns);
+        return CT2_EINVAL;
+    }
+    if (!sino || !fv || !vol) { set_error("ct2_backproject: null pointer"); return CT2_EINVAL; }
+    rc = d->interp == CT2_LINEAR
+             ? dispatch_out<CT2_LINEAR>(d, g, dt, view0, nviews, vol, (const float*)sino, ind, fv, st)
+             : dispatch_out<CT2_NEAREST>(d, g, dt, view0, nviews, vol, (const float*)sino, ind, fv, st);
+    if (rc) return rc;
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int launch_ray_backproject(const ct2_desc* d, float* vol, const float* sino, const int8_t* ind,
+                           const float* rv, int nr, int strict, hipStream_t st) {
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    if (nr < 0 || !vol) { set_error("ray backprojection: bad arguments"); return CT2_EINVAL; }
+    int rc0 = launch_fill(vol, 0.0f, (size_t)g.n[0] * g.n[1] * g.n[2], st);
+    if (rc0) return rc0;
+    if (nr == 0) return CT2_OK;
+    if (!sino || !rv) { set_error("ray backprojection: null pointer"); return CT2_EINVAL; }
+    dim3 grid((nr + 255) / 256);
+    if (d->interp == CT2_LINEAR)
+        hipLaunchKernelGGL(k_backproject_rays<CT2_LINEAR>, grid, dim3(256), 0, st, g, nr, 0, vol, sino, ind, rv);
+    else
+        hipLaunchKernelGGL(k_backproject_rays<CT2_NEAREST>, grid, dim3(256), 0, st, g, nr, strict, vol, sino, ind, rv);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+}  // namespace ct2

@@ -1,0 +1,227 @@
+// ct2_internal.h -- shared device/host definitions for the ct2 HIP kernels.
+//
+// Geometry follows ct-2's Joseph projector (emil_code/view_proj3D.f:703-910):
+// a ray from source S to detector bin B marches plane by plane along its
+// driving axis (the axis of largest |B-S|/voxel, strict '>' so ties go to the
+// later axis), samples the two transverse axes bilinearly at the plane
+// centre, and weights each sample by d_axis*sqrt(1+slope_b^2+slope_c^2).
+//
+// Device layout: volume element (i,j,k) at i*sx + j*sy + k*sz (32-bit
+// offsets; the default C-order [nx][ny][nz] has sz = 1), sinogram element
+// (p,j,k) at (p*nu + j)*nv + k, frame vectors [ns][18] float32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bfloat16.h>
+#include <stdint.h>
+
+#include "../../include/ct2.h"
+
+namespace ct2 {
+
+// ---------------------------------------------------------------- errors ----
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define CT2_HIP(call)                                        \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return ::ct2::hip_fail(e_, #call); \
+    } while (0)
+
+// ------------------------------------------------------------- geometry ----
+struct Grid {
+    int n[3];
+    float d[3];
+    float o[3];
+    int s[3];   // element strides of x, y, z
+};
+
+struct Det {
+    int ns, nu, nv;
+    float du, dv, u0, v0;
+};
+
+// Per-ray scalars of the driving axis a and transverse axes b, c, chosen once
+// so the inner loop never indexes a runtime axis.
+struct AxisSel {
+    int na, nb, nc;
+    int sa, sb, sc;
+    float da, db, dc;
+    float oa, ob, oc;
+    float Sa, Sb, Sc;
+    float r1, r2, trav;
+    int a;
+};
+
+__device__ __forceinline__ float sel3(int a, float x, float y, float z) {
+    return a == 0 ? x : (a == 1 ? y : z);
+}
+__device__ __forceinline__ int sel3i(int a, int x, int y, int z) {
+    return a == 0 ? x : (a == 1 ? y : z);
+}
+
+// Bin position of view ray (p,j,k): B = D + e_u*u + e_v*v with
+// u = u0 + (j+0.5)du, v = v0 + (k+0.5)dv (view_proj3D.f:784-797).
+__device__ __forceinline__ void view_ray(const float* __restrict__ fv, int p, int j, int k,
+                                         const Det& dt, float S[3], float B[3]) {
+#pragma clang fp contract(off)
+    const float* f = fv + (size_t)p * 18;
+    float u = dt.u0 + ((float)j + 0.5f) * dt.du;
+    float v = dt.v0 + ((float)k + 0.5f) * dt.dv;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        S[q] = f[q];
+        B[q] = f[6 + q] + f[9 + q] * u + f[12 + q] * v;
+    }
+}
+
+// Ray setup in the reference's FP32 operation order (view_proj3D.f:802-815,
+// :843-847, :873-876; forced z: view_proj3D_tomo.f:111-113).
+__device__ __forceinline__ void setup_exact(const Grid& g, const float S[3], const float B[3],
+                                            bool zonly, AxisSel& A) {
+#pragma clang fp contract(off)
+    float dx = B[0] - S[0], dy = B[1] - S[1], dz = B[2] - S[2];
+    int a;
+    if (zonly) {
+        a = 2;
+    } else {
+        float xad = fabsf(dx) / g.d[0];
+        float yad = fabsf(dy) / g.d[1];
+        float zad = fabsf(dz) / g.d[2];
+        a = ((xad > yad) && (xad > zad)) ? 0 : ((yad > zad) ? 1 : 2);
+    }
+    int b = (a == 0) ? 1 : 0;
+    int c = (a == 2) ? 1 : 2;
+    float diffa = sel3(a, dx, dy, dz);
+    float diffb = sel3(b, dx, dy, dz);
+    float diffc = sel3(c, dx, dy, dz);
+    A.a = a;
+    A.na = sel3i(a, g.n[0], g.n[1], g.n[2]);
+    A.nb = sel3i(b, g.n[0], g.n[1], g.n[2]);
+    A.nc = sel3i(c, g.n[0], g.n[1], g.n[2]);
+    A.sa = sel3i(a, g.s[0], g.s[1], g.s[2]);
+    A.sb = sel3i(b, g.s[0], g.s[1], g.s[2]);
+    A.sc = sel3i(c, g.s[0], g.s[1], g.s[2]);
+    A.da = sel3(a, g.d[0], g.d[1], g.d[2]);
+    A.db = sel3(b, g.d[0], g.d[1], g.d[2]);
+    A.dc = sel3(c, g.d[0], g.d[1], g.d[2]);
+    A.oa = sel3(a, g.o[0], g.o[1], g.o[2]);
+    A.ob = sel3(b, g.o[0], g.o[1], g.o[2]);
+    A.oc = sel3(c, g.o[0], g.o[1], g.o[2]);
+    A.Sa = sel3(a, S[0], S[1], S[2]);
+    A.Sb = sel3(b, S[0], S[1], S[2]);
+    A.Sc = sel3(c, S[0], S[1], S[2]);
+    A.r1 = diffb / diffa;
+    A.r2 = diffc / diffa;
+    A.trav = A.da * sqrtf(1.0f + A.r1 * A.r1 + A.r2 * A.r2);
+}
+
+// One sample on plane l, reference FP32 order.  Linear: cell
+// floor((p-o)/d-0.5), valid 0 <= i < n-1, w2 = (p-(d*(i+0.5)+o))/d
+// (view_proj3D.f:816-826).  Nearest: floor((p-o)/d), valid 0 <= i <= hi
+// with hi = n-1, or n-2 for the ray-list bp/ART/EM (ray_proj3D.f:895).
+struct Sample {
+    int ib, ic;
+    float wb1, wb2, wc1, wc2;
+};
+
+__device__ __forceinline__ bool sample_linear_exact(const AxisSel& A, int l, Sample& sm) {
+#pragma clang fp contract(off)
+    float t = A.oa + A.da * ((float)l + 0.5f);
+    float pb = A.Sb + A.r1 * (t - A.Sa);
+    float pc = A.Sc + A.r2 * (t - A.Sa);
+    int ib = (int)floorf((pb - A.ob) / A.db - 0.5f);
+    int ic = (int)floorf((pc - A.oc) / A.dc - 0.5f);
+    if (!((ib >= 0) && (ib < A.nb - 1) && (ic >= 0) && (ic < A.nc - 1))) return false;
+    sm.ib = ib;
+    sm.ic = ic;
+    sm.wb2 = (pb - (A.db * ((float)ib + 0.5f) + A.ob)) / A.db;
+    sm.wb1 = 1.0f - sm.wb2;
+    sm.wc2 = (pc - (A.dc * ((float)ic + 0.5f) + A.oc)) / A.dc;
+    sm.wc1 = 1.0f - sm.wc2;
+    return true;
+}
+
+__device__ __forceinline__ bool sample_nearest_exact(const AxisSel& A, int l, bool strict,
+                                                     Sample& sm) {
+#pragma clang fp contract(off)
+    float t = A.oa + A.da * ((float)l + 0.5f);
+    float pb = A.Sb + A.r1 * (t - A.Sa);
+    float pc = A.Sc + A.r2 * (t - A.Sa);
+    int ib = (int)floorf((pb - A.ob) / A.db);
+    int ic = (int)floorf((pc - A.oc) / A.dc);
+    int hb = strict ? A.nb - 2 : A.nb - 1;
+    int hc = strict ? A.nc - 2 : A.nc - 1;
+    if (!((ib >= 0) && (ib <= hb) && (ic >= 0) && (ic <= hc))) return false;
+    sm.ib = ib;
+    sm.ic = ic;
+    return true;
+}
+
+// ------------------------------------------------------- storage types ----
+template <typename T> struct Store;
+template <> struct Store<float> {
+    __device__ __forceinline__ static float ld(const float* p, int i) { return p[i]; }
+    __device__ __forceinline__ static void st(float* p, size_t i, float v) { p[i] = v; }
+};
+template <> struct Store<__half> {
+    __device__ __forceinline__ static float ld(const __half* p, int i) { return __half2float(p[i]); }
+    __device__ __forceinline__ static void st(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
+};
+template <> struct Store<hip_bfloat16> {
+    __device__ __forceinline__ static float ld(const hip_bfloat16* p, int i) {
+        return static_cast<float>(p[i]);
+    }
+    __device__ __forceinline__ static void st(hip_bfloat16* p, size_t i, float v) {
+        p[i] = hip_bfloat16(v);
+    }
+};
+
+// ------------------------------------------------------- host utilities ----
+inline Grid make_grid(int nx, int ny, int nz, float dx, float dy, float dz, float x0, float y0,
+                      float z0) {
+    Grid g;
+    g.n[0] = nx; g.n[1] = ny; g.n[2] = nz;
+    g.d[0] = dx; g.d[1] = dy; g.d[2] = dz;
+    g.o[0] = x0; g.o[1] = y0; g.o[2] = z0;
+    g.s[0] = ny * nz; g.s[1] = nz; g.s[2] = 1;
+    return g;
+}
+
+inline Det make_det(int ns, int nu, int nv, float du, float dv, float u0, float v0) {
+    Det d;
+    d.ns = ns; d.nu = nu; d.nv = nv;
+    d.du = du; d.dv = dv; d.u0 = u0; d.v0 = v0;
+    return d;
+}
+
+int check_grid(const Grid& g);
+int check_det(const Det& d);
+
+// ------------------------------------------------------ launchers (host) ----
+int launch_project(const ct2_desc* desc, void* sino, const int8_t* ind, const float* fv,
+                   const void* vol, hipStream_t st);
+int launch_backproject(const ct2_desc* desc, void* vol, const void* sino, const int8_t* ind,
+                       const float* fv, hipStream_t st);
+int launch_ray_project(const ct2_desc* desc, float* sino, const int8_t* ind, const float* rv,
+                       int nr, const float* vol, int strict, hipStream_t st);
+int launch_ray_backproject(const ct2_desc* desc, float* vol, const float* sino,
+                           const int8_t* ind, const float* rv, int nr, int strict,
+                           hipStream_t st);
+int launch_art(const ct2_desc* d, const Grid* g2, float reg_fac, const float* sino,
+               const int8_t* ind, const float* fv, float* vol, float* vol2, hipStream_t st);
+int launch_ray_art(const ct2_desc* d, int strict, float reg_fac, const float* sino,
+                   const int8_t* ind, const float* rv, int nr, float* vol, hipStream_t st);
+size_t em_work_floats(const ct2_desc* d);
+int launch_em(const ct2_desc* d, const float* sino, const int8_t* ind, const float* fv,
+              float* vol, float* work, const float* sens, hipStream_t st);
+int launch_ray_em(const ct2_desc* d, int strict, const float* sino, const int8_t* ind,
+                  const float* rv, int nr, float* vol, float* work, hipStream_t st);
+int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
+                          hipStream_t st);
+// device fill (graph-capture safe, unlike a memset node)
+int launch_fill(float* p, float v, size_t n, hipStream_t st);
+
+}  // namespace ct2

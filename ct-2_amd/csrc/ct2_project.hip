@@ -1,0 +1,244 @@
+// ct2_project.hip -- Joseph forward projection, one ray per lane.
+//
+// A 256-thread workgroup covers a 4 (u) x 64 (v) tile of one view: each
+// wavefront is 64 consecutive detector rows v of one column u, so its rays
+// sample neighbouring voxels (v runs along z for circular orbits, and z is
+// the fastest volume axis) and its sinogram stores are one coalesced row.
+//
+// Reference: emil_code/view_proj3D.f:703-910 (linear), :915-1092 (nearest),
+// view_proj3D_tomo.f:4-148 (z forced), ray_proj3D.f:514-804 (ray lists).
+#include "ct2_internal.h"
+
+namespace ct2 {
+
+// March one ray and return its sum; reference FP32 order (sequential sum
+// over planes), so the result is bit-identical to the Fortran.
+template <int INTERP, typename VT>
+__device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restrict__ vol,
+                                             bool strict) {
+#pragma clang fp contract(off)
+    float total = 0.0f;
+    for (int l = 0; l < A.na; ++l) {
+        Sample sm;
+        if (INTERP == CT2_LINEAR) {
+            if (!sample_linear_exact(A, l, sm)) continue;
+            int i00 = l * A.sa + sm.ib * A.sb + sm.ic * A.sc;
+            float v00 = Store<VT>::ld(vol, i00);
+            float v10 = Store<VT>::ld(vol, i00 + A.sb);
+            float v01 = Store<VT>::ld(vol, i00 + A.sc);
+            float v11 = Store<VT>::ld(vol, i00 + A.sb + A.sc);
+            float vi1 = sm.wb1 * v00 + sm.wb2 * v10;
+            float vi2 = sm.wb1 * v01 + sm.wb2 * v11;
+            float val = sm.wc1 * vi1 + sm.wc2 * vi2;
+            total = total + A.trav * val;
+        } else {
+            if (!sample_nearest_exact(A, l, strict, sm)) continue;
+            int i00 = l * A.sa + sm.ib * A.sb + sm.ic * A.sc;
+            total = total + A.trav * Store<VT>::ld(vol, i00);
+        }
+    }
+    return total;
+}
+
+// Fast arithmetic: per-ray setup in FP64, the transverse cell coordinate of
+// plane l is one FMA (u(l) = u0 + l*du), FP32 bilinear lerps, compensated sum.
+struct RayFast {
+    int na, sa, sb, sc, nb, nc;
+    float ub0, dub, uc0, duc;   // cell coordinate (p-o)/d - 0.5 (linear) or (p-o)/d (nearest)
+    float trav;
+};
+
+__device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], const double B[3],
+                                           bool zonly, bool linear, RayFast& R) {
+    double dd[3] = {B[0] - S[0], B[1] - S[1], B[2] - S[2]};
+    int a;
+    if (zonly) {
+        a = 2;
+    } else {
+        double xad = fabs(dd[0]) / g.d[0], yad = fabs(dd[1]) / g.d[1], zad = fabs(dd[2]) / g.d[2];
+        a = ((xad > yad) && (xad > zad)) ? 0 : ((yad > zad) ? 1 : 2);
+    }
+    int b = (a == 0) ? 1 : 0;
+    int c = (a == 2) ? 1 : 2;
+    double da = g.d[a], db = g.d[b], dc = g.d[c];
+    double r1 = dd[b] / dd[a], r2 = dd[c] / dd[a];
+    double t0 = (double)g.o[a] + 0.5 * da;   // centre of plane 0
+    double half = linear ? 0.5 : 0.0;
+    double pb0 = S[b] + r1 * (t0 - S[a]);
+    double pc0 = S[c] + r2 * (t0 - S[a]);
+    R.ub0 = (float)((pb0 - g.o[b]) / db - half);
+    R.uc0 = (float)((pc0 - g.o[c]) / dc - half);
+    R.dub = (float)(r1 * da / db);
+    R.duc = (float)(r2 * da / dc);
+    R.trav = (float)(da * sqrt(1.0 + r1 * r1 + r2 * r2));
+    R.na = g.n[a]; R.nb = g.n[b]; R.nc = g.n[c];
+    R.sa = g.s[a]; R.sb = g.s[b]; R.sc = g.s[c];
+}
+
+template <int INTERP, typename VT>
+__device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restrict__ vol,
+                                            bool strict) {
+    float sum = 0.0f, comp = 0.0f;
+    const unsigned hb = (unsigned)(INTERP == CT2_LINEAR ? R.nb - 1 : (strict ? R.nb - 1 : R.nb));
+    const unsigned hc = (unsigned)(INTERP == CT2_LINEAR ? R.nc - 1 : (strict ? R.nc - 1 : R.nc));
+    for (int l = 0; l < R.na; ++l) {
+        float ub = __builtin_fmaf((float)l, R.dub, R.ub0);
+        float uc = __builtin_fmaf((float)l, R.duc, R.uc0);
+        float fb = floorf(ub), fc = floorf(uc);
+        int ib = (int)fb, ic = (int)fc;
+        if ((unsigned)ib >= hb || (unsigned)ic >= hc) continue;
+        int i00 = l * R.sa + ib * R.sb + ic * R.sc;
+        float val;
+        if (INTERP == CT2_LINEAR) {
+            float wb = ub - fb, wc = uc - fc;
+            float v00 = Store<VT>::ld(vol, i00);
+            float v10 = Store<VT>::ld(vol, i00 + R.sb);
+            float v01 = Store<VT>::ld(vol, i00 + R.sc);
+            float v11 = Store<VT>::ld(vol, i00 + R.sb + R.sc);
+            float vi1 = __builtin_fmaf(wb, v10 - v00, v00);
+            float vi2 = __builtin_fmaf(wb, v11 - v01, v01);
+            val = __builtin_fmaf(wc, vi2 - vi1, vi1);
+        } else {
+            val = Store<VT>::ld(vol, i00);
+        }
+        // Kahan compensation
+        float y = val - comp;
+        float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+    }
+    return sum * R.trav;
+}
+
+template <int INTERP, int ARITH, typename VT>
+__global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0,
+                                                       int zonly, float* __restrict__ sino,
+                                                       const int8_t* __restrict__ ind,
+                                                       const float* __restrict__ fv,
+                                                       const VT* __restrict__ vol) {
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = view0 + blockIdx.z;
+    if (k >= dt.nv || j >= dt.nu) return;
+    const size_t r = ((size_t)p * dt.nu + j) * dt.nv + k;
+    float out = 0.0f;
+    if (ind == nullptr || ind[r] == 1) {
+        if (ARITH == CT2_ARITH_EXACT) {
+            float S[3], B[3];
+            view_ray(fv, p, j, k, dt, S, B);
+            AxisSel A;
+            setup_exact(g, S, B, zonly != 0, A);
+            out = march_exact<INTERP, VT>(A, vol, false);
+        } else {
+            const float* f = fv + (size_t)p * 18;
+            double u = (double)dt.u0 + ((double)j + 0.5) * dt.du;
+            double v = (double)dt.v0 + ((double)k + 0.5) * dt.dv;
+            double S[3], B[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                S[q] = f[q];
+                B[q] = (double)f[6 + q] + (double)f[9 + q] * u + (double)f[12 + q] * v;
+            }
+            RayFast R;
+            setup_fast(g, S, B, zonly != 0, INTERP == CT2_LINEAR, R);
+            out = march_fast<INTERP, VT>(R, vol, false);
+        }
+    }
+    sino[r] = out;
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_project_rays(Grid g, int nr, int strict,
+                                                      float* __restrict__ sino,
+                                                      const int8_t* __restrict__ ind,
+                                                      const float* __restrict__ rv,
+                                                      const float* __restrict__ vol) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nr) return;
+    float out = 0.0f;
+    if (ind == nullptr || ind[r] == 1) {
+        float S[3], B[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            S[q] = rv[(size_t)r * 6 + q];
+            B[q] = rv[(size_t)r * 6 + 3 + q];
+        }
+        AxisSel A;
+        setup_exact(g, S, B, false, A);
+        out = march_exact<INTERP, float>(A, vol, strict != 0);
+    }
+    sino[r] = out;
+}
+
+// ---------------------------------------------------------------- host ----
+template <int INTERP, int ARITH, typename VT>
+static void launch_views(const Grid& g, const Det& dt, int view0, int nviews, int zonly,
+                         float* sino, const int8_t* ind, const float* fv, const void* vol,
+                         hipStream_t st) {
+    dim3 grid((dt.nv + 63) / 64, (dt.nu + 3) / 4, nviews);
+    hipLaunchKernelGGL((k_project_views<INTERP, ARITH, VT>), grid, dim3(256), 0, st, g, dt,
+                       view0, zonly, sino, ind, fv, (const VT*)vol);
+}
+
+template <int INTERP, int ARITH>
+static int dispatch_dtype(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nv,
+                          float* sino, const int8_t* ind, const float* fv, const void* vol,
+                          hipStream_t st) {
+    int z = d->axis == CT2_AXIS_Z;
+    switch (d->vol_dtype) {
+        case CT2_F32: launch_views<INTERP, ARITH, float>(g, dt, view0, nv, z, sino, ind, fv, vol, st); break;
+        case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, view0, nv, z, sino, ind, fv, vol, st); break;
+        case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, view0, nv, z, sino, ind, fv, vol, st); break;
+        default: set_error("ct2_project: unknown vol_dtype %d", d->vol_dtype); return CT2_EINVAL;
+    }
+    return CT2_OK;
+}
+
+int launch_project(const ct2_desc* d, void* sino, const int8_t* ind, const float* fv,
+                   const void* vol, hipStream_t st) {
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
+    int rc;
+    if ((rc = check_grid(g)) || (rc = check_det(dt))) return rc;
+    int view0 = d->view0, nviews = d->nviews ? d->nviews : d->ns - d->view0;
+    if (view0 < 0 || nviews <= 0 || view0 + nviews > d->ns) {
+        set_error("ct2_project: bad view range [%d, %d) of %d", view0, view0 + nviews, d->ns);
+        return CT2_EINVAL;
+    }
+    if (!sino || !fv || !vol) { set_error("ct2_project: null pointer"); return CT2_EINVAL; }
+    float* s = (float*)sino;
+    if (d->arith == CT2_ARITH_EXACT) {
+        rc = d->interp == CT2_LINEAR
+                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, s, ind, fv, vol, st)
+                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, s, ind, fv, vol, st);
+    } else if (d->arith == CT2_ARITH_FAST) {
+        rc = d->interp == CT2_LINEAR
+                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, view0, nviews, s, ind, fv, vol, st)
+                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, view0, nviews, s, ind, fv, vol, st);
+    } else {
+        set_error("ct2_project: unknown arith %d", d->arith);
+        return CT2_EINVAL;
+    }
+    if (rc) return rc;
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int launch_ray_project(const ct2_desc* d, float* sino, const int8_t* ind, const float* rv,
+                       int nr, const float* vol, int strict, hipStream_t st) {
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    int rc;
+    if ((rc = check_grid(g))) return rc;
+    if (nr < 0) { set_error("ray projection: nr < 0"); return CT2_EINVAL; }
+    if (nr == 0) return CT2_OK;
+    if (!sino || !rv || !vol) { set_error("ray projection: null pointer"); return CT2_EINVAL; }
+    dim3 grid((nr + 255) / 256);
+    if (d->interp == CT2_LINEAR)
+        hipLaunchKernelGGL(k_project_rays<CT2_LINEAR>, grid, dim3(256), 0, st, g, nr, 0, sino, ind, rv, vol);
+    else
+        hipLaunchKernelGGL(k_project_rays<CT2_NEAREST>, grid, dim3(256), 0, st, g, nr, strict, sino, ind, rv, vol);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+}  // namespace ct2

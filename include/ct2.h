@@ -1,0 +1,234 @@
+/*
+ * ct2.h -- C ABI of libct2.so, the MI355X (gfx950) implementation of ct-2's
+ * reconstruction hot path.
+ *
+ * The reference reaches its Fortran kernels through f2py extension modules
+ * (emil_code/tomo3D.py:10-14, dispatch tables tomo3D.py:55-103).  Each
+ * entry point below replaces one f2py-wrapped subroutine and keeps its
+ * argument list and order (outputs first, as in the Fortran), with three
+ * differences:
+ *   - every array is a DEVICE pointer (hipMalloc'd or a torch tensor's
+ *     data_ptr), C-ordered over the reference's logical indices:
+ *       volume   smat[nx][ny][nz]   element (i,j,k) at (i*ny+j)*nz+k
+ *       sinogram sinomat[ns][nu][nv]
+ *       indicator indsino[ns][nu][nv] as int8 (a ray is used iff == 1);
+ *                NULL means every ray is used
+ *       frame_vectors[ns][18], ray_vectors[nr][6] float32;
+ *   - outputs that f2py would return are written in place (zero-filled by
+ *     the library where f2py zero-fills an intent(out) array);
+ *   - a trailing `stream` (a hipStream_t, NULL = default stream); every call
+ *     is asynchronous on that stream and allocates nothing unless stated.
+ * Every function returns CT2_OK (0) or a negative status; ct2_last_error()
+ * then describes the failure (thread-local).
+ */
+#ifndef CT2_H
+#define CT2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* ct2_stream_t;
+
+enum { CT2_OK = 0, CT2_EINVAL = -1, CT2_EHIP = -2, CT2_ENOMEM = -3 };
+enum { CT2_LINEAR = 0, CT2_NEAREST = 1 };
+enum { CT2_AXIS_FREE = 0, CT2_AXIS_Z = 1 };
+enum { CT2_F32 = 0, CT2_F16 = 1, CT2_BF16 = 2 };
+/* CT2_ARITH_EXACT: the reference's FP32 operation order, correctly rounded
+ * divisions, no FMA contraction -- bit-identical to the reference Fortran
+ * for the forward projector.  CT2_ARITH_FAST: per-ray setup in FP64, FMA
+ * plane stepping in FP32, compensated (Kahan) ray sums. */
+enum { CT2_ARITH_EXACT = 0, CT2_ARITH_FAST = 1 };
+
+/* Operator descriptor of the extended API. */
+typedef struct ct2_desc {
+    int ns, nu, nv;                    /* views, detector columns (u), rows (v) */
+    float du, dv, u0, v0;              /* bin pitch and left edge of the detector */
+    int nx, ny, nz;                    /* volume shape */
+    float dx, dy, dz, x0, y0, z0;      /* voxel size and lower volume corner */
+    int interp;                        /* CT2_LINEAR | CT2_NEAREST */
+    int axis;                          /* CT2_AXIS_FREE | CT2_AXIS_Z */
+    int vol_dtype;                     /* storage type of the volume operand */
+    int arith;                         /* CT2_ARITH_EXACT | CT2_ARITH_FAST */
+    int view0, nviews;                 /* view subset [view0, view0+nviews); nviews 0 = all */
+    int brick[3];                      /* adjoint voxel brick (0 = autotuned/default) */
+} ct2_desc;
+
+/* ---------------------------------------------------------------- misc -- */
+int ct2_version(void);
+const char* ct2_last_error(void);
+/* Fills *desc with defaults for the given geometry. */
+int ct2_desc_init(ct2_desc* desc, int ns, int nu, int nv, float du, float dv, float u0,
+                  float v0, int nx, int ny, int nz, float dx, float dy, float dz, float x0,
+                  float y0, float z0);
+
+/* ----------------------------------------------------- extended API ---- */
+/* Joseph forward projection of vol (storage desc->vol_dtype) into the
+ * float32 sinogram; rays with indicator != 1 are set to 0. */
+int ct2_project(const ct2_desc* desc, float* sino, const int8_t* ind, const float* fv,
+                const void* vol, ct2_stream_t stream);
+/* Exact adjoint of ct2_project (same matrix entries), written over vol
+ * (storage desc->vol_dtype). */
+int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
+                    const float* fv, ct2_stream_t stream);
+/* SART update over the view subset of desc: x += lambda * A^T((b-Ax)/A1) / A^T1
+ * restricted to those views, with work = 2*ns*nu*nv + 2*nx*ny*nz floats of
+ * device scratch supplied by the caller. */
+int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const int8_t* ind,
+                  const float* fv, float* vol, float* work, ct2_stream_t stream);
+
+/* ------------------------------------------ view_proj3D.f replacements -- */
+/* view_proj3D.f:703 sinoproj_linear */
+int ct2_sinoproj_linear(float* sinomat, const int8_t* indsino, const float* frame_vectors,
+                        int ns, int nu, int nv, float du, float dv, float u0, float v0,
+                        const float* smat, float dx, float dy, float dz, float x0, float y0,
+                        float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* view_proj3D.f:915 sinoproj_nearest */
+int ct2_sinoproj_nearest(float* sinomat, const int8_t* indsino, const float* frame_vectors,
+                         int ns, int nu, int nv, float du, float dv, float u0, float v0,
+                         const float* smat, float dx, float dy, float dz, float x0, float y0,
+                         float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* view_proj3D.f:1096 sinobackproject_linear */
+int ct2_sinobackproject_linear(const float* sinomat, const int8_t* indsino,
+                               const float* frame_vectors, int ns, int nu, int nv, float du,
+                               float dv, float u0, float v0, float* smat, float dx, float dy,
+                               float dz, float x0, float y0, float z0, int nx, int ny, int nz,
+                               ct2_stream_t stream);
+/* view_proj3D.f:1374 sinobackproject_nearest */
+int ct2_sinobackproject_nearest(const float* sinomat, const int8_t* indsino,
+                                const float* frame_vectors, int ns, int nu, int nv, float du,
+                                float dv, float u0, float v0, float* smat, float dx, float dy,
+                                float dz, float x0, float y0, float z0, int nx, int ny, int nz,
+                                ct2_stream_t stream);
+/* view_proj3D.f:1769 pd_backproj_linear (voxel driven, not an adjoint) */
+int ct2_pd_backproj_linear(const float* sinomat, const int8_t* indsino,
+                           const float* frame_vectors, int ns, int nu, int nv, float du,
+                           float dv, float u0, float v0, float* smat, float dx, float dy,
+                           float dz, float x0, float y0, float z0, int nx, int ny, int nz,
+                           ct2_stream_t stream);
+/* view_proj3D.f:1598 pd_backproj_nearest */
+int ct2_pd_backproj_nearest(const float* sinomat, const int8_t* indsino,
+                            const float* frame_vectors, int ns, int nu, int nv, float du,
+                            float dv, float u0, float v0, float* smat, float dx, float dy,
+                            float dz, float x0, float y0, float z0, int nx, int ny, int nz,
+                            ct2_stream_t stream);
+/* view_proj3D.f:1950 artstep_linear (sequential Kaczmarz; smat in/out) */
+int ct2_artstep_linear(float reg_fac, const float* sinomat, const int8_t* indsino,
+                       const float* frame_vectors, int ns, int nu, int nv, float du, float dv,
+                       float u0, float v0, float* smat, float dx, float dy, float dz, float x0,
+                       float y0, float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* view_proj3D.f:2275 artstep_nearest */
+int ct2_artstep_nearest(float reg_fac, const float* sinomat, const int8_t* indsino,
+                        const float* frame_vectors, int ns, int nu, int nv, float du, float dv,
+                        float u0, float v0, float* smat, float dx, float dy, float dz, float x0,
+                        float y0, float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* view_proj3D.f:2507 emstep_linear (MLEM; smat in/out).  Allocates two
+ * volume-sized scratch arrays stream-ordered (hipMallocAsync). */
+int ct2_emstep_linear(const float* sinomat, const int8_t* indsino, const float* frame_vectors,
+                      int ns, int nu, int nv, float du, float dv, float u0, float v0,
+                      float* smat, float dx, float dy, float dz, float x0, float y0, float z0,
+                      int nx, int ny, int nz, ct2_stream_t stream);
+/* view_proj3D.f:2834 emstep_nearest */
+int ct2_emstep_nearest(const float* sinomat, const int8_t* indsino, const float* frame_vectors,
+                       int ns, int nu, int nv, float du, float dv, float u0, float v0,
+                       float* smat, float dx, float dy, float dz, float x0, float y0, float z0,
+                       int nx, int ny, int nz, ct2_stream_t stream);
+/* MLEM with caller-supplied scratch work[2*nx*ny*nz] and an optional cached
+ * sensitivity image sens = A^T 1 (NULL: computed into the scratch). */
+int ct2_em_step(const ct2_desc* desc, const float* sino, const int8_t* ind, const float* fv,
+                float* vol, float* work, const float* sens, ct2_stream_t stream);
+/* ART over a view-geometry with descriptor options (interp, axis). */
+int ct2_art_step(const ct2_desc* desc, float reg_fac, const float* sino, const int8_t* ind,
+                 const float* fv, float* vol, ct2_stream_t stream);
+
+/* ------------------------------------ view_proj3D_tomo.f replacements -- */
+/* view_proj3D_tomo.f:4 sinoprojz (driving axis forced to z) */
+int ct2_sinoprojz(float* sinomat, const int8_t* indsino, const float* frame_vectors, int ns,
+                  int nu, int nv, float du, float dv, float u0, float v0, const float* smat,
+                  float dx, float dy, float dz, float x0, float y0, float z0, int nx, int ny,
+                  int nz, ct2_stream_t stream);
+/* view_proj3D_tomo.f:154 artstepz */
+int ct2_artstepz(float reg_fac, const float* sinomat, const int8_t* indsino,
+                 const float* frame_vectors, int ns, int nu, int nv, float du, float dv,
+                 float u0, float v0, float* smat, float dx, float dy, float dz, float x0,
+                 float y0, float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* view_proj3D_tomo.f:352 artstepz2vol (main + auxiliary grid, both in/out) */
+int ct2_artstepz2vol(float reg_fac, const float* sinomat, const int8_t* indsino,
+                     const float* frame_vectors, int ns, int nu, int nv, float du, float dv,
+                     float u0, float v0, float* smat, float dx, float dy, float dz, float x0,
+                     float y0, float z0, int nx, int ny, int nz, float* smata, float dxa,
+                     float dya, float dza, float x0a, float y0a, float z0a, int nxa, int nya,
+                     int nza, ct2_stream_t stream);
+/* view_proj3D_tomo.f:615 emstep_linearz */
+int ct2_emstep_linearz(const float* sinomat, const int8_t* indsino, const float* frame_vectors,
+                       int ns, int nu, int nv, float du, float dv, float u0, float v0,
+                       float* smat, float dx, float dy, float dz, float x0, float y0, float z0,
+                       int nx, int ny, int nz, ct2_stream_t stream);
+
+/* ------------------------------------------- ray_proj3D.f replacements -- */
+/* ray_proj3D.f:514 sinoproj_linear (ray list: ray_vectors[nr][6] = source, bin) */
+int ct2_ray_sinoproj_linear(float* sinomat, const int8_t* indsino, const float* ray_vectors,
+                            int nr, const float* smat, float dx, float dy, float dz, float x0,
+                            float y0, float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* ray_proj3D.f:677 sinoproj_nearest */
+int ct2_ray_sinoproj_nearest(float* sinomat, const int8_t* indsino, const float* ray_vectors,
+                             int nr, const float* smat, float dx, float dy, float dz, float x0,
+                             float y0, float z0, int nx, int ny, int nz, ct2_stream_t stream);
+/* ray_proj3D.f:809 sinobackproject_nearest (bounds 0 <= i < n-1, ray_proj3D.f:895) */
+int ct2_ray_sinobackproject_nearest(const float* sinomat, const int8_t* indsino,
+                                    const float* ray_vectors, int nr, float* smat, float dx,
+                                    float dy, float dz, float x0, float y0, float z0, int nx,
+                                    int ny, int nz, ct2_stream_t stream);
+/* ray_proj3D.f:987 artstep_linear */
+int ct2_ray_artstep_linear(float reg_fac, const float* sinomat, const int8_t* indsino,
+                           const float* ray_vectors, int nr, float* smat, float dx, float dy,
+                           float dz, float x0, float y0, float z0, int nx, int ny, int nz,
+                           ct2_stream_t stream);
+/* ray_proj3D.f:1272 artstep_nearest (bounds < n-1) */
+int ct2_ray_artstep_nearest(float reg_fac, const float* sinomat, const int8_t* indsino,
+                            const float* ray_vectors, int nr, float* smat, float dx, float dy,
+                            float dz, float x0, float y0, float z0, int nx, int ny, int nz,
+                            ct2_stream_t stream);
+/* ray_proj3D.f:1454 emstep_nearest (bounds < n-1); allocates like ct2_emstep_linear */
+int ct2_ray_emstep_nearest(const float* sinomat, const int8_t* indsino, const float* ray_vectors,
+                           int nr, float* smat, float dx, float dy, float dz, float x0,
+                           float y0, float z0, int nx, int ny, int nz, ct2_stream_t stream);
+
+/* ---------------------------------------------- totalvar.f replacements -- */
+/* totalvar.f:35 total_var3d; eps is the reference's 1e-9 unless changed.
+ * The value (float64) is written to the device pointer tv_out. */
+int ct2_total_var3d(float ambdala, const float* smat, int nx, int ny, int nz, float eps,
+                    double* tv_out, ct2_stream_t stream);
+/* totalvar.f:476 total_var_grad3d; the gradient norm bobo (float32, with the
+ * reference's |bobo| < 1e-5 -> 1 rule) is written to the device pointer bobo. */
+int ct2_total_var_grad3d(float ambdala, float* tvgmat, const float* smat, int nx, int ny,
+                         int nz, int normalizeq, float eps, float* bobo, ct2_stream_t stream);
+
+/* --------------------------------------------------- FDK.py replacements -- */
+/* FDK.py:10 fdk_sino_weight3D: out[p,j,k] = F/sqrt((F+D)^2 + vu[j,k]^2 + vv[j,k]^2)
+ * * sinomat[p,j,k] with vu[j,k] = u00+(j+uoff)*du, vv[j,k] = v00+(k+voff)*dv. */
+int ct2_fdk_sino_weight3d(float radius, float center2detector, int ns, int nu, int nv,
+                          float u00, float du, float uoff, float v00, float dv, float voff,
+                          const float* sinomat, float* out, ct2_stream_t stream);
+/* FDK.py:25 fdk_sino_filter3D, dim = 1 (filter along u) or 2 (along v):
+ * zero-pad to npad = 2^round(log2(2n-1)), multiply the FFT by the
+ * Kak-Slaney ramp (optionally Hann-windowed), crop.  Implemented as the
+ * equivalent real circulant matrix applied to every detector line. */
+int ct2_fdk_sino_filter3d(float dp, int dim, int ns, int nu, int nv, const float* sinomat,
+                          float* out, int hann, float L, ct2_stream_t stream);
+/* FDK.py:130 fdk_backproj_linear3D over the voxel grid
+ * x = x00+(i+xoff)dx, y = y00+(j+yoff)dy, z = z00+(k+zoff)dz, angles
+ * vs[p] = s0 + p*ds; recon[i][j][k] (C order over x, y, z). */
+int ct2_fdk_backproj_linear3d(int ns, int nu, int nv, float du, float dv, float u0, float v0,
+                              float radius, float source2detector, float uoffset,
+                              float voffset, double s0, double ds, int nx, int ny, int nz,
+                              float x00, float y00, float z00, float dx, float dy, float dz,
+                              float xoff, float yoff, float zoff, const float* sinomat,
+                              float* recon, ct2_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CT2_H */
