@@ -221,6 +221,17 @@ int ct2_ray_artstep_nearest(float reg_fac, const float* sinomat, const int8_t* i
     return launch_ray_art(&d, 1, reg_fac, sinomat, indsino, rv, nr, smat, (hipStream_t)stream);
 }
 
+// ---- SART ----
+int ct2_sart_step(const ct2_desc* d, float lambda, const float* sino, const int8_t* ind,
+                  const float* fv, float* vol, float* work, ct2_stream_t stream) {
+    if (!d) { set_error("ct2_sart_step: null descriptor"); return CT2_EINVAL; }
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
+    int rc;
+    if ((rc = check_grid(g)) || (rc = check_det(dt))) return rc;
+    return launch_sart(d, lambda, sino, ind, fv, vol, work, (hipStream_t)stream);
+}
+
 // ---- MLEM ----
 int ct2_em_step(const ct2_desc* d, const float* sino, const int8_t* ind, const float* fv,
                 float* vol, float* work, const float* sens, ct2_stream_t stream) {

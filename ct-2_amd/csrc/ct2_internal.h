@@ -219,6 +219,9 @@ int launch_em(const ct2_desc* d, const float* sino, const int8_t* ind, const flo
               float* vol, float* work, const float* sens, hipStream_t st);
 int launch_ray_em(const ct2_desc* d, int strict, const float* sino, const int8_t* ind,
                   const float* rv, int nr, float* vol, float* work, hipStream_t st);
+size_t sart_work_floats(const ct2_desc* d);
+int launch_sart(const ct2_desc* d, float lambda, const float* sino, const int8_t* ind,
+                const float* fv, float* vol, float* work, hipStream_t st);
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
                           hipStream_t st);
 // device fill (graph-capture safe, unlike a memset node)

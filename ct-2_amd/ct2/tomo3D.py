@@ -1,0 +1,343 @@
+"""Object model of the reference (emil_code/tomo3D.py), Python 3: the call
+sites of the reconstruction hot path.
+
+``sinogram3D``, ``image3D`` and ``linogram3D`` keep the reference's
+constructors, attributes (``mat``, ``indicator``, ``frame_vectors``,
+``ray_vectors``, ``ds/du/dv``, ``dx/dy/dz``...) and methods, and dispatch
+through the same algorithm-name dictionaries (emil_code/tomo3D.py:55-103) to
+the drop-in modules of this package.  ``mat`` may be a numpy array (each call
+round-trips to the device, as the reference round-trips through f2py) or a
+torch device tensor (everything stays in HBM); use ``to_device()``.
+Bad algorithm names raise ValueError (the reference calls ``sys.exit`` without
+importing ``sys``, emil_code/tomo3D.py:15, :440-441).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import FDK as _FDK  # noqa: F401  (the FDK call surface lives next to the model)
+from . import configs3D, ray_proj3D, totalvar, view_proj3D
+from . import view_proj3D_tomo as vpt
+
+pi = np.pi
+default_type = np.float32
+indicator_type = np.int8
+default_data_shape = 32, 128, 128
+default_image_shape = 128, 128, 128
+default_num_rays = 1000000
+
+bp_algs_dict = {
+    "adj_nearest": view_proj3D.sinobackproject_nearest,
+    "adj_linear": view_proj3D.sinobackproject_linear,
+    "pd_nearest": view_proj3D.pd_backproj_nearest,
+    "pd_linear": view_proj3D.pd_backproj_linear,
+}
+proj_algs_dict = {
+    "proj_linear": view_proj3D.sinoproj_linear,
+    "proj_nearest": view_proj3D.sinoproj_nearest,
+    "zproj_linear": vpt.sinoprojz,
+}
+art_algs_dict = {
+    "art_linear": view_proj3D.artstep_linear,
+    "art_nearest": view_proj3D.artstep_nearest,
+    "zart_linear": vpt.artstepz,
+}
+em_algs_dict = {
+    "em_linear": view_proj3D.emstep_linear,
+    "em_nearest": view_proj3D.emstep_nearest,
+    "zem_linear": vpt.emstep_linearz,
+}
+ray_bp_algs_dict = {"adj_nearest": ray_proj3D.sinobackproject_nearest}
+ray_proj_algs_dict = {"proj_nearest": ray_proj3D.sinoproj_nearest,
+                      "proj_linear": ray_proj3D.sinoproj_linear}
+ray_art_algs_dict = {"art_nearest": ray_proj3D.artstep_nearest,
+                     "art_linear": ray_proj3D.artstep_linear}
+ray_em_algs_dict = {"em_nearest": ray_proj3D.emstep_nearest}
+
+
+def _pick(table, alg, what):
+    if alg not in table:
+        raise ValueError(f"bad algorithm choice for {what}: {alg!r} (choose from {sorted(table)})")
+    return table[alg]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_tensor(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _xp_sum_sq(a, b, w=None):
+    """sqrt(sum(w*(a-b)^2)) in float64 for numpy or torch."""
+    if _is_tensor(a) or _is_tensor(b):
+        torch = _torch()
+        a = torch.as_tensor(a).to(dtype=torch.float64)
+        b = torch.as_tensor(b, device=a.device).to(dtype=torch.float64)
+        d = (a - b) ** 2
+        if w is not None:
+            d = d * torch.as_tensor(w, device=a.device).to(torch.float64)
+        return float(torch.sqrt(d.sum()))
+    d = (np.asarray(a, np.float64).ravel() - np.asarray(b, np.float64).ravel()) ** 2
+    if w is not None:
+        d = d * np.asarray(w, np.float64).ravel()
+    return float(np.sqrt(d.sum(dtype=np.float64)))
+
+
+class sinogram3D:
+    """3D sinogram array (emil_code/tomo3D.py:110-476)."""
+
+    def __init__(self, config_name='circular_conebeam',
+                 parms={"source_to_detector": 10., "radius": 5.},
+                 shape=default_data_shape, slen=pi, ulen=4.0, vlen=4.0,
+                 s0=0., u0=-2.0, v0=-2.0, s_include_endpoints=1):
+        self.ns, self.nu, self.nv = shape
+        self.mat = np.zeros(shape, default_type)
+        self.indicator = np.ones(shape, indicator_type)
+        self.config_name = config_name
+        self.parms = parms
+        self.frame = configs3D.configs[config_name]
+        self.slen, self.ulen, self.vlen = slen, ulen, vlen
+        self.s_include_endpoints = s_include_endpoints
+        self.s0, self.u0, self.v0 = s0, u0, v0
+        self.ds = self.get_ds()
+        self.du = self.get_du()
+        self.dv = self.get_dv()
+        self.frame_vectors = configs3D.frame_vectors(config_name, parms, self.ns, slen, s0,
+                                                     s_include_endpoints)
+
+    def get_ds(self):
+        return self.slen / (self.ns - 1.) if self.s_include_endpoints else self.slen / self.ns
+
+    def get_du(self):
+        return self.ulen / self.nu
+
+    def get_dv(self):
+        return self.vlen / self.nv
+
+    def get_parms(self):
+        return self.parms
+
+    def frame_vecs(self, s):
+        c = self.frame(s, self.parms)
+        return (np.array(c[0:3]), np.array(c[3:6]), np.array(c[6:9]), np.array(c[9:12]),
+                np.array(c[12:15]), np.array(c[15:18]))
+
+    def duplicate(self):
+        new = sinogram3D(self.config_name, self.parms, (self.ns, self.nu, self.nv), self.slen,
+                         self.ulen, self.vlen, self.s0, self.u0, self.v0, self.s_include_endpoints)
+        new.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
+        new.indicator = self.indicator.clone() if _is_tensor(self.indicator) else self.indicator.copy()
+        new.frame_vectors = self.frame_vectors
+        return new
+
+    def to_device(self):
+        """Move mat, indicator and frame vectors to the current HIP device."""
+        torch = _torch()
+        from . import _dev
+        self.mat = _dev.to_dev(self.mat, torch.float32, (self.ns, self.nu, self.nv))
+        self.indicator = _dev.ind_dev(self.indicator, (self.ns, self.nu, self.nv))
+        self.frame_vectors = _dev.to_dev(self.frame_vectors, torch.float32, (self.ns, 18))
+        return self
+
+    def clear(self):
+        self.mat *= 0.
+
+    def dist_to(self, sino):
+        """emil_code/tomo3D.py:319-324 (float64 norm over used rays)."""
+        return _xp_sum_sq(self.mat, sino.mat, self.indicator)
+
+    def copy_to(self, dest):
+        dest.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
+        dest.indicator = self.indicator.clone() if _is_tensor(self.indicator) else self.indicator.copy()
+
+    def add_poisson_noise(self, photon_num_per_bin=1000, seed=None):
+        """Poisson counts I ~ P(N e^-p), p_hat = -ln(max(I,1)/N).  The reference
+        uses an unseeded Gaussian approximation (emil_code/tomo3D.py:368-382);
+        this draws true Poisson counts from a seeded generator."""
+        rng = np.random.default_rng(seed)
+        m = self.mat.cpu().numpy() if _is_tensor(self.mat) else self.mat
+        I = rng.poisson(photon_num_per_bin * np.exp(-np.asarray(m, np.float64)))
+        p = -np.log(np.maximum(I, 1) / photon_num_per_bin).astype(default_type)
+        if _is_tensor(self.mat):
+            self.mat = _torch().from_numpy(p).to(self.mat.device)
+        else:
+            self.mat = p
+
+    def backproject_to(self, image, alg='adj_linear'):
+        """emil_code/tomo3D.py:432-472."""
+        bp = _pick(bp_algs_dict, alg, "backprojection")
+        image.mat = bp(self.mat, self.indicator, self.frame_vectors, self.ns, self.nu, self.nv,
+                       self.du, self.dv, self.u0, self.v0, image.dx, image.dy, image.dz,
+                       image.x0, image.y0, image.z0, image.nx, image.ny, image.nz)
+
+
+class linogram3D:
+    """Ray-list data (emil_code/tomo3D.py:541-645)."""
+
+    def __init__(self, config_name='ball', parms={"source_radius": 1., "det_radius": 1.},
+                 nrays=default_num_rays, is_duplicate=0):
+        self.mat = np.zeros([nrays], default_type)
+        self.config_name = config_name
+        self.parms = parms
+        self.nr = nrays
+        self.frame = configs3D.lino_configs.get(config_name)
+        if not is_duplicate:
+            self.ray_vectors = np.zeros((self.nr, 6), default_type)
+            self.indicator = np.ones([nrays], indicator_type)
+
+    def duplicate(self):
+        new = linogram3D(self.config_name, dict(self.parms), self.nr, is_duplicate=1)
+        new.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
+        new.indicator = self.indicator
+        new.ray_vectors = self.ray_vectors
+        return new
+
+    def frame_vecs(self, s):
+        c = self.frame(s, self.parms)
+        return np.array(c[0:3]), np.array(c[3:6])
+
+    def dist_to(self, lino):
+        return _xp_sum_sq(self.mat, lino.mat)
+
+    def copy_to(self, dest):
+        dest.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
+        dest.indicator = self.indicator.clone() if _is_tensor(self.indicator) else self.indicator.copy()
+
+    def backproject_to(self, image):
+        """emil_code/tomo3D.py:620-645: adjoint of nearest ray projection."""
+        image.mat = ray_proj3D.sinobackproject_nearest(
+            self.mat, self.indicator, self.ray_vectors, self.nr, image.dx, image.dy, image.dz,
+            image.x0, image.y0, image.z0, image.nx, image.ny, image.nz)
+
+
+class image3D:
+    """3D image array (emil_code/tomo3D.py:784-1500)."""
+
+    def __init__(self, shape=default_image_shape, xlen=2.0, ylen=2.0, zlen=2.0,
+                 x0=-1.0, y0=-1.0, z0=-1.0, data_type='real'):
+        self.mat = np.zeros(shape, default_type)
+        self.indicator = np.ones(shape, indicator_type)
+        self.nx, self.ny, self.nz = shape
+        self.x0, self.y0, self.z0 = x0, y0, z0
+        self.xlen, self.ylen, self.zlen = xlen, ylen, zlen
+        self.dx = self.get_dx()
+        self.dy = self.get_dy()
+        self.dz = self.get_dz()
+
+    def get_dx(self):
+        return self.xlen / self.nx
+
+    def get_dy(self):
+        return self.ylen / self.ny
+
+    def get_dz(self):
+        return self.zlen / self.nz
+
+    def _grid(self):
+        return (self.dx, self.dy, self.dz, self.x0, self.y0, self.z0, self.nx, self.ny, self.nz)
+
+    def duplicate(self):
+        new = image3D((self.nx, self.ny, self.nz), self.xlen, self.ylen, self.zlen,
+                      self.x0, self.y0, self.z0)
+        new.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
+        return new
+
+    def to_device(self):
+        torch = _torch()
+        from . import _dev
+        self.mat = _dev.to_dev(self.mat, torch.float32, (self.nx, self.ny, self.nz))
+        return self
+
+    def copy_to(self, dest):
+        dest.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
+
+    def clear(self):
+        self.mat *= 0.
+
+    def fill(self, v):
+        self.mat[...] = v
+
+    def make_positive(self):
+        """emil_code/tomo3D.py:902-903."""
+        if _is_tensor(self.mat):
+            self.mat.clamp_(min=0.0)
+        else:
+            self.mat[self.mat < 0.] = 0.
+
+    def make_positive_softly(self, beta=1.0):
+        self.mat[self.mat < 0.] *= (1.0 - beta)
+
+    def zero_edges(self):
+        """Two-voxel border set to zero (emil_code/tomo3D.py:928-943)."""
+        m = self.mat
+        for sl in (np.s_[0:2, :, :], np.s_[-2:, :, :], np.s_[:, 0:2, :], np.s_[:, -2:, :],
+                   np.s_[:, :, 0:2], np.s_[:, :, -2:]):
+            m[sl] = 0.
+
+    def clamp_edges(self):
+        """emil_code/tomo3D.py:907-925."""
+        m = self.mat
+        m[0], m[1], m[-1], m[-2] = m[2] * 1., m[2] * 1., m[-3] * 1., m[-3] * 1.
+        m[:, 0], m[:, 1], m[:, -1], m[:, -2] = m[:, 2] * 1., m[:, 2] * 1., m[:, -3] * 1., m[:, -3] * 1.
+        m[:, :, 0], m[:, :, 1] = m[:, :, 2] * 1., m[:, :, 2] * 1.
+        m[:, :, -1], m[:, :, -2] = m[:, :, -3] * 1., m[:, :, -3] * 1.
+
+    def dist_to(self, image):
+        """emil_code/tomo3D.py:946-951 (float64)."""
+        return _xp_sum_sq(self.mat, image.mat)
+
+    def calc_total_var(self, lam1=0.):
+        """emil_code/tomo3D.py:953-958."""
+        return float(totalvar.total_var3d(lam1, self.mat, self.nx, self.ny, self.nz))
+
+    def tv_grad(self, lam1=0., normalizeq=1):
+        """emil_code/tomo3D.py:1031-1036: mat <- TV gradient, returns its norm."""
+        self.mat, bobo = totalvar.total_var_grad3d(lam1, self.mat, self.nx, self.ny, self.nz,
+                                                   normalizeq)
+        return bobo
+
+    def scale(self, factor):
+        self.mat = factor * self.mat
+
+    def project_to(self, sino, alg='proj_linear'):
+        """emil_code/tomo3D.py:1127-1168."""
+        proj = _pick(proj_algs_dict, alg, "projection")
+        sino.mat = proj(sino.indicator, sino.frame_vectors, sino.ns, sino.nu, sino.nv, sino.du,
+                        sino.dv, sino.u0, sino.v0, self.mat, *self._grid())
+
+    def lino_project_to(self, lino, alg='proj_linear'):
+        """emil_code/tomo3D.py:1172-1195."""
+        proj = _pick(ray_proj_algs_dict, alg, "projection")
+        lino.mat = proj(lino.indicator, lino.ray_vectors, lino.nr, self.mat, *self._grid())
+
+    def art_step(self, sino, reg_fac=1.0, alg='art_linear'):
+        """emil_code/tomo3D.py:1235-1277."""
+        art = _pick(art_algs_dict, alg, "ART")
+        self.mat = art(reg_fac, sino.mat, sino.indicator, sino.frame_vectors, sino.ns, sino.nu,
+                       sino.nv, sino.du, sino.dv, sino.u0, sino.v0, self.mat, *self._grid())
+
+    def tomo_art_step_2vol(self, iaux, sino, reg_fac=1.0):
+        """emil_code/tomo3D.py:1282-1325."""
+        self.mat, iaux.mat = vpt.artstepz2vol(
+            reg_fac, sino.mat, sino.indicator, sino.frame_vectors, sino.ns, sino.nu, sino.nv,
+            sino.du, sino.dv, sino.u0, sino.v0, self.mat, *self._grid(), iaux.mat, *iaux._grid())
+
+    def lino_art_step(self, lino, reg_fac=1.0, alg='art_linear'):
+        """emil_code/tomo3D.py:1329-1353."""
+        art = _pick(ray_art_algs_dict, alg, "ART")
+        self.mat = art(reg_fac, lino.mat, lino.indicator, lino.ray_vectors, lino.nr, self.mat,
+                       *self._grid())
+
+    def em_step(self, sino, alg='em_nearest'):
+        """emil_code/tomo3D.py:1394-1435."""
+        em = _pick(em_algs_dict, alg, "EM")
+        self.mat = em(sino.mat, sino.indicator, sino.frame_vectors, sino.ns, sino.nu, sino.nv,
+                      sino.du, sino.dv, sino.u0, sino.v0, self.mat, *self._grid())
+
+    def lino_em_step(self, lino):
+        """emil_code/tomo3D.py:1476-1499."""
+        self.mat = ray_proj3D.emstep_nearest(lino.mat, lino.indicator, lino.ray_vectors, lino.nr,
+                                             self.mat, *self._grid())

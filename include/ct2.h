@@ -73,8 +73,9 @@ int ct2_project(const ct2_desc* desc, float* sino, const int8_t* ind, const floa
  * (storage desc->vol_dtype). */
 int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
                     const float* fv, ct2_stream_t stream);
-/* SART update over the view subset of desc: x += lambda * A^T((b-Ax)/A1) / A^T1
- * restricted to those views, with work = 2*ns*nu*nv + 2*nx*ny*nz floats of
+/* SART update over the view subset of desc (no counterpart in the reference,
+ * which has only sequential ART): x += lambda * A^T((b-Ax)/A1) / A^T1
+ * restricted to those views, with work = 2*ns*nu*nv + 3*nx*ny*nz floats of
  * device scratch supplied by the caller. */
 int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const int8_t* ind,
                   const float* fv, float* vol, float* work, ct2_stream_t stream);
