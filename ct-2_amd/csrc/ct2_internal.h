@@ -160,6 +160,94 @@ __device__ __forceinline__ bool sample_nearest_exact(const AxisSel& A, int l, bo
     return true;
 }
 
+// ------------------------------------------------------ fast arithmetic ----
+// Per-ray setup in FP64 (bin position, driving axis, slopes), then the
+// transverse cell coordinate of plane l is one FP32 FMA, u(l) = u0 + l*du,
+// with u = (p-o)/d - 0.5 (linear) or (p-o)/d (nearest).  The forward kernel
+// and the adjoint call this same function, so their matrix entries agree.
+struct RayFast {
+    int a, b, c;
+    int na, nb, nc;
+    int sa, sb, sc;
+    float ub0, dub, uc0, duc;
+    float trav;
+};
+
+__device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], const double B[3],
+                                           bool zonly, bool linear, RayFast& R) {
+    const double dd[3] = {B[0] - S[0], B[1] - S[1], B[2] - S[2]};
+    const double inv[3] = {1.0 / (double)g.d[0], 1.0 / (double)g.d[1], 1.0 / (double)g.d[2]};
+    int a;
+    if (zonly) {
+        a = 2;
+    } else {
+        double xad = fabs(dd[0]) * inv[0], yad = fabs(dd[1]) * inv[1], zad = fabs(dd[2]) * inv[2];
+        a = ((xad > yad) && (xad > zad)) ? 0 : ((yad > zad) ? 1 : 2);
+    }
+    const int b = (a == 0) ? 1 : 0;
+    const int c = (a == 2) ? 1 : 2;
+    const double da = (a == 0) ? g.d[0] : (a == 1 ? g.d[1] : g.d[2]);
+    const double oa = (a == 0) ? g.o[0] : (a == 1 ? g.o[1] : g.o[2]);
+    const double ob = (b == 0) ? g.o[0] : g.o[1];
+    const double oc = (c == 1) ? g.o[1] : g.o[2];
+    const double ib = (b == 0) ? inv[0] : inv[1];
+    const double ic = (c == 1) ? inv[1] : inv[2];
+    const double Sa = (a == 0) ? S[0] : (a == 1 ? S[1] : S[2]);
+    const double Sb = (b == 0) ? S[0] : S[1];
+    const double Sc = (c == 1) ? S[1] : S[2];
+    const double ddb = (b == 0) ? dd[0] : dd[1];
+    const double ddc = (c == 1) ? dd[1] : dd[2];
+    const double dda = (a == 0) ? dd[0] : (a == 1 ? dd[1] : dd[2]);
+    const double ia = 1.0 / dda;
+    const double r1 = ddb * ia, r2 = ddc * ia;
+    const double t0 = oa + 0.5 * da;   // centre of plane 0
+    const double half = linear ? 0.5 : 0.0;
+    R.ub0 = (float)((Sb + r1 * (t0 - Sa) - ob) * ib - half);
+    R.uc0 = (float)((Sc + r2 * (t0 - Sa) - oc) * ic - half);
+    R.dub = (float)(r1 * da * ib);
+    R.duc = (float)(r2 * da * ic);
+    R.trav = (float)(da * sqrt(1.0 + r1 * r1 + r2 * r2));
+    R.a = a; R.b = b; R.c = c;
+    R.na = sel3i(a, g.n[0], g.n[1], g.n[2]);
+    R.nb = sel3i(b, g.n[0], g.n[1], g.n[2]);
+    R.nc = sel3i(c, g.n[0], g.n[1], g.n[2]);
+    R.sa = sel3i(a, g.s[0], g.s[1], g.s[2]);
+    R.sb = sel3i(b, g.s[0], g.s[1], g.s[2]);
+    R.sc = sel3i(c, g.s[0], g.s[1], g.s[2]);
+}
+
+// FP64 source and bin of view ray (p,j,k) for the fast setup.
+__device__ __forceinline__ void view_ray_f64(const float* __restrict__ fv, int p, int j, int k,
+                                             const Det& dt, double S[3], double B[3]) {
+    const float* f = fv + (size_t)p * 18;
+    const double u = (double)dt.u0 + ((double)j + 0.5) * (double)dt.du;
+    const double v = (double)dt.v0 + ((double)k + 0.5) * (double)dt.dv;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        S[q] = f[q];
+        B[q] = (double)f[6 + q] + (double)f[9 + q] * u + (double)f[12 + q] * v;
+    }
+}
+
+// Restrict the inclusive plane range [l0,l1] to planes where the cell
+// coordinate u0 + l*du may lie in [lo, hi); the interval is widened by `pad`
+// cells and one plane so FP32 rounding of the exact per-sample test never
+// falls outside it.
+__device__ __forceinline__ void clip_planes(double u0, double du, double lo, double hi, double pad,
+                                            int& l0, int& l1) {
+    lo -= pad;
+    hi += pad;
+    if (fabs(du) < 1e-12) {
+        if (!(u0 >= lo && u0 < hi)) l1 = l0 - 1;
+        return;
+    }
+    double ta = (lo - u0) / du, tb = (hi - u0) / du;
+    double tmin = fmax(fmin(ta, tb), -1e9), tmax = fmin(fmax(ta, tb), 1e9);
+    if (!(tmin <= tmax)) { l1 = l0 - 1; return; }
+    l0 = max(l0, (int)floor(tmin) - 1);
+    l1 = min(l1, (int)ceil(tmax) + 1);
+}
+
 // ------------------------------------------------------- storage types ----
 template <typename T> struct Store;
 template <> struct Store<float> {

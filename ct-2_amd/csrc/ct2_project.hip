@@ -42,46 +42,15 @@ __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restr
 
 // Fast arithmetic: per-ray setup in FP64, the transverse cell coordinate of
 // plane l is one FMA (u(l) = u0 + l*du), FP32 bilinear lerps, compensated sum.
-struct RayFast {
-    int na, sa, sb, sc, nb, nc;
-    float ub0, dub, uc0, duc;   // cell coordinate (p-o)/d - 0.5 (linear) or (p-o)/d (nearest)
-    float trav;
-};
-
-__device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], const double B[3],
-                                           bool zonly, bool linear, RayFast& R) {
-    double dd[3] = {B[0] - S[0], B[1] - S[1], B[2] - S[2]};
-    int a;
-    if (zonly) {
-        a = 2;
-    } else {
-        double xad = fabs(dd[0]) / g.d[0], yad = fabs(dd[1]) / g.d[1], zad = fabs(dd[2]) / g.d[2];
-        a = ((xad > yad) && (xad > zad)) ? 0 : ((yad > zad) ? 1 : 2);
-    }
-    int b = (a == 0) ? 1 : 0;
-    int c = (a == 2) ? 1 : 2;
-    double da = g.d[a], db = g.d[b], dc = g.d[c];
-    double r1 = dd[b] / dd[a], r2 = dd[c] / dd[a];
-    double t0 = (double)g.o[a] + 0.5 * da;   // centre of plane 0
-    double half = linear ? 0.5 : 0.0;
-    double pb0 = S[b] + r1 * (t0 - S[a]);
-    double pc0 = S[c] + r2 * (t0 - S[a]);
-    R.ub0 = (float)((pb0 - g.o[b]) / db - half);
-    R.uc0 = (float)((pc0 - g.o[c]) / dc - half);
-    R.dub = (float)(r1 * da / db);
-    R.duc = (float)(r2 * da / dc);
-    R.trav = (float)(da * sqrt(1.0 + r1 * r1 + r2 * r2));
-    R.na = g.n[a]; R.nb = g.n[b]; R.nc = g.n[c];
-    R.sa = g.s[a]; R.sb = g.s[b]; R.sc = g.s[c];
-}
-
 template <int INTERP, typename VT>
 __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restrict__ vol,
-                                            bool strict) {
+                                            bool strict, int l0, int l1) {
     float sum = 0.0f, comp = 0.0f;
     const unsigned hb = (unsigned)(INTERP == CT2_LINEAR ? R.nb - 1 : (strict ? R.nb - 1 : R.nb));
     const unsigned hc = (unsigned)(INTERP == CT2_LINEAR ? R.nc - 1 : (strict ? R.nc - 1 : R.nc));
-    for (int l = 0; l < R.na; ++l) {
+    // [l0, l1] is wavefront-uniform: all lanes step the same plane together,
+    // so one load instruction touches one plane's neighbourhood
+    for (int l = l0; l <= l1; ++l) {
         float ub = __builtin_fmaf((float)l, R.dub, R.ub0);
         float uc = __builtin_fmaf((float)l, R.duc, R.uc0);
         float fb = floorf(ub), fc = floorf(uc);
@@ -119,32 +88,42 @@ __global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int p = view0 + blockIdx.z;
-    if (k >= dt.nv || j >= dt.nu) return;
+    const bool active = k < dt.nv && j < dt.nu;
     const size_t r = ((size_t)p * dt.nu + j) * dt.nv + k;
+    const bool use = active && (ind == nullptr || ind[r] == 1);
     float out = 0.0f;
-    if (ind == nullptr || ind[r] == 1) {
-        if (ARITH == CT2_ARITH_EXACT) {
+    if (ARITH == CT2_ARITH_EXACT) {
+        if (use) {
             float S[3], B[3];
             view_ray(fv, p, j, k, dt, S, B);
             AxisSel A;
             setup_exact(g, S, B, zonly != 0, A);
             out = march_exact<INTERP, VT>(A, vol, false);
-        } else {
-            const float* f = fv + (size_t)p * 18;
-            double u = (double)dt.u0 + ((double)j + 0.5) * dt.du;
-            double v = (double)dt.v0 + ((double)k + 0.5) * dt.dv;
-            double S[3], B[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                S[q] = f[q];
-                B[q] = (double)f[6 + q] + (double)f[9 + q] * u + (double)f[12 + q] * v;
-            }
-            RayFast R;
-            setup_fast(g, S, B, zonly != 0, INTERP == CT2_LINEAR, R);
-            out = march_fast<INTERP, VT>(R, vol, false);
         }
+    } else {
+        RayFast R;
+        int l0 = 0x7fffffff, l1 = -1;
+        if (use) {
+            double S[3], B[3];
+            view_ray_f64(fv, p, j, k, dt, S, B);
+            setup_fast(g, S, B, zonly != 0, INTERP == CT2_LINEAR, R);
+            // planes where the ray can be inside the volume
+            int a0 = 0, a1 = R.na - 1;
+            const double hb = INTERP == CT2_LINEAR ? R.nb - 1 : R.nb;
+            const double hc = INTERP == CT2_LINEAR ? R.nc - 1 : R.nc;
+            clip_planes(R.ub0, R.dub, 0.0, hb, 0.05, a0, a1);
+            clip_planes(R.uc0, R.duc, 0.0, hc, 0.05, a0, a1);
+            if (a0 <= a1) { l0 = a0; l1 = a1; }
+        }
+        // union over the wavefront (every lane takes part in the shuffles)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            l0 = min(l0, __shfl_xor(l0, off, 64));
+            l1 = max(l1, __shfl_xor(l1, off, 64));
+        }
+        if (use) out = march_fast<INTERP, VT>(R, vol, false, l0, min(l1, R.na - 1));
     }
-    sino[r] = out;
+    if (active) sino[r] = out;
 }
 
 template <int INTERP>
