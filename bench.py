@@ -1,0 +1,278 @@
+"""Headline benchmark: giga voxel-ray updates per second of one Joseph
+forward projection plus one exact-adjoint backprojection of a 256^3 volume,
+360 views, 512x512 detector, with 16-bit (FP16) volume storage and FP32
+accumulation ("mixed"), next to the plain FP32 path, on one MI355X.
+
+Workload (one step = fwd + bp over all views, inputs resident in HBM):
+  volume  256^3, 0.8 mm voxels, 40 seeded random ellipsoids (seed 20260101)
+  scan    circular cone beam, 360 views over 360 deg, source-isocentre 500 mm,
+          source-detector 1000 mm, 512x512 detector at 0.8 mm
+  updates N = 2 * 360 * 512 * 512 * 256 = 4.832e10 per step
+Also reported: the relative L2 error of the mixed and FP32 paths against the
+FP64 oracle on a subset of views, the dominant kernel's roofline fraction
+(algorithmic bytes 4 * bytes-per-voxel per update over the MI355X HBM peak),
+and the reference path (the reference Fortran compiled from its sources,
+1 thread) timed on a bounded sample of the same workload.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "ct-2_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = ("Gvox-ray updates/s (fwd+bp, 256³, 360 views), mixed vs FP32 at "
+          "≤1e-4 rel err")
+HBM_PEAK_GBS = 8000.0
+N, NS, ND = 256, 360, 512
+VOX, R, SDD, PITCH = 0.8, 500.0, 1000.0, 0.8
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def geometry():
+    import numpy as np
+    from ct2 import configs3D
+    fv = configs3D.frame_vectors("circular_conebeam", {"radius": R, "source_to_detector": SDD},
+                                 NS, 2 * np.pi, 0.0, 0)
+    hw = N * VOX / 2
+    geo = (NS, ND, ND, PITCH, PITCH, -ND * PITCH / 2, -ND * PITCH / 2)
+    grid = (VOX, VOX, VOX, -hw, -hw, -hw, N, N, N)
+    return fv, geo, grid
+
+
+def phantom():
+    from ct2 import phantoms
+    return phantoms.random_ellipsoids(N, N * VOX / 2, 40, 20260101)
+
+
+def make_op(fv, geo, grid, arith):
+    from ct2.operator import ConeBeamOperator
+    ns, nu, nv, du, dv, u0, v0 = geo
+    dx, dy, dz, x0, y0, z0, nx, ny, nz = grid
+    return ConeBeamOperator(fv, ns, nu, nv, du, dv, u0, v0, (nx, ny, nz), (dx, dy, dz),
+                            (x0, y0, z0), arith=arith)
+
+
+def to_storage(vol32, vol16, stochastic, seed=20260107):
+    """FP32 volume -> FP16 storage through the library's conversion kernel."""
+    import ctypes as C
+    from ct2 import _dev
+    from ct2._lib import CT2_F16, CT2_ROUND_NEAREST, CT2_ROUND_STOCHASTIC, call
+    call("ct2_convert_volume", _dev.ptr(vol32), _dev.ptr(vol16), CT2_F16, C.c_size_t(vol32.numel()),
+         CT2_ROUND_STOCHASTIC if stochastic else CT2_ROUND_NEAREST, seed, _dev.stream_handle())
+    return vol16
+
+
+def time_path(op, vol32, vol_store, sino, recon, steps, warmup):
+    """Time `steps` steps of [FP32 -> storage conversion (mixed only), forward,
+    adjoint]; per-kernel durations from HIP events on the stream the kernels
+    run on (torch's current stream)."""
+    import torch
+
+    def step(ev=None):
+        if ev: ev[0].record()
+        if vol_store is not None:
+            to_storage(vol32, vol_store, stochastic=True)
+        v = vol_store if vol_store is not None else vol32
+        if ev: ev[1].record()
+        op.forward(v, out=sino)
+        if ev: ev[2].record()
+        op.adjoint(sino, out=recon)
+        if ev: ev[3].record()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s in range(steps):
+        step(ev[s])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_cvt = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    t_fwd = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    t_bp = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
+    return wall, t_cvt, t_fwd, t_bp
+
+
+def precision(fv, geo, grid, vol_np, sino_gpu, op, check_views):
+    """Relative L2 of the GPU forward (all views, compared on a subset) and
+    of the GPU adjoint of that subset, against the FP64 oracle."""
+    import numpy as np
+    import torch
+    import oracle
+    o64 = oracle.restated_raw("f64")
+    ns, nu, nv = geo[:3]
+    sub = list(range(0, ns, ns // check_views))[:check_views]
+    fv_sub = np.ascontiguousarray(fv[sub])
+    geo_sub = (len(sub),) + tuple(geo[1:])
+    ref = o64.sinoproj(oracle.LINEAR, 0, None, fv_sub, *geo_sub, vol_np, *grid)
+    got = sino_gpu[sub].cpu().numpy().astype(np.float64)
+    e_fwd = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    # adjoint of the FP64 sinogram subset, one view at a time on both sides
+    y = ref.astype(np.float32)
+    refb = o64.sinobackproject(oracle.LINEAR, 0, y, None, fv_sub, *geo_sub, *grid)
+    ysub = torch.zeros(op.sino_shape, dtype=torch.float32, device=op.device)
+    for i, p in enumerate(sub):
+        ysub[p] = torch.from_numpy(y[i]).to(op.device)
+    gotb = op.adjoint(ysub).cpu().numpy().astype(np.float64)
+    e_bp = float(np.linalg.norm(gotb - refb) / np.linalg.norm(refb))
+    return e_fwd, e_bp, len(sub)
+
+
+def cpu_baseline(fv, geo, grid, vol_np, views=8):
+    """The reference path timed on the host: sinoproj_linear +
+    sinobackproject_linear of the reference Fortran (oracle/_ref, compiled
+    from the reference's own sources) on `views` views of the workload,
+    1 thread.  Falls back to the FP32 C restatement (kind "port") when the
+    compiled reference is not present."""
+    import numpy as np
+    import oracle
+    ns, nu, nv = geo[:3]
+    sub = list(range(0, ns, ns // views))[:views]
+    fv_sub = np.ascontiguousarray(fv[sub])
+    geo_sub = (len(sub),) + tuple(geo[1:])
+    if oracle.ref_available():
+        mods, kind = oracle.reference(), "reference"
+    else:
+        mods, kind = oracle.restated("f32"), "port"
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    t0 = time.perf_counter()
+    s = mods.view_proj3D.sinoproj_linear(None, fv_sub, *geo_sub, vol_np, *grid)
+    t1 = time.perf_counter()
+    mods.view_proj3D.sinobackproject_linear(s, None, fv_sub, *geo_sub, *grid)
+    t2 = time.perf_counter()
+    nupd = 2 * len(sub) * nu * nv * N
+    return {"value": round(nupd / (t2 - t0) / 1e9, 6), "unit": "Gvox-ray updates/s",
+            "cores": 1, "kind": kind,
+            "sample": (f"{len(sub)} of {ns} views (fwd {t1 - t0:.1f} s + bp {t2 - t1:.1f} s), "
+                       f"{N}^3 volume, {nu}x{nv} detector, single thread"
+                       + (", reference Fortran compiled by amdflang -O2 from emil_code/"
+                          "view_proj3D.f, called through ctypes" if kind == "reference" else
+                          ", FP32 C restatement of the reference"))}
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary, if present."""
+    p = os.path.join(ROOT, "profiles", "bp_brick_fast_c2_hbm_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 precision check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU sharding is design-only: views for A, a z-slab reduce-scatter "
+              "for A^T)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    fv, geo, grid = geometry()
+    log("building phantom")
+    vol_np = phantom()
+    vol32 = torch.from_numpy(vol_np).cuda()
+    vol16 = vol32.half()
+    op_mixed = make_op(fv, geo, grid, "fast")
+    op_fp32 = make_op(fv, geo, grid, "exact")
+    sino = torch.empty(op_mixed.sino_shape, dtype=torch.float32, device="cuda")
+    recon = torch.empty(op_mixed.shape, dtype=torch.float32, device="cuda")
+    nupd = 2 * op_mixed.updates_per_direction()
+
+    log("timing mixed path (FP16 storage with stochastic rounding, fast arithmetic)")
+    wall_m, tc_m, tf_m, tb_m = time_path(op_mixed, vol32, vol16, sino, recon, args.steps,
+                                         args.warmup)
+    log(f"  convert {tc_m:.3f} ms  fwd {tf_m:.2f} ms  bp {tb_m:.2f} ms  "
+        f"wall/step {1e3 * wall_m / args.steps:.2f} ms")
+    prec = {}
+    if not args.no_check:
+        log("precision vs FP64 oracle (mixed)")
+        e_fwd, e_bp, nchk = precision(fv, geo, grid, vol_np, sino, op_mixed, 4)
+        prec["mixed"] = {"fwd": e_fwd, "bp": e_bp}
+        # the same forward with round-to-nearest FP16 storage, for comparison
+        to_storage(vol32, vol16, stochastic=False)
+        op_mixed.forward(vol16, out=sino)
+        e_near, _, _ = precision(fv, geo, grid, vol_np, sino, op_mixed, 4)
+        prec["mixed_round_nearest_fwd"] = e_near
+    log("timing FP32 path (FP32 storage, reference operation order)")
+    sino32 = torch.empty_like(sino)
+    wall_f, _, tf_f, tb_f = time_path(op_fp32, vol32, None, sino32, recon,
+                                      max(1, args.steps // 2), 1)
+    log(f"  fwd {tf_f:.2f} ms  bp {tb_f:.2f} ms")
+    if not args.no_check:
+        log("precision vs FP64 oracle (FP32)")
+        e_fwd, e_bp, nchk = precision(fv, geo, grid, vol_np, sino32, op_fp32, 4)
+        prec["fp32"] = {"fwd": e_fwd, "bp": e_bp}
+
+    value = nupd / (wall_m / args.steps) / 1e9
+    fp32_value = nupd / (wall_f / max(1, args.steps // 2)) / 1e9
+    # dominant kernel of the mixed step and its roofline
+    if tb_m >= tf_m:
+        kname, t_k, bpv = "k_backproject_brick<linear,fast,f32>", tb_m, 4  # f32 volume written
+    else:
+        kname, t_k, bpv = "k_project_views<linear,fast,f16>", tf_m, 2      # f16 volume read
+    alg_bytes = (nupd / 2) * 4 * bpv
+    achieved = alg_bytes / (t_k * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvox-ray updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall_m / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": "f16 volume (forward), f32 sinogram, f32 volume out (adjoint)",
+        "data": "synthetic: seeded 40-ellipsoid phantom, views of a circular cone-beam scan",
+        "config": {"workload": "Joseph forward + exact-adjoint backprojection, 256^3 volume "
+                               "(0.8 mm), 360 views, 512x512 detector (0.8 mm), R=500 mm, "
+                               "SDD=1000 mm",
+                   "updates_per_step": nupd},
+        "kernels_ms": {"convert_mixed": round(tc_m, 4), "fwd_mixed": round(tf_m, 3),
+                       "bp_mixed": round(tb_m, 3),
+                       "fwd_fp32": round(tf_f, 3), "bp_fp32": round(tb_f, 3)},
+        "fp32_value": round(fp32_value, 3),
+        "mixed_over_fp32": round(value / fp32_value, 3),
+        "rel_l2_vs_fp64": prec or None,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic()},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (reference path, bounded sample)")
+        line["cpu_baseline"] = cpu_baseline(fv, geo, grid, vol_np)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

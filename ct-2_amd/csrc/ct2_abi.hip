@@ -221,6 +221,11 @@ int ct2_ray_artstep_nearest(float reg_fac, const float* sinomat, const int8_t* i
     return launch_ray_art(&d, 1, reg_fac, sinomat, indsino, rv, nr, smat, (hipStream_t)stream);
 }
 
+int ct2_convert_volume(const float* src, void* dst, int dtype, size_t n, int rounding,
+                       unsigned seed, ct2_stream_t stream) {
+    return launch_convert(src, dst, dtype, n, rounding, seed, (hipStream_t)stream);
+}
+
 // ---- SART ----
 int ct2_sart_step(const ct2_desc* d, float lambda, const float* sino, const int8_t* ind,
                   const float* fv, float* vol, float* work, ct2_stream_t stream) {

@@ -312,6 +312,8 @@ int launch_sart(const ct2_desc* d, float lambda, const float* sino, const int8_t
                 const float* fv, float* vol, float* work, hipStream_t st);
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
                           hipStream_t st);
+int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,
+                   hipStream_t st);
 // device fill (graph-capture safe, unlike a memset node)
 int launch_fill(float* p, float v, size_t n, hipStream_t st);
 

@@ -6,6 +6,8 @@
 // distance weight; it is not the adjoint of the projector.  Reference FP32
 // operation order per view (view_proj3D.f:1890-1933), sequential over views,
 // so results are bit-identical to the Fortran.
+#include <type_traits>
+
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -23,6 +25,53 @@ int launch_fill(float* p, float v, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
     CT2_HIP(hipGetLastError());
     return CT2_OK;
+}
+
+// ---- float32 -> 16-bit volume storage ----
+// Round to nearest, or stochastically: the 16-bit neighbours lo <= |x| <= hi
+// are chosen with probabilities (hi-|x|)/(hi-lo), (|x|-lo)/(hi-lo), so the
+// stored value is unbiased and rounding errors are independent across voxels
+// (a constant region no longer carries one common bias into every ray sum).
+// The uniform variate is a counter-based hash of (index, seed): deterministic.
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    // integer finaliser (xorshift-multiply rounds)
+    x ^= x >> 16; x *= 0x7feb352dU;
+    x ^= x >> 15; x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+template <typename OT>
+__global__ void k_convert(const float* __restrict__ src, OT* __restrict__ dst, size_t n,
+                          int stochastic, uint32_t seed) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float x = src[i];
+        if (!stochastic) { Store<OT>::st(dst, i, x); continue; }
+        uint16_t lo_bits;
+        float lo;
+        if constexpr (std::is_same<OT, __half>::value) {
+            __half h = __float2half_rz(x);
+            lo_bits = __half_as_ushort(h);
+            lo = __half2float(h);
+        } else {
+            uint32_t b = __float_as_uint(x);
+            lo_bits = (uint16_t)(b >> 16);           // truncation = toward zero
+            lo = __uint_as_float((uint32_t)lo_bits << 16);
+        }
+        uint16_t out = lo_bits;
+        if (lo != x && isfinite(x)) {
+            const uint16_t hi_bits = (uint16_t)(lo_bits + 1);   // next magnitude up
+            float hi;
+            if constexpr (std::is_same<OT, __half>::value) hi = __half2float(__ushort_as_half(hi_bits));
+            else hi = __uint_as_float((uint32_t)hi_bits << 16);
+            const float p = (fabsf(x) - fabsf(lo)) / (fabsf(hi) - fabsf(lo));
+            const float u = (float)(hash32((uint32_t)i ^ hash32(seed + (uint32_t)(i >> 32))) >> 8) *
+                            (1.0f / 16777216.0f);
+            if (u < p && isfinite(hi)) out = hi_bits;
+        }
+        reinterpret_cast<uint16_t*>(dst)[i] = out;
+    }
 }
 
 // Per-view constants: S, D, e_u, e_v, e_w and the source-to-panel distance
@@ -96,6 +145,34 @@ __global__ __launch_bounds__(256) void k_pd_backproject(Grid g, Det dt,
         }
     }
     vol[(size_t)ix * g.s[0] + (size_t)iy * g.s[1] + (size_t)iz * g.s[2]] = total;
+}
+
+int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,
+                   hipStream_t st) {
+    if (n == 0) return CT2_OK;
+    if (!src || !dst) { set_error("ct2_convert_volume: null pointer"); return CT2_EINVAL; }
+    if (rounding != CT2_ROUND_NEAREST && rounding != CT2_ROUND_STOCHASTIC) {
+        set_error("ct2_convert_volume: unknown rounding %d", rounding);
+        return CT2_EINVAL;
+    }
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    int s = rounding == CT2_ROUND_STOCHASTIC;
+    switch (dtype) {
+        case CT2_F16:
+            hipLaunchKernelGGL(k_convert<__half>, dim3((unsigned)blocks), dim3(256), 0, st, src,
+                               (__half*)dst, n, s, (uint32_t)seed);
+            break;
+        case CT2_BF16:
+            hipLaunchKernelGGL(k_convert<hip_bfloat16>, dim3((unsigned)blocks), dim3(256), 0, st,
+                               src, (hip_bfloat16*)dst, n, s, (uint32_t)seed);
+            break;
+        default:
+            set_error("ct2_convert_volume: destination must be CT2_F16 or CT2_BF16");
+            return CT2_EINVAL;
+    }
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
 }
 
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,

@@ -18,6 +18,7 @@ CT2_LINEAR, CT2_NEAREST = 0, 1
 CT2_AXIS_FREE, CT2_AXIS_Z = 0, 1
 CT2_F32, CT2_F16, CT2_BF16 = 0, 1, 2
 CT2_ARITH_EXACT, CT2_ARITH_FAST = 0, 1
+CT2_ROUND_NEAREST, CT2_ROUND_STOCHASTIC = 0, 1
 
 _V = "iiiffff"          # ns, nu, nv, du, dv, u0, v0
 _G = "ffffffiii"        # dx, dy, dz, x0, y0, z0, nx, ny, nz
@@ -29,6 +30,7 @@ SIGNATURES = {
     "ct2_project": "PPPPPS",
     "ct2_backproject": "PPPPPS",
     "ct2_sart_step": "PfPPPPPS",
+    "ct2_convert_volume": "PPiziuS",
     "ct2_sinoproj_linear": "PPP" + _V + "P" + _G + "S",
     "ct2_sinoproj_nearest": "PPP" + _V + "P" + _G + "S",
     "ct2_sinobackproject_linear": "PPP" + _V + "P" + _G + "S",
@@ -58,7 +60,8 @@ SIGNATURES = {
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",
 }
 
-_CT = {"P": C.c_void_p, "i": C.c_int, "f": C.c_float, "S": C.c_void_p, "d": C.c_double}
+_CT = {"P": C.c_void_p, "i": C.c_int, "f": C.c_float, "S": C.c_void_p, "d": C.c_double,
+       "z": C.c_size_t, "u": C.c_uint}
 
 
 class ct2_desc(C.Structure):

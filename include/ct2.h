@@ -24,6 +24,7 @@
 #ifndef CT2_H
 #define CT2_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -41,6 +42,7 @@ enum { CT2_F32 = 0, CT2_F16 = 1, CT2_BF16 = 2 };
  * for the forward projector.  CT2_ARITH_FAST: per-ray setup in FP64, FMA
  * plane stepping in FP32, compensated (Kahan) ray sums. */
 enum { CT2_ARITH_EXACT = 0, CT2_ARITH_FAST = 1 };
+enum { CT2_ROUND_NEAREST = 0, CT2_ROUND_STOCHASTIC = 1 };
 
 /* Operator descriptor of the extended API. */
 typedef struct ct2_desc {
@@ -73,6 +75,13 @@ int ct2_project(const ct2_desc* desc, float* sino, const int8_t* ind, const floa
  * (storage desc->vol_dtype). */
 int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
                     const float* fv, ct2_stream_t stream);
+/* float32 volume -> 16-bit storage (dtype CT2_F16 or CT2_BF16) for the
+ * mixed-precision forward projection.  CT2_ROUND_STOCHASTIC picks the upper
+ * or lower 16-bit neighbour with probability proportional to proximity,
+ * driven by a counter-based hash of (index, seed), so the stored volume is
+ * unbiased and deterministic.  No counterpart in the reference (all REAL*4). */
+int ct2_convert_volume(const float* src, void* dst, int dtype, size_t n, int rounding,
+                       unsigned seed, ct2_stream_t stream);
 /* SART update over the view subset of desc (no counterpart in the reference,
  * which has only sequential ART): x += lambda * A^T((b-Ax)/A1) / A^T1
  * restricted to those views, with work = 2*ns*nu*nv + 3*nx*ny*nz floats of
