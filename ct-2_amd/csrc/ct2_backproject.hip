@@ -12,6 +12,12 @@
 //
 // Linear weights per corner are (trav*wb)*wc and the contribution is
 // datum*weight, the reference's order; only the summation order differs.
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -82,7 +88,13 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
     br.ext[1] = min(bdim.y, g.n[1] - br.lo[1]);
     br.ext[2] = min(bdim.z, g.n[2] - br.lo[2]);
     const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
-    for (int q = threadIdx.x; q < nloc; q += blockDim.x) acc[q] = 0.0;
+    // LDS layout with a one-voxel halo on every side: local voxel (x,y,z),
+    // -1 <= x <= ext, lives at org + x*ls[0] + y*ls[1] + z.  Every corner of a
+    // sample whose cell touches the brick then has a slot; halo slots belong
+    // to neighbouring bricks and are dropped at write-back.
+    const int hy = br.ext[1] + 2, hz = br.ext[2] + 2;
+    const int nhalo = (br.ext[0] + 2) * hy * hz;
+    for (int q = threadIdx.x; q < nhalo; q += blockDim.x) acc[q] = 0.0;
 
     // box containing every sample point that can touch the brick: centres of
     // voxels lo-1 .. lo+ext (one voxel margin on both sides)
@@ -93,7 +105,8 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
         box_hi[q] = (double)g.o[q] + ((double)(br.lo[q] + br.ext[q]) + 0.5) * g.d[q];
     }
     // LDS strides of x, y, z inside the brick
-    const int ls[3] = {br.ext[1] * br.ext[2], br.ext[2], 1};
+    const int ls[3] = {hy * hz, hz, 1};
+    const int org = ls[0] + ls[1] + 1;
 
     for (int pv = 0; pv < nviews; ++pv) {
         const int p = view0 + pv;
@@ -179,31 +192,33 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
                 const unsigned hc = (unsigned)(INTERP == CT2_LINEAR ? R.nc - 1 : R.nc);
                 const float dtr = datum * R.trav;
                 // lanes step the brick's planes together (same LDS plane per
-                // instruction); planes outside this ray's range are skipped
-                for (int l = la0; l < la1; ++l) {
-                    if (l < l0 || l > l1) continue;
+                // instruction); planes outside this ray's range are masked
+                int base = org;
+                for (int l = la0; l < la1; ++l, base += lsa) {
                     const float ub = __builtin_fmaf((float)l, R.dub, R.ub0);
                     const float uc = __builtin_fmaf((float)l, R.duc, R.uc0);
                     const float fb = floorf(ub), fc = floorf(uc);
                     const int ib = (int)fb, ic = (int)fc;
-                    if ((unsigned)ib >= hb || (unsigned)ic >= hc) continue;
                     const int qb = ib - lob, qc = ic - loc;
-                    const int base = (l - la0) * lsa;
+                    const bool valid = (l >= l0) & (l <= l1) & ((unsigned)ib < hb) &
+                                       ((unsigned)ic < hc);
                     if (INTERP == CT2_LINEAR) {
-                        if (qb < -1 || qb >= lobext || qc < -1 || qc >= locext) continue;
+                        if (!(valid & (qb >= -1) & (qb < lobext) & (qc >= -1) & (qc < locext)))
+                            continue;
                         const float wb = ub - fb, wc = uc - fc;
-                        const float w1b = 1.0f - wb, w1c = 1.0f - wc;
-                        if (qb >= 0 && qc >= 0)
-                            lds_add(&acc[base + qb * lsb + qc * lsc], dtr * (w1b * w1c));
-                        if (qb + 1 < lobext && qc >= 0)
-                            lds_add(&acc[base + (qb + 1) * lsb + qc * lsc], dtr * (wb * w1c));
-                        if (qb >= 0 && qc + 1 < locext)
-                            lds_add(&acc[base + qb * lsb + (qc + 1) * lsc], dtr * (w1b * wc));
-                        if (qb + 1 < lobext && qc + 1 < locext)
-                            lds_add(&acc[base + (qb + 1) * lsb + (qc + 1) * lsc], dtr * (wb * wc));
+                        const float tb0 = dtr * (1.0f - wb), tb1 = dtr * wb;
+                        const float w1c = 1.0f - wc;
+                        // qb, qc >= -1 and qb+1, qc+1 <= ext: all four corners
+                        // have a slot (interior or halo)
+                        const int a00 = base + __mul24(qb, lsb) + __mul24(qc, lsc);
+                        lds_add(&acc[a00], tb0 * w1c);
+                        lds_add(&acc[a00 + lsb], tb1 * w1c);
+                        lds_add(&acc[a00 + lsc], tb0 * wc);
+                        lds_add(&acc[a00 + lsb + lsc], tb1 * wc);
                     } else {
-                        if (qb < 0 || qb >= lobext || qc < 0 || qc >= locext) continue;
-                        lds_add(&acc[base + qb * lsb + qc * lsc], dtr);
+                        if (!(valid & (qb >= 0) & (qb < lobext) & (qc >= 0) & (qc < locext)))
+                            continue;
+                        lds_add(&acc[base + __mul24(qb, lsb) + __mul24(qc, lsc)], dtr);
                     }
                 }
                 continue;
@@ -244,7 +259,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
                     const int qb = sm.ib - lob, qc = sm.ic - loc;
                     // cell corners qb, qb+1 and qc, qc+1 must meet the brick
                     if (qb < -1 || qb >= lobext || qc < -1 || qc >= locext) continue;
-                    const int base = (l - la0) * lsa;
+                    const int base = org + (l - la0) * lsa;
                     float w;
                     {
                         if (qb >= 0 && qc >= 0) {
@@ -268,7 +283,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
                     if (!sample_nearest_exact(A, l, false, sm)) continue;
                     const int qb = sm.ib - lob, qc = sm.ic - loc;
                     if (qb < 0 || qb >= lobext || qc < 0 || qc >= locext) continue;
-                    lds_add(&acc[(l - la0) * lsa + qb * lsb + qc * lsc], datum * A.trav);
+                    lds_add(&acc[org + (l - la0) * lsa + qb * lsb + qc * lsc], datum * A.trav);
                 }
             }
         }
@@ -281,7 +296,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
         int lx = q / (br.ext[2] * br.ext[1]);
         size_t gi = (size_t)(br.lo[0] + lx) * g.s[0] + (size_t)(br.lo[1] + ly) * g.s[1] +
                     (size_t)(br.lo[2] + lz) * g.s[2];
-        Store<OT>::st(vol, gi, (float)acc[q]);
+        Store<OT>::st(vol, gi, (float)acc[org + lx * ls[0] + ly * ls[1] + lz]);
     }
 }
 
@@ -328,7 +343,7 @@ static int3 default_brick(const ct2_desc* d) {
     int3 b;
     b.x = d->brick[0] > 0 ? d->brick[0] : 16;
     b.y = d->brick[1] > 0 ? d->brick[1] : 16;
-    b.z = d->brick[2] > 0 ? d->brick[2] : 16;
+    b.z = d->brick[2] > 0 ? d->brick[2] : 12;
     return b;
 }
 
@@ -338,7 +353,8 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, int vie
                         hipStream_t st) {
     int3 b = default_brick(d);
     b.x = min(b.x, g.n[0]); b.y = min(b.y, g.n[1]); b.z = min(b.z, g.n[2]);
-    size_t lds = (size_t)b.x * b.y * b.z * sizeof(double);
+    // brick accumulator with a one-voxel halo on every side
+    size_t lds = (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double);
     if (lds > 63 * 1024) {
         set_error("ct2_backproject: brick %dx%dx%d needs %zu B of LDS (max 64512)", b.x, b.y,
                   b.z, lds);
@@ -365,6 +381,98 @@ static int dispatch_out(const ct2_desc* d, const Grid& g, const Det& dt, int vie
     return CT2_EINVAL;
 }
 
+static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+                       void* vol, const float* sn, const int8_t* ind, const float* fv,
+                       hipStream_t st);
+
+// ---- brick autotuner ----
+// The best brick depends on the geometry (footprint per view, LDS bank
+// pattern of the halo layout, occupancy).  With desc->brick all zero, the
+// first call for a geometry times each candidate once on the caller's stream
+// (every launch overwrites the whole output, so the final launch with the
+// winner leaves the right result) and caches the winner per geometry and
+// device.  No tuning while the stream is being captured into a graph, or when
+// CT2_AUTOTUNE=0: the default brick is used then.
+struct TuneKey {
+    int v[15];
+    bool operator<(const TuneKey& o) const { return memcmp(v, o.v, sizeof v) < 0; }
+};
+
+static std::mutex g_tune_mu;
+static std::map<TuneKey, int3>& tune_cache() {
+    static std::map<TuneKey, int3>* m = new std::map<TuneKey, int3>();
+    return *m;
+}
+
+static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt, int view0,
+                             int nviews, void* vol, const float* sn, const int8_t* ind,
+                             const float* fv, hipStream_t st) {
+    int dev = 0;
+    CT2_HIP(hipGetDevice(&dev));
+    TuneKey key;
+    memset(&key, 0, sizeof key);
+    int i = 0;
+    for (int v : {d->nx, d->ny, d->nz, d->ns, d->nu, d->nv, nviews, d->interp, d->arith,
+                  d->axis, d->vol_dtype, dev})
+        key.v[i++] = v;
+    memcpy(&key.v[i++], &d->dx, sizeof(float));
+    memcpy(&key.v[i++], &d->du, sizeof(float));
+    key.v[i++] = ind != nullptr;
+    ct2_desc e = *d;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = tune_cache().find(key);
+        if (it != tune_cache().end()) {
+            e.brick[0] = it->second.x; e.brick[1] = it->second.y; e.brick[2] = it->second.z;
+            return launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+        }
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    CT2_HIP(hipStreamIsCapturing(st, &cs));
+    const char* env = getenv("CT2_AUTOTUNE");
+    if (cs != hipStreamCaptureStatusNone || (env && env[0] == '0'))
+        return launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+    static const int cand[][3] = {{16, 16, 12}, {16, 16, 13}, {12, 12, 14}, {16, 12, 12},
+                                  {14, 14, 14}, {16, 16, 16}, {8, 16, 16}, {16, 16, 8}};
+    const int nc = (int)(sizeof cand / sizeof cand[0]);
+    hipEvent_t ev[2 * 8];
+    for (int c = 0; c < 2 * nc; ++c) CT2_HIP(hipEventCreate(&ev[c]));
+    int rc = CT2_OK;
+    int3 tried[8];
+    int ntried = 0;
+    for (int c = 0; c < nc && rc == CT2_OK; ++c) {
+        int3 b = make_int3(min(cand[c][0], g.n[0]), min(cand[c][1], g.n[1]), min(cand[c][2], g.n[2]));
+        bool dup = false;
+        for (int t = 0; t < ntried; ++t)
+            dup |= tried[t].x == b.x && tried[t].y == b.y && tried[t].z == b.z;
+        if (dup) continue;
+        e.brick[0] = b.x; e.brick[1] = b.y; e.brick[2] = b.z;
+        hipEventRecord(ev[2 * ntried], st);
+        rc = launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+        hipEventRecord(ev[2 * ntried + 1], st);
+        tried[ntried++] = b;
+    }
+    int3 best = tried[0];
+    if (rc == CT2_OK) {
+        hipError_t he = hipEventSynchronize(ev[2 * ntried - 1]);
+        if (he != hipSuccess) rc = hip_fail(he, "autotune");
+        float tbest = 1e30f;
+        for (int t = 0; t < ntried && rc == CT2_OK; ++t) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[2 * t], ev[2 * t + 1]);
+            if (ms < tbest) { tbest = ms; best = tried[t]; }
+        }
+    }
+    for (int c = 0; c < 2 * nc; ++c) hipEventDestroy(ev[c]);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_cache()[key] = best;
+    }
+    e.brick[0] = best.x; e.brick[1] = best.y; e.brick[2] = best.z;
+    return launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+}
+
 int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int8_t* ind,
                        const float* fv, hipStream_t st) {
     Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
@@ -377,7 +485,20 @@ int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int
         return CT2_EINVAL;
     }
     if (!sino || !fv || !vol) { set_error("ct2_backproject: null pointer"); return CT2_EINVAL; }
-    const float* sn = (const float*)sino;
+    if (d->arith != CT2_ARITH_EXACT && d->arith != CT2_ARITH_FAST) {
+        set_error("ct2_backproject: unknown arith %d", d->arith);
+        return CT2_EINVAL;
+    }
+    if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0)
+        return tuned_backproject(d, g, dt, view0, nviews, vol, (const float*)sino, ind, fv, st);
+    return launch_with(d, g, dt, view0, nviews, vol, (const float*)sino, ind, fv, st);
+}
+
+// One launch with the brick of *d (0 entries take the defaults).
+static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+                       void* vol, const float* sn, const int8_t* ind, const float* fv,
+                       hipStream_t st) {
+    int rc;
     if (d->arith == CT2_ARITH_EXACT)
         rc = d->interp == CT2_LINEAR
                  ? dispatch_out<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, vol, sn, ind, fv, st)

@@ -241,7 +241,8 @@ __device__ __forceinline__ void clip_planes(double u0, double du, double lo, dou
         if (!(u0 >= lo && u0 < hi)) l1 = l0 - 1;
         return;
     }
-    double ta = (lo - u0) / du, tb = (hi - u0) / du;
+    const double idu = 1.0 / du;
+    double ta = (lo - u0) * idu, tb = (hi - u0) * idu;
     double tmin = fmax(fmin(ta, tb), -1e9), tmax = fmin(fmax(ta, tb), 1e9);
     if (!(tmin <= tmax)) { l1 = l0 - 1; return; }
     l0 = max(l0, (int)floor(tmin) - 1);
