@@ -185,6 +185,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 precision check")
+    ap.add_argument("--profile-once", action="store_true",
+                    help="for counter runs: one warm-up and one mixed fwd+bp step, then a "
+                         "512 MiB device copy that calibrates FETCH_SIZE/WRITE_SIZE; no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -205,6 +208,19 @@ def main():
     sino = torch.empty(op_mixed.sino_shape, dtype=torch.float32, device="cuda")
     recon = torch.empty(op_mixed.shape, dtype=torch.float32, device="cuda")
     nupd = 2 * op_mixed.updates_per_direction()
+
+    if args.profile_once:
+        for _ in range(2):
+            to_storage(vol32, vol16, stochastic=True)
+            op_mixed.forward(vol16, out=sino)
+            op_mixed.adjoint(sino, out=recon)
+        src = torch.zeros(128 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 512 MiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile step done")
+        return
 
     log("timing mixed path (FP16 storage with stochastic rounding, fast arithmetic)")
     wall_m, tc_m, tf_m, tb_m = time_path(op_mixed, vol32, vol16, sino, recon, args.steps,
