@@ -20,6 +20,7 @@
 namespace ct2 {
 
 constexpr int TV_BLOCKS = 1024;
+static_assert(TV_BLOCKS + 1 <= CT2_TV_WORK_DOUBLES, "TV workspace too small");
 constexpr int TV_THREADS = 256;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -154,6 +155,34 @@ __global__ __launch_bounds__(TV_THREADS) void k_tv_value_final(int nparts,
     if (threadIdx.x == 0) out[0] = (double)(float)(t - shift);   // Real*4 function result
 }
 
+// Distance sqrt(sum mask*(a-b)^2) in FP64 (tomo3D.py:319-324 for sinograms
+// with their indicator, :946-951 for images): per-block partial sums, then
+// one block adds the partials in a fixed order (deterministic).
+__global__ __launch_bounds__(TV_THREADS) void k_dist_partial(size_t n, const float* __restrict__ a,
+                                                             const float* __restrict__ b,
+                                                             const int8_t* __restrict__ mask,
+                                                             double* __restrict__ partial) {
+    __shared__ double sh[TV_THREADS / 64];
+    double acc = 0.0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        double d = (double)a[q] - (b ? (double)b[q] : 0.0);
+        if (!mask || mask[q] == 1) acc += d * d;
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(TV_THREADS) void k_dist_final(int nparts,
+                                                           const double* __restrict__ partial,
+                                                           double* __restrict__ out) {
+    __shared__ double sh[TV_THREADS / 64];
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += blockDim.x) acc += partial[q];
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[0] = sqrt(t);
+}
+
 static int tv_check(int nx, int ny, int nz) {
     if (nx <= 0 || ny <= 0 || nz <= 0) {
         set_error("TV: volume shape (%d, %d, %d) must be positive", nx, ny, nz);
@@ -169,28 +198,70 @@ using namespace ct2;
 
 extern "C" {
 
+int ct2_tv_grad(float ambdala, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                int normalizeq, float eps, float* bobo, double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!tvgmat || !smat || !work) { set_error("ct2_tv_grad: null pointer"); return CT2_EINVAL; }
+    hipStream_t st = (hipStream_t)stream;
+    TvDims d{nx, ny, nz, ambdala, eps};
+    size_t n = (size_t)nx * ny * nz;
+    int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
+    if (nb > TV_BLOCKS) nb = TV_BLOCKS;
+    // work: nb partial sums, then the float norm used by the normalisation
+    double* partial = work;
+    float* bscr = (float*)(partial + nb);
+    hipLaunchKernelGGL(k_tv_grad, dim3(nb), dim3(TV_THREADS), 0, st, d, smat, tvgmat, partial);
+    hipLaunchKernelGGL(k_tv_norm, dim3(1), dim3(TV_THREADS), 0, st, nb, partial, bobo, bscr);
+    if (normalizeq == 1)
+        hipLaunchKernelGGL(k_tv_normalize, dim3(nb), dim3(TV_THREADS), 0, st, n, tvgmat, bscr);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int ct2_tv_value(float ambdala, const float* smat, int nx, int ny, int nz, float eps,
+                 double* tv_out, double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!smat || !tv_out || !work) { set_error("ct2_tv_value: null pointer"); return CT2_EINVAL; }
+    hipStream_t st = (hipStream_t)stream;
+    TvDims d{nx, ny, nz, ambdala, eps};
+    size_t n = (size_t)nx * ny * nz;
+    int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
+    if (nb > TV_BLOCKS) nb = TV_BLOCKS;
+    // shift = (nx-1.)*(ny-1.)*(nz-1.)*sqrt(eps), Real*4 (totalvar.f:59)
+    float shift = (float)(nx - 1) * (float)(ny - 1) * (float)(nz - 1) * sqrtf(eps);
+    hipLaunchKernelGGL(k_tv_value, dim3(nb), dim3(TV_THREADS), 0, st, d, smat, work);
+    hipLaunchKernelGGL(k_tv_value_final, dim3(1), dim3(TV_THREADS), 0, st, nb, work,
+                       (double)shift, tv_out);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int ct2_distance(const float* a, const float* b, const int8_t* mask, size_t n, double* out,
+                 double* work, ct2_stream_t stream) {
+    if (!a || !out || !work) { set_error("ct2_distance: null pointer"); return CT2_EINVAL; }
+    hipStream_t st = (hipStream_t)stream;
+    size_t nbz = (n + TV_THREADS - 1) / TV_THREADS;
+    int nb = nbz > (size_t)TV_BLOCKS ? TV_BLOCKS : (nbz == 0 ? 1 : (int)nbz);
+    hipLaunchKernelGGL(k_dist_partial, dim3(nb), dim3(TV_THREADS), 0, st, n, a, b, mask, work);
+    hipLaunchKernelGGL(k_dist_final, dim3(1), dim3(TV_THREADS), 0, st, nb, work, out);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
 int ct2_total_var_grad3d(float ambdala, float* tvgmat, const float* smat, int nx, int ny, int nz,
                          int normalizeq, float eps, float* bobo, ct2_stream_t stream) {
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!tvgmat || !smat) { set_error("ct2_total_var_grad3d: null pointer"); return CT2_EINVAL; }
     hipStream_t st = (hipStream_t)stream;
-    TvDims d{nx, ny, nz, ambdala, eps};
-    size_t n = (size_t)nx * ny * nz;
-    int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
-    if (nb > TV_BLOCKS) nb = TV_BLOCKS;
-    // scratch: nb partial sums + the float norm (stream-ordered allocation)
     void* work = nullptr;
-    CT2_HIP(hipMallocAsync(&work, sizeof(double) * (nb + 1), st));
-    double* partial = (double*)work;
-    float* bscr = (float*)(partial + nb);
-    hipLaunchKernelGGL(k_tv_grad, dim3(nb), dim3(TV_THREADS), 0, st, d, smat, tvgmat, partial);
-    hipLaunchKernelGGL(k_tv_norm, dim3(1), dim3(TV_THREADS), 0, st, nb, partial, bobo, bscr);
-    if (normalizeq == 1)
-        hipLaunchKernelGGL(k_tv_normalize, dim3(nb), dim3(TV_THREADS), 0, st, n, tvgmat, bscr);
-    hipError_t e = hipGetLastError();
+    CT2_HIP(hipMallocAsync(&work, sizeof(double) * CT2_TV_WORK_DOUBLES, st));
+    rc = ct2_tv_grad(ambdala, tvgmat, smat, nx, ny, nz, normalizeq, eps, bobo, (double*)work,
+                     stream);
     hipError_t f = hipFreeAsync(work, st);
-    CT2_HIP(e);
+    if (rc) return rc;
     CT2_HIP(f);
     return CT2_OK;
 }
@@ -201,20 +272,11 @@ int ct2_total_var3d(float ambdala, const float* smat, int nx, int ny, int nz, fl
     if (rc) return rc;
     if (!smat || !tv_out) { set_error("ct2_total_var3d: null pointer"); return CT2_EINVAL; }
     hipStream_t st = (hipStream_t)stream;
-    TvDims d{nx, ny, nz, ambdala, eps};
-    size_t n = (size_t)nx * ny * nz;
-    int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
-    if (nb > TV_BLOCKS) nb = TV_BLOCKS;
     void* work = nullptr;
-    CT2_HIP(hipMallocAsync(&work, sizeof(double) * nb, st));
-    // shift = (nx-1.)*(ny-1.)*(nz-1.)*sqrt(eps), Real*4 (totalvar.f:59)
-    float shift = (float)(nx - 1) * (float)(ny - 1) * (float)(nz - 1) * sqrtf(eps);
-    hipLaunchKernelGGL(k_tv_value, dim3(nb), dim3(TV_THREADS), 0, st, d, smat, (double*)work);
-    hipLaunchKernelGGL(k_tv_value_final, dim3(1), dim3(TV_THREADS), 0, st, nb, (double*)work,
-                       (double)shift, tv_out);
-    hipError_t e = hipGetLastError();
+    CT2_HIP(hipMallocAsync(&work, sizeof(double) * CT2_TV_WORK_DOUBLES, st));
+    rc = ct2_tv_value(ambdala, smat, nx, ny, nz, eps, tv_out, (double*)work, stream);
     hipError_t f = hipFreeAsync(work, st);
-    CT2_HIP(e);
+    if (rc) return rc;
     CT2_HIP(f);
     return CT2_OK;
 }

@@ -83,3 +83,27 @@ def fdk_backproj_cube(ns, nu, nv, du, dv, u0, v0, radius, source2detector, uoffs
          uoffset, voffset, s0, ds, nx, ny, nz, x00, y00, z00, dx, dy, dz, 0.0, 0.0, 0.0,
          _dev.ptr(s), _dev.ptr(rec), _dev.stream_handle())
     return _dev.out(rec, host)
+
+
+def fdk_reconstruct(sinomat, sino, image, opt_window='', L=1):
+    """The FDK pipeline of emil_code/recon3D.py:50-139 for a circular
+    cone-beam tomo3D.sinogram3D ``sino`` onto the voxel centres of
+    tomo3D.image3D ``image``: angles vs = s0 + p*ds, detector and voxel
+    offsets 0.5, weighting called with (radius, source_to_detector) as at
+    recon3D.py:65-67, ramp filter along u (dim 1, pitch du), then
+    linear-interpolation backprojection.  Returns the (nx, ny, nz) cube."""
+    ns, nu, nv = sino.ns, sino.nu, sino.nv
+    R = sino.parms["radius"]
+    SDD = sino.parms["source_to_detector"]
+    vs = sino.s0 + np.arange(ns) * sino.ds
+    vu = sino.u0 + (np.arange(nu) + 0.5) * sino.du
+    vv = sino.v0 + (np.arange(nv) + 0.5) * sino.dv
+    vvg, vug = np.meshgrid(vv, vu)
+    w = fdk_sino_weight3D(R, SDD, ns, vug, vvg, sinomat)
+    f = fdk_sino_filter3D(sino.du, 1, w, opt_window, L)
+    vx = image.x0 + (np.arange(image.nx) + 0.5) * image.dx
+    vy = image.y0 + (np.arange(image.ny) + 0.5) * image.dy
+    vz = image.z0 + (np.arange(image.nz) + 0.5) * image.dz
+    vxg, vyg = np.meshgrid(vx, vy)
+    return fdk_backproj_cube(ns, nu, nv, sino.du, sino.dv, sino.u0, sino.v0, R, SDD, 0.5, 0.5,
+                             vs, vxg.ravel(), vyg.ravel(), vz, f)

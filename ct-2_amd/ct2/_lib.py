@@ -19,6 +19,7 @@ CT2_AXIS_FREE, CT2_AXIS_Z = 0, 1
 CT2_F32, CT2_F16, CT2_BF16 = 0, 1, 2
 CT2_ARITH_EXACT, CT2_ARITH_FAST = 0, 1
 CT2_ROUND_NEAREST, CT2_ROUND_STOCHASTIC = 0, 1
+CT2_TV_WORK_DOUBLES = 1040
 
 _V = "iiiffff"          # ns, nu, nv, du, dv, u0, v0
 _G = "ffffffiii"        # dx, dy, dz, x0, y0, z0, nx, ny, nz
@@ -55,6 +56,9 @@ SIGNATURES = {
     "ct2_ray_emstep_nearest": "PPPiP" + _G + "S",
     "ct2_total_var3d": "fPiiifPS",
     "ct2_total_var_grad3d": "fPPiiiifPS",
+    "ct2_tv_value": "fPiiifPPS",
+    "ct2_tv_grad": "fPPiiiifPPS",
+    "ct2_distance": "PPPzPPS",
     "ct2_fdk_sino_weight3d": "ffiiiffffffPPS",
     "ct2_fdk_sino_filter3d": "fiiiiPPifS",
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",

@@ -1,0 +1,233 @@
+"""Iteration drivers built on the device operators.
+
+* ``em_tv`` -- MLEM alternated with normalised TV-gradient descent, the
+  ASD-POCS pattern of emil_code/examples/jawPhantomCCB.py:191-327 with the
+  EM step (tomo3D.py:1394-1435) in place of ART: per outer iteration one EM
+  update, positivity, then ``n_tv`` steps x <- max(x - gf*g/|g|, 0) with
+  g = total_var_grad3d(x) (totalvar.f:476-602).  gf starts as
+  ``alpha * |x_EM - x_prev|`` (jawPhantomCCB.py:170, :213-215) and is reduced
+  by ``red_fact`` whenever the TV descent moved the image at least ``conv``
+  times as far as the data step (:321-322).  Every quantity stays on the
+  device (distances by ct2_distance, step sizes as 0-d tensors), so one
+  outer iteration can be captured into a HIP graph and replayed.
+* ``asd_pocs`` -- the jawPhantomCCB.py driver itself on tomo3D objects
+  (ART sweeps, zero_edges, make_positive, TV descent, adaptive gf and
+  reg_fac, data-error tolerance at ``iter_eps``), with a checkpoint that
+  resumes correctly (the reference reads gf, reg_fac and meps from one
+  ``read()`` of parms.txt, jawPhantomCCB.py:160-162, so its resume fails,
+  and saves in the middle of an iteration) and the traffic-file stop of
+  :235-244.
+* ``tv_least_squares`` -- 1/2||Ax-b||^2 + lam*TV(x) minimised by torch.optim
+  through the autograd projector (ct2.autograd).
+"""
+from __future__ import annotations
+
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _dev
+from ._lib import CT2_TV_WORK_DOUBLES, call
+from .autograd import project, total_variation, tv_grad_dev
+
+
+def distance(a, b=None, mask=None, out=None, work=None):
+    """sqrt(sum mask*(a-b)^2) in FP64 on the device, as a (1,) float64 tensor."""
+    out = torch.empty((1,), dtype=torch.float64, device=a.device) if out is None else out
+    work = torch.empty(CT2_TV_WORK_DOUBLES, dtype=torch.float64, device=a.device) \
+        if work is None else work
+    call("ct2_distance", _dev.ptr(a), _dev.ptr(b), _dev.ptr(mask), a.numel(), _dev.ptr(out),
+         _dev.ptr(work), _dev.stream_handle())
+    return out
+
+
+class EmTv:
+    """State and one outer iteration of ``em_tv`` with every buffer
+    preallocated, so that ``iterate`` launches no allocation of its own and
+    can be captured into a torch.cuda.CUDAGraph."""
+
+    def __init__(self, op, sino, x, n_tv=5, eps=1e-8, alpha=0.2, red_fact=0.98, conv=0.99):
+        self.op, self.sino, self.x = op, sino, x
+        self.n_tv, self.eps, self.alpha = int(n_tv), float(eps), float(alpha)
+        self.red_fact, self.conv = float(red_fact), float(conv)
+        dev = x.device
+        n_work = 2 * sino.numel() + 2 * x.numel()
+        self.work = torch.empty(n_work, dtype=torch.float32, device=dev)
+        self.sens = op.sensitivity()
+        self.prev = torch.empty_like(x)
+        self.g = torch.empty_like(x)
+        self.bobo = torch.empty((1,), dtype=torch.float32, device=dev)
+        self.rwork = torch.empty(CT2_TV_WORK_DOUBLES, dtype=torch.float64, device=dev)
+        self.adist = torch.zeros((1,), dtype=torch.float64, device=dev)
+        self.gdist = torch.zeros((1,), dtype=torch.float64, device=dev)
+        self.gf = torch.zeros((1,), dtype=torch.float32, device=dev)
+        self.started = torch.zeros((1,), dtype=torch.float32, device=dev)
+
+    def iterate(self):
+        x = self.x
+        self.prev.copy_(x)
+        self.op.em_step(self.sino, x, work=self.work)
+        x.clamp_(min=0.0)
+        distance(x, self.prev, out=self.adist, work=self.rwork)
+        # first iteration: gf <- alpha*adist; afterwards gf is kept
+        a32 = self.adist.to(torch.float32)
+        self.gf.copy_(torch.where(self.started > 0, self.gf, self.alpha * a32))
+        self.started.fill_(1.0)
+        self.prev.copy_(x)
+        for _ in range(self.n_tv):
+            tv_grad_dev(x, 0.0, self.eps, normalizeq=1, out=self.g, bobo=self.bobo,
+                        work=self.rwork)
+            x.sub_(self.gf * self.g)
+            x.clamp_(min=0.0)
+        distance(x, self.prev, out=self.gdist, work=self.rwork)
+        shrink = self.gdist >= self.conv * self.adist
+        self.gf.mul_(torch.where(shrink, self.red_fact, 1.0).to(torch.float32))
+
+
+def em_tv(op, sino, x, n_em=20, n_tv=5, eps=1e-8, alpha=0.2, graph=False):
+    """Run ``n_em`` outer iterations in place on x; with ``graph=True`` the
+    first iteration runs eagerly (it also tunes the adjoint) and the rest are
+    replays of one captured iteration."""
+    st = EmTv(op, sino, x, n_tv=n_tv, eps=eps, alpha=alpha)
+    if n_em <= 0:
+        return x
+    st.iterate()
+    if not graph:
+        for _ in range(n_em - 1):
+            st.iterate()
+        return x
+    g = capture(st.iterate)
+    for _ in range(n_em - 1):
+        g.replay()
+    return x
+
+
+def capture(fn):
+    """Capture fn (device work only, no host synchronisation, already run
+    once eagerly so that lazy set-up and the adjoint's brick tuning are done)
+    into a graph; returns the torch.cuda.CUDAGraph to replay."""
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    return g
+
+
+def fdk_em_tv(sino, image, op, n_em=20, n_tv=5, eps=1e-8, graph=True):
+    """FDK reconstruction followed by ``em_tv`` from max(FDK, 0) on device
+    tensors.  ``sino`` is a tomo3D.sinogram3D (circular cone beam), ``image``
+    the tomo3D.image3D geometry; returns the (nx, ny, nz) device volume."""
+    from . import FDK
+    b = _dev.to_dev(sino.mat, torch.float32, (sino.ns, sino.nu, sino.nv))
+    x = FDK.fdk_reconstruct(b, sino, image).contiguous()
+    x.clamp_(min=0.0)
+    em_tv(op, b, x, n_em=n_em, n_tv=n_tv, eps=eps, graph=graph)
+    return x
+
+
+# --------------------------------------------------------------- ASD-POCS --
+def _save_checkpoint(path, it, gf, reg_fac, meps, image):
+    os.makedirs(path, exist_ok=True)
+    m = image.mat
+    arr = m.detach().cpu().numpy() if isinstance(m, torch.Tensor) else np.asarray(m)
+    tmp = os.path.join(path, "imageTemp.dat.tmp")
+    arr.astype(np.float32).tofile(tmp)
+    os.replace(tmp, os.path.join(path, "imageTemp.dat"))
+    with open(os.path.join(path, "state.json.tmp"), "w") as f:
+        json.dump({"iter": it, "gf": gf, "reg_fac": reg_fac, "meps": meps}, f)
+    os.replace(os.path.join(path, "state.json.tmp"), os.path.join(path, "state.json"))
+
+
+def _load_checkpoint(path, image):
+    with open(os.path.join(path, "state.json")) as f:
+        s = json.load(f)
+    arr = np.fromfile(os.path.join(path, "imageTemp.dat"), dtype=np.float32)
+    arr = arr.reshape(image.nx, image.ny, image.nz)
+    if isinstance(image.mat, torch.Tensor):
+        image.mat = torch.from_numpy(arr).to(image.mat.device)
+    else:
+        image.mat = arr
+    return s["iter"], s["gf"], s["reg_fac"], s["meps"]
+
+
+def asd_pocs(image, sino, work_sino, max_iter, nart=1, ngradmax=20, gf=0.2, reg_fac=1.0,
+             red_fact=0.98, red_reg=0.995, conv=0.99, iter_eps=500, art_alg='art_linear',
+             checkpoint_dir=None, resume=False, traffic_file=None, truth=None, log=None):
+    """ASD-POCS of emil_code/examples/jawPhantomCCB.py:191-327 on tomo3D
+    objects (numpy or device arrays).  Returns the list of per-iteration
+    records (iter, data distance, TV, image error) that the reference appends
+    to derrEvol.txt."""
+    meps = -1.0
+    it = 0
+    if resume:
+        it, gf, reg_fac, meps = _load_checkpoint(checkpoint_dir, image)
+    pre = image.duplicate()
+    gimage = image.duplicate()
+    records = []
+    done = False
+    while not done and it < max_iter:
+        it += 1
+        image.copy_to(pre)
+        for _ in range(nart):
+            image.art_step(sino, reg_fac, alg=art_alg)
+        image.zero_edges()
+        image.make_positive()
+        adist = image.dist_to(pre)
+        if it == 1:
+            gf *= adist
+        if traffic_file is not None and os.path.exists(traffic_file):
+            with open(traffic_file) as f:
+                done = f.read().strip() == "stop"
+        posttv = image.calc_total_var(0.0)
+        image.project_to(work_sino)
+        ddist = sino.dist_to(work_sino)
+        if it == iter_eps:
+            meps = ddist
+        ierr = image.dist_to(truth) if truth is not None else float("nan")
+        if not done:
+            records.append((it, ddist, posttv, ierr))
+            if log is not None:
+                log(it, ddist, posttv, ierr)
+        image.copy_to(pre)
+        ngrad = 0
+        while ngrad < ngradmax and not done:
+            ngrad += 1
+            gimage.mat = image.mat
+            gimage.tv_grad()
+            gimage.scale(gf)
+            image.mat = image.mat - gimage.mat
+            image.zero_edges()
+            image.make_positive()
+        gdist = image.dist_to(pre)
+        if gdist >= conv * adist and ddist > meps:
+            gf *= red_fact
+        reg_fac *= red_reg
+        # saved after the whole iteration, so a resumed run continues the
+        # same trajectory (the reference saves before its TV descent)
+        if checkpoint_dir is not None:
+            _save_checkpoint(checkpoint_dir, it, gf, reg_fac, meps, image)
+    return records
+
+
+# ------------------------------------------------ TV-regularised LS (Adam) --
+def tv_least_squares(op, b, lam=1e-3, steps=200, lr=1e-2, x0=None, eps=1e-8, callback=None):
+    """min_x 1/2||Ax-b||^2 + lam*TV(x) by torch Adam through the exact
+    projector/adjoint pair; returns (x, list of losses)."""
+    x = torch.zeros(op.shape, dtype=torch.float32, device=b.device) if x0 is None \
+        else x0.detach().clone()
+    x.requires_grad_(True)
+    opt = torch.optim.Adam([x], lr=lr)
+    losses = []
+    for k in range(steps):
+        opt.zero_grad(set_to_none=False)
+        r = project(x, op) - b
+        loss = 0.5 * (r * r).sum(dtype=torch.float64).to(torch.float32) + \
+            lam * total_variation(x, 0.0, eps)
+        loss.backward()
+        opt.step()
+        losses.append(loss.detach())
+        if callback is not None:
+            callback(k, x, loss)
+    return x.detach(), [float(v) for v in losses]

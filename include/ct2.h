@@ -215,6 +215,22 @@ int ct2_total_var3d(float ambdala, const float* smat, int nx, int ny, int nz, fl
  * reference's |bobo| < 1e-5 -> 1 rule) is written to the device pointer bobo. */
 int ct2_total_var_grad3d(float ambdala, float* tvgmat, const float* smat, int nx, int ny,
                          int nz, int normalizeq, float eps, float* bobo, ct2_stream_t stream);
+/* Same two operations with a caller-owned device workspace of
+ * CT2_TV_WORK_DOUBLES doubles instead of a stream-ordered allocation, so
+ * that iteration loops can be captured into a graph.  No reference
+ * counterpart: the reference allocates inside the f2py wrapper. */
+#define CT2_TV_WORK_DOUBLES 1040
+int ct2_tv_value(float ambdala, const float* smat, int nx, int ny, int nz, float eps,
+                 double* tv_out, double* work, ct2_stream_t stream);
+int ct2_tv_grad(float ambdala, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                int normalizeq, float eps, float* bobo, double* work, ct2_stream_t stream);
+/* *out = sqrt(sum over q of (a[q]-b[q])^2), summed in FP64 in a fixed order,
+ * over the q with mask[q] == 1 (mask NULL = all; b NULL = 0).  This is the
+ * distance of sinogram3D.dist_to (tomo3D.py:319-324, indicator-masked) and
+ * image3D.dist_to (tomo3D.py:946-951) on device arrays; work holds
+ * CT2_TV_WORK_DOUBLES doubles. */
+int ct2_distance(const float* a, const float* b, const int8_t* mask, size_t n, double* out,
+                 double* work, ct2_stream_t stream);
 
 /* --------------------------------------------------- FDK.py replacements -- */
 /* FDK.py:10 fdk_sino_weight3D: out[p,j,k] = F/sqrt((F+D)^2 + vu[j,k]^2 + vv[j,k]^2)

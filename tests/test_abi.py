@@ -29,9 +29,33 @@ def test_library_exports_header():
     assert not missing, f"not exported: {missing}"
 
 
+def header_prototypes():
+    """name -> signature string in the ctypes table's letters, parsed from the header."""
+    text = open(os.path.join(ROOT, "include", "ct2.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    protos = {}
+    for m in re.finditer(r"\b(?:int|const char\*)\s+(ct2_\w+)\s*\(([^)]*)\)\s*;", text):
+        args = [a.strip() for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
+        sig = ""
+        for a in args:
+            if "*" in a:
+                sig += "P"
+            elif a.startswith("ct2_stream_t"):
+                sig += "S"
+            else:
+                ty = a.rsplit(None, 1)[0]
+                sig += {"int": "i", "float": "f", "double": "d", "size_t": "z",
+                        "unsigned": "u", "unsigned int": "u"}[ty]
+        protos[m.group(1)] = sig
+    return protos
+
+
 def test_prototypes_match_header():
     from ct2 import _lib
     assert sorted(_lib.SIGNATURES) == header_functions()
+    protos = header_prototypes()
+    wrong = {n: (s, protos[n]) for n, s in _lib.SIGNATURES.items() if protos[n] != s}
+    assert not wrong, f"ctypes table disagrees with the header: {wrong}"
 
 
 def test_error_path_without_device():

@@ -1,0 +1,144 @@
+"""Iteration drivers: EM + TV descent captured in a HIP graph, ASD-POCS with
+checkpoint/resume and the traffic-file stop, and FDK -> EM-TV."""
+import numpy as np
+import pytest
+import torch
+
+from cases import cone_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def small_problem(n=48, ns=40, nd=72, seed=5, photons=None):
+    from ct2 import configs3D, phantoms
+    from ct2.operator import ConeBeamOperator
+    R, SDD, dx, du = 500.0, 1000.0, 1.0, 1.6
+    fv = configs3D.frame_vectors("circular_conebeam", {"radius": R, "source_to_detector": SDD},
+                                 ns, 2 * np.pi, 0.0, 0)
+    hw = n * dx / 2
+    op = ConeBeamOperator(fv, ns, nd, nd, du, du, -nd * du / 2, -nd * du / 2, (n, n, n),
+                          (dx, dx, dx), (-hw, -hw, -hw), arith="fast")
+    truth = torch.from_numpy(np.ascontiguousarray(
+        phantoms.random_ellipsoids(n, hw, 12, seed) * 0.02, np.float32)).cuda()
+    b = op.forward(truth)
+    if photons:
+        rng = np.random.default_rng(seed + 1)
+        p = b.cpu().numpy().astype(np.float64)
+        I = rng.poisson(photons * np.exp(-p))
+        b = torch.from_numpy((-np.log(np.maximum(I, 1) / photons)).astype(np.float32)).cuda()
+    return op, truth, b
+
+
+def test_distance_kernel():
+    from ct2.solvers import distance
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(300001).astype(np.float32)
+    b = rng.standard_normal(300001).astype(np.float32)
+    m = (rng.random(300001) < 0.7).astype(np.int8)
+    ta, tb, tm = (torch.from_numpy(v).cuda() for v in (a, b, m))
+    got = distance(ta, tb, tm).item()
+    ref = np.sqrt(np.sum(m * (a.astype(np.float64) - b) ** 2))
+    assert abs(got - ref) <= 1e-12 * ref
+    assert abs(distance(ta).item() - np.linalg.norm(a.astype(np.float64))) <= 1e-12 * ref
+
+
+def test_em_tv_graph_replays_match_eager():
+    """One captured EM+TV iteration replayed; each replay (with eager work and
+    synchronisations in between) matches the eager iteration."""
+    from ct2.solvers import EmTv, capture
+    op, truth, b = small_problem(photons=1e5)
+    x0 = torch.full(op.shape, 0.01, device="cuda")
+    xe = x0.clone()
+    eager = EmTv(op, b, xe, n_tv=5, eps=1e-8)
+    ref = []
+    for _ in range(5):
+        eager.iterate()
+        ref.append(xe.clone())
+    xg = x0.clone()
+    st = EmTv(op, b, xg, n_tv=5, eps=1e-8)
+    st.iterate()
+    assert rel_l2(xg.cpu().numpy(), ref[0].cpu().numpy()) < 1e-5
+    g = capture(st.iterate)
+    # capture records without running; state is still after iteration 1
+    assert rel_l2(xg.cpu().numpy(), ref[0].cpu().numpy()) < 1e-5
+    for k in range(1, 5):
+        g.replay()
+        torch.cuda.synchronize()
+        junk = torch.randn(1000, device="cuda").sum().item()   # eager work between replays
+        assert np.isfinite(junk)
+        err = rel_l2(xg.cpu().numpy(), ref[k].cpu().numpy())
+        assert err < 1e-4, (k, err)
+    assert float(st.gf.item()) > 0
+
+
+def test_em_tv_reduces_error():
+    from ct2.solvers import em_tv
+    op, truth, b = small_problem(photons=1e5)
+    x = torch.full(op.shape, 0.01, device="cuda")
+    e0 = rel_l2(x.cpu().numpy(), truth.cpu().numpy())
+    em_tv(op, b, x, n_em=10, n_tv=5, eps=1e-8, graph=True)
+    e1 = rel_l2(x.cpu().numpy(), truth.cpu().numpy())
+    assert float(x.min()) >= 0.0
+    assert e1 < 0.6 * e0, (e0, e1)
+
+
+def _jaw_objects(n=32, ns=24, nu=48, nv=48):
+    from ct2 import tomo3D
+    ind, fv, geo, vol, g2 = cone_case(n=n, ns=ns, nu=nu, nv=nv, holes=False)
+    sino = tomo3D.sinogram3D('circular_conebeam', {"radius": 500.0, "source_to_detector": 1000.0},
+                             (ns, nu, nv), 2 * np.pi, nu * 1.5, nv * 1.5, 0.0, -nu * 0.75,
+                             -nv * 0.75, 0)
+    img = tomo3D.image3D((n, n, n), n * 1.0, n * 1.0, n * 1.0, -n / 2, -n / 2, -n / 2)
+    truth = img.duplicate()
+    truth.mat = np.ascontiguousarray(vol, np.float32)
+    truth.to_device()
+    sino.to_device()
+    img.to_device()
+    truth.project_to(sino)
+    return img, sino, truth
+
+
+def test_asd_pocs_resume_continues_trajectory(tmp_path):
+    from ct2.solvers import asd_pocs
+    img, sino, truth = _jaw_objects()
+    work = sino.duplicate()
+    a = img.duplicate()
+    rec_a = asd_pocs(a, sino, work, max_iter=4, ngradmax=5, truth=truth)
+    b = img.duplicate()
+    rec_b1 = asd_pocs(b, sino, work, max_iter=2, ngradmax=5, truth=truth,
+                      checkpoint_dir=str(tmp_path))
+    c = img.duplicate()
+    c.mat = torch.zeros_like(c.mat)
+    rec_b2 = asd_pocs(c, sino, work, max_iter=4, ngradmax=5, truth=truth,
+                      checkpoint_dir=str(tmp_path), resume=True)
+    assert [r[0] for r in rec_b1 + rec_b2] == [1, 2, 3, 4]
+    assert rel_l2(c.mat.cpu().numpy(), a.mat.cpu().numpy()) < 1e-6
+    for ra, rb in zip(rec_a, rec_b1 + rec_b2):
+        assert abs(ra[1] - rb[1]) <= 1e-5 * abs(ra[1])
+    # data distance falls
+    assert rec_a[-1][1] < rec_a[0][1]
+
+
+def test_asd_pocs_traffic_file_stop(tmp_path):
+    from ct2.solvers import asd_pocs
+    img, sino, truth = _jaw_objects(n=24, ns=12, nu=36, nv=36)
+    work = sino.duplicate()
+    stop = tmp_path / "trafficC.txt"
+    stop.write_text("stop\n")
+    rec = asd_pocs(img, sino, work, max_iter=10, ngradmax=3, traffic_file=str(stop))
+    # the stop is read after the first ART sweep; no record is written then
+    assert rec == []
+
+
+def test_fdk_then_em_tv():
+    from ct2.operator import ConeBeamOperator
+    from ct2.solvers import fdk_em_tv
+    img, sino, truth = _jaw_objects(n=40, ns=60, nu=64, nv=64)
+    op = ConeBeamOperator.from_objects(sino, img, arith="fast")
+    x = fdk_em_tv(sino, img, op, n_em=5, n_tv=5, graph=True)
+    assert x.shape == (40, 40, 40) and bool(torch.isfinite(x).all())
+    w = sino.duplicate()
+    r0 = sino.dist_to(w)   # w == sino
+    assert r0 == 0.0
+    w.mat = op.forward(x)
+    assert sino.dist_to(w) < 0.5 * float(torch.linalg.vector_norm(sino.mat.double()))
