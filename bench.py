@@ -15,6 +15,9 @@ and the reference path (the reference Fortran compiled from its sources,
 1 thread) timed on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
+
+The oracle (CPU restatement and compiled reference) is used only in the CPU
+leg (cpu_leg), as the timed baseline and as the checker of the GPU outputs.
 """
 from __future__ import annotations
 
@@ -66,12 +69,8 @@ def make_op(fv, geo, grid, arith):
 
 def to_storage(vol32, vol16, stochastic, seed=20260107):
     """FP32 volume -> FP16 storage through the library's conversion kernel."""
-    import ctypes as C
-    from ct2 import _dev
-    from ct2._lib import CT2_F16, CT2_ROUND_NEAREST, CT2_ROUND_STOCHASTIC, call
-    call("ct2_convert_volume", _dev.ptr(vol32), _dev.ptr(vol16), CT2_F16, C.c_size_t(vol32.numel()),
-         CT2_ROUND_STOCHASTIC if stochastic else CT2_ROUND_NEAREST, seed, _dev.stream_handle())
-    return vol16
+    from ct2.operator import to_storage as convert
+    return convert(vol32, vol16.dtype, "stochastic" if stochastic else "nearest", seed, out=vol16)
 
 
 def time_path(op, vol32, vol_store, sino, recon, steps, warmup):
@@ -106,61 +105,83 @@ def time_path(op, vol32, vol_store, sino, recon, steps, warmup):
     return wall, t_cvt, t_fwd, t_bp
 
 
-def precision(fv, geo, grid, vol_np, sino_gpu, op, check_views):
-    """Relative L2 of the GPU forward (all views, compared on a subset) and
-    of the GPU adjoint of that subset, against the FP64 oracle."""
-    import numpy as np
+def check_views(ns, count=4):
+    return list(range(0, ns, ns // count))[:count]
+
+
+def gpu_precision_outputs(op_mixed, op_fp32, vol32, vol16, sub):
+    """GPU outputs on the views `sub` that the CPU leg checks against the FP64
+    oracle: the forward in each storage mode and the adjoint (both
+    arithmetics) of the FP32 forward restricted to those views."""
     import torch
-    import oracle
-    o64 = oracle.restated_raw("f64")
-    ns, nu, nv = geo[:3]
-    sub = list(range(0, ns, ns // check_views))[:check_views]
-    fv_sub = np.ascontiguousarray(fv[sub])
-    geo_sub = (len(sub),) + tuple(geo[1:])
-    ref = o64.sinoproj(oracle.LINEAR, 0, None, fv_sub, *geo_sub, vol_np, *grid)
-    got = sino_gpu[sub].cpu().numpy().astype(np.float64)
-    e_fwd = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-    # adjoint of the FP64 sinogram subset, one view at a time on both sides
-    y = ref.astype(np.float32)
-    refb = o64.sinobackproject(oracle.LINEAR, 0, y, None, fv_sub, *geo_sub, *grid)
-    ysub = torch.zeros(op.sino_shape, dtype=torch.float32, device=op.device)
-    for i, p in enumerate(sub):
-        ysub[p] = torch.from_numpy(y[i]).to(op.device)
-    gotb = op.adjoint(ysub).cpu().numpy().astype(np.float64)
-    e_bp = float(np.linalg.norm(gotb - refb) / np.linalg.norm(refb))
-    return e_fwd, e_bp, len(sub)
+    out = {}
+    s = torch.empty(op_mixed.sino_shape, dtype=torch.float32, device=op_mixed.device)
+    to_storage(vol32, vol16, stochastic=True)
+    op_mixed.forward(vol16, out=s)
+    out["fwd_mixed"] = s[sub].cpu().numpy()
+    to_storage(vol32, vol16, stochastic=False)
+    op_mixed.forward(vol16, out=s)
+    out["fwd_mixed_round_nearest"] = s[sub].cpu().numpy()
+    op_fp32.forward(vol32, out=s)
+    out["fwd_fp32"] = s[sub].cpu().numpy()
+    y = torch.zeros_like(s)
+    y[sub] = s[sub]
+    out["y"] = out["fwd_fp32"]
+    out["bp_mixed"] = op_mixed.adjoint(y).cpu().numpy()
+    out["bp_fp32"] = op_fp32.adjoint(y).cpu().numpy()
+    return out
 
 
-def cpu_baseline(fv, geo, grid, vol_np, views=8):
-    """The reference path timed on the host: sinoproj_linear +
-    sinobackproject_linear of the reference Fortran (oracle/_ref, compiled
-    from the reference's own sources) on `views` views of the workload,
-    1 thread.  Falls back to the FP32 C restatement (kind "port") when the
-    compiled reference is not present."""
+def cpu_leg(fv, geo, grid, vol_np, views=8, gpu_out=None, sub=None):
+    """The CPU leg.  (1) The reference path timed on the host:
+    sinoproj_linear + sinobackproject_linear of the reference Fortran
+    (oracle/_ref, compiled from the reference's own sources) on `views` views
+    of the workload, 1 thread; the FP32 C restatement (kind "port") stands in
+    when the compiled reference is not present.  (2) As the checker: the FP64
+    oracle on the views `sub`, against which the GPU outputs `gpu_out` are
+    scored (relative L2)."""
     import numpy as np
     import oracle
     ns, nu, nv = geo[:3]
-    sub = list(range(0, ns, ns // views))[:views]
-    fv_sub = np.ascontiguousarray(fv[sub])
-    geo_sub = (len(sub),) + tuple(geo[1:])
+    tsub = list(range(0, ns, ns // views))[:views]
+    fv_sub = np.ascontiguousarray(fv[tsub])
+    geo_sub = (len(tsub),) + tuple(geo[1:])
     if oracle.ref_available():
         mods, kind = oracle.reference(), "reference"
     else:
         mods, kind = oracle.restated("f32"), "port"
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     t0 = time.perf_counter()
-    s = mods.view_proj3D.sinoproj_linear(None, fv_sub, *geo_sub, vol_np, *grid)
+    sref = mods.view_proj3D.sinoproj_linear(None, fv_sub, *geo_sub, vol_np, *grid)
     t1 = time.perf_counter()
-    mods.view_proj3D.sinobackproject_linear(s, None, fv_sub, *geo_sub, *grid)
+    mods.view_proj3D.sinobackproject_linear(sref, None, fv_sub, *geo_sub, *grid)
     t2 = time.perf_counter()
-    nupd = 2 * len(sub) * nu * nv * N
-    return {"value": round(nupd / (t2 - t0) / 1e9, 6), "unit": "Gvox-ray updates/s",
+    nupd = 2 * len(tsub) * nu * nv * N
+    base = {"value": round(nupd / (t2 - t0) / 1e9, 6), "unit": "Gvox-ray updates/s",
             "cores": 1, "kind": kind,
-            "sample": (f"{len(sub)} of {ns} views (fwd {t1 - t0:.1f} s + bp {t2 - t1:.1f} s), "
+            "sample": (f"{len(tsub)} of {ns} views (fwd {t1 - t0:.1f} s + bp {t2 - t1:.1f} s), "
                        f"{N}^3 volume, {nu}x{nv} detector, single thread"
                        + (", reference Fortran compiled by amdflang -O2 from emil_code/"
                           "view_proj3D.f, called through ctypes" if kind == "reference" else
                           ", FP32 C restatement of the reference"))}
+    prec = None
+    if gpu_out is not None:
+        o64 = oracle.restated_raw("f64")
+        fv_c = np.ascontiguousarray(fv[sub])
+        geo_c = (len(sub),) + tuple(geo[1:])
+        ref = o64.sinoproj(oracle.LINEAR, 0, None, fv_c, *geo_c, vol_np, *grid)
+        refb = o64.sinobackproject(oracle.LINEAR, 0, gpu_out["y"], None, fv_c, *geo_c, *grid)
+
+        def rel(a, b):
+            a = np.asarray(a, np.float64)
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        prec = {"views_checked": len(sub),
+                "mixed": {"fwd": rel(gpu_out["fwd_mixed"], ref),
+                          "bp": rel(gpu_out["bp_mixed"], refb)},
+                "mixed_round_nearest_fwd": rel(gpu_out["fwd_mixed_round_nearest"], ref),
+                "fp32": {"fwd": rel(gpu_out["fwd_fp32"], ref),
+                         "bp": rel(gpu_out["bp_fp32"], refb)}}
+    return base, prec
 
 
 def read_traffic():
@@ -183,7 +204,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU leg (baseline timing and FP64 check)")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 precision check")
     ap.add_argument("--profile-once", action="store_true",
                     help="for counter runs: one warm-up and one mixed fwd+bp step, then a "
@@ -227,25 +249,14 @@ def main():
                                          args.warmup)
     log(f"  convert {tc_m:.3f} ms  fwd {tf_m:.2f} ms  bp {tb_m:.2f} ms  "
         f"wall/step {1e3 * wall_m / args.steps:.2f} ms")
-    prec = {}
-    if not args.no_check:
-        log("precision vs FP64 oracle (mixed)")
-        e_fwd, e_bp, nchk = precision(fv, geo, grid, vol_np, sino, op_mixed, 4)
-        prec["mixed"] = {"fwd": e_fwd, "bp": e_bp}
-        # the same forward with round-to-nearest FP16 storage, for comparison
-        to_storage(vol32, vol16, stochastic=False)
-        op_mixed.forward(vol16, out=sino)
-        e_near, _, _ = precision(fv, geo, grid, vol_np, sino, op_mixed, 4)
-        prec["mixed_round_nearest_fwd"] = e_near
     log("timing FP32 path (FP32 storage, reference operation order)")
     sino32 = torch.empty_like(sino)
     wall_f, _, tf_f, tb_f = time_path(op_fp32, vol32, None, sino32, recon,
                                       max(1, args.steps // 2), 1)
     log(f"  fwd {tf_f:.2f} ms  bp {tb_f:.2f} ms")
-    if not args.no_check:
-        log("precision vs FP64 oracle (FP32)")
-        e_fwd, e_bp, nchk = precision(fv, geo, grid, vol_np, sino32, op_fp32, 4)
-        prec["fp32"] = {"fwd": e_fwd, "bp": e_bp}
+    sub = check_views(NS)
+    gpu_out = None if args.no_check else gpu_precision_outputs(op_mixed, op_fp32, vol32, vol16,
+                                                               sub)
 
     value = nupd / (wall_m / args.steps) / 1e9
     fp32_value = nupd / (wall_f / max(1, args.steps // 2)) / 1e9
@@ -279,14 +290,16 @@ def main():
                        "fwd_fp32": round(tf_f, 3), "bp_fp32": round(tb_f, 3)},
         "fp32_value": round(fp32_value, 3),
         "mixed_over_fp32": round(value / fp32_value, 3),
-        "rel_l2_vs_fp64": prec or None,
+        "rel_l2_vs_fp64": None,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic()},
     }
     if not args.no_cpu:
-        log("CPU baseline (reference path, bounded sample)")
-        line["cpu_baseline"] = cpu_baseline(fv, geo, grid, vol_np)
+        log("CPU leg: reference path on a bounded sample; FP64 oracle check of the GPU outputs")
+        base, prec = cpu_leg(fv, geo, grid, vol_np, gpu_out=gpu_out, sub=sub)
+        line["cpu_baseline"] = base
+        line["rel_l2_vs_fp64"] = prec
     print(json.dumps(line), flush=True)
 
 

@@ -19,9 +19,26 @@ import torch
 
 from . import _dev
 from ._lib import (CT2_ARITH_EXACT, CT2_ARITH_FAST, CT2_AXIS_FREE, CT2_AXIS_Z, CT2_BF16,
-                   CT2_F16, CT2_F32, CT2_LINEAR, CT2_NEAREST, call, ct2_desc)
+                   CT2_F16, CT2_F32, CT2_LINEAR, CT2_NEAREST, CT2_ROUND_NEAREST,
+                   CT2_ROUND_STOCHASTIC, call, ct2_desc)
 
 _DT = {torch.float32: CT2_F32, torch.float16: CT2_F16, torch.bfloat16: CT2_BF16}
+
+
+def to_storage(vol32, dtype=torch.float16, rounding="stochastic", seed=20260107, out=None):
+    """float32 volume -> 16-bit storage (float16 or bfloat16) by the library's
+    conversion kernel; ``rounding`` is "stochastic" (unbiased, deterministic
+    in ``seed``) or "nearest"."""
+    if vol32.dtype != torch.float32 or not vol32.is_contiguous():
+        raise ValueError("source must be a contiguous float32 tensor")
+    if dtype not in (torch.float16, torch.bfloat16):
+        raise ValueError("storage dtype must be float16 or bfloat16")
+    out = torch.empty(vol32.shape, dtype=dtype, device=vol32.device) if out is None else out
+    call("ct2_convert_volume", _dev.ptr(vol32), _dev.ptr(out), _DT[dtype],
+         C.c_size_t(vol32.numel()),
+         {"stochastic": CT2_ROUND_STOCHASTIC, "nearest": CT2_ROUND_NEAREST}[rounding],
+         int(seed), _dev.stream_handle())
+    return out
 
 
 class ConeBeamOperator:
