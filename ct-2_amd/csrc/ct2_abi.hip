@@ -43,6 +43,19 @@ int check_det(const Det& d) {
     return CT2_OK;
 }
 
+int view_set(const ct2_desc* d, const char* who, ViewSet& vs) {
+    vs.vstride = d->vstride > 0 ? d->vstride : 1;
+    vs.view0 = d->view0;
+    vs.nviews = d->nviews ? d->nviews : (d->ns - d->view0 + vs.vstride - 1) / vs.vstride;
+    if (d->vstride < 0 || vs.view0 < 0 || vs.nviews <= 0 ||
+        (long long)vs.view0 + (long long)(vs.nviews - 1) * vs.vstride >= d->ns) {
+        set_error("%s: bad view subset (view0 %d, nviews %d, vstride %d) of %d views", who,
+                  d->view0, d->nviews, d->vstride, d->ns);
+        return CT2_EINVAL;
+    }
+    return CT2_OK;
+}
+
 static void fill_desc(ct2_desc* d, int ns, int nu, int nv, float du, float dv, float u0,
                       float v0, int nx, int ny, int nz, float dx, float dy, float dz, float x0,
                       float y0, float z0) {
@@ -56,6 +69,7 @@ static void fill_desc(ct2_desc* d, int ns, int nu, int nv, float du, float dv, f
     d->axis = CT2_AXIS_FREE;
     d->vol_dtype = CT2_F32;
     d->arith = CT2_ARITH_EXACT;
+    d->vstride = 1;
 }
 
 }  // namespace ct2

@@ -65,20 +65,9 @@ __device__ void project_corner(const float* __restrict__ f, const double P[3], d
     ok = 1;
 }
 
-template <int INTERP, int ARITH, typename OT>
-__global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int view0,
-                                                           int nviews, int zonly, int3 bdim,
-                                                           int3 nbr, OT* __restrict__ vol,
-                                                           const float* __restrict__ sino,
-                                                           const int8_t* __restrict__ ind,
-                                                           const float* __restrict__ fv) {
-    extern __shared__ double acc[];
-    __shared__ int s_range[4];
-    __shared__ double s_fp[8][2];
-    __shared__ int s_ok[8];
-
-    // brick coordinates: x-major over bricks so neighbouring workgroups share z
-    int bid = blockIdx.x;
+// Brick number bid of a volume tiled by bdim (nbr bricks per axis, z
+// fastest, so neighbouring workgroups share z).
+__device__ __forceinline__ Brick brick_of(const Grid& g, int bid, int3 bdim, int3 nbr) {
     int bz = bid % nbr.z;
     int by = (bid / nbr.z) % nbr.y;
     int bx = bid / (nbr.z * nbr.y);
@@ -87,7 +76,27 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
     br.ext[0] = min(bdim.x, g.n[0] - br.lo[0]);
     br.ext[1] = min(bdim.y, g.n[1] - br.lo[1]);
     br.ext[2] = min(bdim.z, g.n[2] - br.lo[2]);
-    const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
+    return br;
+}
+
+// LDS slot of local voxel (x, y, z) of a brick accumulator (halo layout).
+__device__ __forceinline__ int brick_slot(const Brick& br, int x, int y, int z) {
+    const int hy = br.ext[1] + 2, hz = br.ext[2] + 2;
+    return (x + 1) * hy * hz + (y + 1) * hz + (z + 1);
+}
+
+// Accumulate A^T sino over views view0 + pv*vstride (pv < nviews) for the
+// voxels of brick br into the LDS accumulator acc (zeroed here).  Called by
+// the whole workgroup; ends with a barrier, after which acc holds the sums.
+template <int INTERP, int ARITH>
+__device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nviews,
+                                 int vstride, bool zonly, const Brick& br, double* acc,
+                                 const float* __restrict__ sino,
+                                 const int8_t* __restrict__ ind,
+                                 const float* __restrict__ fv) {
+    __shared__ int s_range[4];
+    __shared__ double s_fp[8][2];
+    __shared__ int s_ok[8];
     // LDS layout with a one-voxel halo on every side: local voxel (x,y,z),
     // -1 <= x <= ext, lives at org + x*ls[0] + y*ls[1] + z.  Every corner of a
     // sample whose cell touches the brick then has a slot; halo slots belong
@@ -109,7 +118,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
     const int org = ls[0] + ls[1] + 1;
 
     for (int pv = 0; pv < nviews; ++pv) {
-        const int p = view0 + pv;
+        const int p = view0 + pv * vstride;
         const float* f = fv + (size_t)p * 18;
         __syncthreads();   // previous view's readers of s_range are done
         if (threadIdx.x < 8) {
@@ -174,7 +183,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
                 double Sd[3], Bd[3];
                 view_ray_f64(fv, p, j, k, dt, Sd, Bd);
                 RayFast R;
-                setup_fast(g, Sd, Bd, zonly != 0, INTERP == CT2_LINEAR, R);
+                setup_fast(g, Sd, Bd, zonly, INTERP == CT2_LINEAR, R);
                 const int la0 = sel3i(R.a, br.lo[0], br.lo[1], br.lo[2]);
                 const int la1 = la0 + sel3i(R.a, br.ext[0], br.ext[1], br.ext[2]);
                 const int lob = sel3i(R.b, br.lo[0], br.lo[1], br.lo[2]);
@@ -226,7 +235,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
             float S[3], B[3];
             view_ray(fv, p, j, k, dt, S, B);
             AxisSel A;
-            setup_exact(g, S, B, zonly != 0, A);
+            setup_exact(g, S, B, zonly, A);
             const int a = A.a, b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
             // select per-axis brick scalars without indexing a local array
             const int la0 = sel3i(a, br.lo[0], br.lo[1], br.lo[2]);
@@ -289,15 +298,126 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
         }
     }
     __syncthreads();
+}
+
+template <int INTERP, int ARITH, typename OT>
+__global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int view0,
+                                                           int nviews, int vstride, int zonly,
+                                                           int3 bdim, int3 nbr,
+                                                           OT* __restrict__ vol,
+                                                           const float* __restrict__ sino,
+                                                           const int8_t* __restrict__ ind,
+                                                           const float* __restrict__ fv) {
+    extern __shared__ double acc[];
+    const Brick br = brick_of(g, blockIdx.x, bdim, nbr);
+    brick_accumulate<INTERP, ARITH>(g, dt, view0, nviews, vstride, zonly != 0, br, acc, sino,
+                                    ind, fv);
     // write the brick: z-rows are contiguous in both LDS and HBM
+    const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
     for (int q = threadIdx.x; q < nloc; q += blockDim.x) {
         int lz = q % br.ext[2];
         int ly = (q / br.ext[2]) % br.ext[1];
         int lx = q / (br.ext[2] * br.ext[1]);
         size_t gi = (size_t)(br.lo[0] + lx) * g.s[0] + (size_t)(br.lo[1] + ly) * g.s[1] +
                     (size_t)(br.lo[2] + lz) * g.s[2];
-        Store<OT>::st(vol, gi, (float)acc[org + lx * ls[0] + ly * ls[1] + lz]);
+        Store<OT>::st(vol, gi, (float)acc[brick_slot(br, lx, ly, lz)]);
     }
+}
+
+// ---- batched, persistent adjoint ----
+// Work items are the bricks of every problem (off[p] = first brick of
+// problem p, off[nprob] = total; problems with no view in the subset have
+// no items), dealt to resident workgroups in XCD-contiguous chunks as in the
+// batched forward.  Modes: BBP_PLAIN adj = A^T sino over all views;
+// BBP_COLSUM colsum_sub = A_sub^T resid (resid holds ones); BBP_SART
+// vol = sart_update(vol, lambda, A_sub^T resid, colsum_sub).
+template <int INTERP, int ARITH, int MODE>
+__global__ __launch_bounds__(256) void k_batch_backproject(const BatchProb* __restrict__ probs,
+                                                           const long long* __restrict__ off,
+                                                           int nprob, int sub, int nsub,
+                                                           int3 bdim, float lambda) {
+    extern __shared__ double acc[];
+    const long long total = off[nprob];
+    const int G = gridDim.x;
+    const int slot = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+    for (long long base = 0;; base += G) {
+        const long long it = base + slot;
+        if (it >= total) break;
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= it) lo = mid; else hi = mid - 1;
+        }
+        const BatchProb& P = probs[lo];
+        const Brick br = brick_of(P.g, (int)(it - off[lo]), bdim, P.nbr);
+        const int v0 = MODE == BBP_PLAIN ? 0 : sub;
+        const int vstr = MODE == BBP_PLAIN ? 1 : nsub;
+        const int nv = MODE == BBP_PLAIN ? P.dt.ns : subset_views(P.dt.ns, sub, nsub);
+        const float* src = MODE == BBP_PLAIN ? P.sino : P.resid;
+        brick_accumulate<INTERP, ARITH>(P.g, P.dt, v0, nv, vstr, P.zonly != 0, br, acc, src,
+                                        P.ind, P.fv);
+        const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
+        for (int q = threadIdx.x; q < nloc; q += blockDim.x) {
+            int lz = q % br.ext[2];
+            int ly = (q / br.ext[2]) % br.ext[1];
+            int lx = q / (br.ext[2] * br.ext[1]);
+            size_t gi = (size_t)(br.lo[0] + lx) * P.g.s[0] + (size_t)(br.lo[1] + ly) * P.g.s[1] +
+                        (size_t)(br.lo[2] + lz) * P.g.s[2];
+            const float c = (float)acc[brick_slot(br, lx, ly, lz)];
+            if (MODE == BBP_PLAIN) {
+                P.adj[gi] = c;
+            } else if (MODE == BBP_COLSUM) {
+                P.colsum[(size_t)sub * P.nvox + gi] = c;
+            } else {
+                P.vol[gi] = sart_update(P.vol[gi], lambda, c, P.colsum[(size_t)sub * P.nvox + gi]);
+            }
+        }
+        __syncthreads();   // write-back done before the next brick clears acc
+    }
+}
+
+size_t brick_lds_bytes(int3 b) { return (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double); }
+
+template <int INTERP, int ARITH>
+static const void* bbp_mode(int mode) {
+    switch (mode) {
+        case BBP_PLAIN: return (const void*)&k_batch_backproject<INTERP, ARITH, BBP_PLAIN>;
+        case BBP_COLSUM: return (const void*)&k_batch_backproject<INTERP, ARITH, BBP_COLSUM>;
+        case BBP_SART: return (const void*)&k_batch_backproject<INTERP, ARITH, BBP_SART>;
+    }
+    return nullptr;
+}
+
+static const void* batch_backproject_fn(int interp, int arith, int mode) {
+    if (interp == CT2_LINEAR)
+        return arith == CT2_ARITH_FAST ? bbp_mode<CT2_LINEAR, CT2_ARITH_FAST>(mode)
+                                       : bbp_mode<CT2_LINEAR, CT2_ARITH_EXACT>(mode);
+    return arith == CT2_ARITH_FAST ? bbp_mode<CT2_NEAREST, CT2_ARITH_FAST>(mode)
+                                   : bbp_mode<CT2_NEAREST, CT2_ARITH_EXACT>(mode);
+}
+
+int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int* blocks_per_cu) {
+    const void* fn = batch_backproject_fn(interp, arith, mode);
+    if (!fn) { set_error("batched backprojection: no kernel for mode %d", mode); return CT2_EINVAL; }
+    CT2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 256,
+                                                         brick_lds_bytes(bdim)));
+    return CT2_OK;
+}
+
+int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* probs,
+                             const long long* off, int nprob, int sub, int nsub, int3 bdim,
+                             float lambda, int grid, hipStream_t st) {
+    const void* fn = batch_backproject_fn(interp, arith, mode);
+    size_t lds = brick_lds_bytes(bdim);
+    if (!fn || grid <= 0 || (grid & 7) || lds > 63 * 1024) {
+        set_error("batched backprojection: bad launch (mode %d, grid %d, brick %dx%dx%d)", mode,
+                  grid, bdim.x, bdim.y, bdim.z);
+        return CT2_EINVAL;
+    }
+    void* args[] = {(void*)&probs, (void*)&off, (void*)&nprob, (void*)&sub, (void*)&nsub,
+                    (void*)&bdim, (void*)&lambda};
+    CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, st));
+    return CT2_OK;
 }
 
 // Ray-list adjoint: no detector structure to bound a footprint, so rays
@@ -348,7 +468,7 @@ static int3 default_brick(const ct2_desc* d) {
 }
 
 template <int INTERP, int ARITH, typename OT>
-static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                         void* vol, const float* sino, const int8_t* ind, const float* fv,
                         hipStream_t st) {
     int3 b = default_brick(d);
@@ -363,25 +483,26 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, int vie
     int3 nb = make_int3((g.n[0] + b.x - 1) / b.x, (g.n[1] + b.y - 1) / b.y,
                         (g.n[2] + b.z - 1) / b.z);
     dim3 grid(nb.x * nb.y * nb.z);
-    hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(256), lds, st, g, dt, view0,
-                       nviews, d->axis == CT2_AXIS_Z ? 1 : 0, b, nb, (OT*)vol, sino, ind, fv);
+    hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(256), lds, st, g, dt,
+                       vs.view0, vs.nviews, vs.vstride, d->axis == CT2_AXIS_Z ? 1 : 0, b, nb,
+                       (OT*)vol, sino, ind, fv);
     return CT2_OK;
 }
 
 template <int INTERP, int ARITH>
-static int dispatch_out(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+static int dispatch_out(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                         void* vol, const float* sino, const int8_t* ind, const float* fv,
                         hipStream_t st) {
     switch (d->vol_dtype) {
-        case CT2_F32: return launch_brick<INTERP, ARITH, float>(d, g, dt, view0, nviews, vol, sino, ind, fv, st);
-        case CT2_F16: return launch_brick<INTERP, ARITH, __half>(d, g, dt, view0, nviews, vol, sino, ind, fv, st);
-        case CT2_BF16: return launch_brick<INTERP, ARITH, hip_bfloat16>(d, g, dt, view0, nviews, vol, sino, ind, fv, st);
+        case CT2_F32: return launch_brick<INTERP, ARITH, float>(d, g, dt, vs, vol, sino, ind, fv, st);
+        case CT2_F16: return launch_brick<INTERP, ARITH, __half>(d, g, dt, vs, vol, sino, ind, fv, st);
+        case CT2_BF16: return launch_brick<INTERP, ARITH, hip_bfloat16>(d, g, dt, vs, vol, sino, ind, fv, st);
     }
     set_error("ct2_backproject: unknown vol_dtype %d", d->vol_dtype);
     return CT2_EINVAL;
 }
 
-static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                        void* vol, const float* sn, const int8_t* ind, const float* fv,
                        hipStream_t st);
 
@@ -404,16 +525,16 @@ static std::map<TuneKey, int3>& tune_cache() {
     return *m;
 }
 
-static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt, int view0,
-                             int nviews, void* vol, const float* sn, const int8_t* ind,
+static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt,
+                             const ViewSet& vs, void* vol, const float* sn, const int8_t* ind,
                              const float* fv, hipStream_t st) {
     int dev = 0;
     CT2_HIP(hipGetDevice(&dev));
     TuneKey key;
     memset(&key, 0, sizeof key);
     int i = 0;
-    for (int v : {d->nx, d->ny, d->nz, d->ns, d->nu, d->nv, nviews, d->interp, d->arith,
-                  d->axis, d->vol_dtype, dev})
+    for (int v : {d->nx, d->ny, d->nz, d->ns, d->nu, d->nv, vs.nviews * 64 + min(vs.vstride, 63),
+                  d->interp, d->arith, d->axis, d->vol_dtype, dev})
         key.v[i++] = v;
     memcpy(&key.v[i++], &d->dx, sizeof(float));
     memcpy(&key.v[i++], &d->du, sizeof(float));
@@ -424,14 +545,14 @@ static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt, in
         auto it = tune_cache().find(key);
         if (it != tune_cache().end()) {
             e.brick[0] = it->second.x; e.brick[1] = it->second.y; e.brick[2] = it->second.z;
-            return launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+            return launch_with(&e, g, dt, vs, vol, sn, ind, fv, st);
         }
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     CT2_HIP(hipStreamIsCapturing(st, &cs));
     const char* env = getenv("CT2_AUTOTUNE");
     if (cs != hipStreamCaptureStatusNone || (env && env[0] == '0'))
-        return launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+        return launch_with(&e, g, dt, vs, vol, sn, ind, fv, st);
     static const int cand[][3] = {{16, 16, 12}, {16, 16, 13}, {12, 12, 14}, {16, 12, 12},
                                   {14, 14, 14}, {16, 16, 16}, {8, 16, 16}, {16, 16, 8}};
     const int nc = (int)(sizeof cand / sizeof cand[0]);
@@ -448,7 +569,7 @@ static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt, in
         if (dup) continue;
         e.brick[0] = b.x; e.brick[1] = b.y; e.brick[2] = b.z;
         hipEventRecord(ev[2 * ntried], st);
-        rc = launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+        rc = launch_with(&e, g, dt, vs, vol, sn, ind, fv, st);
         hipEventRecord(ev[2 * ntried + 1], st);
         tried[ntried++] = b;
     }
@@ -470,7 +591,7 @@ static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt, in
         tune_cache()[key] = best;
     }
     e.brick[0] = best.x; e.brick[1] = best.y; e.brick[2] = best.z;
-    return launch_with(&e, g, dt, view0, nviews, vol, sn, ind, fv, st);
+    return launch_with(&e, g, dt, vs, vol, sn, ind, fv, st);
 }
 
 int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int8_t* ind,
@@ -479,34 +600,31 @@ int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int
     Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
     int rc;
     if ((rc = check_grid(g)) || (rc = check_det(dt))) return rc;
-    int view0 = d->view0, nviews = d->nviews ? d->nviews : d->ns - d->view0;
-    if (view0 < 0 || nviews <= 0 || view0 + nviews > d->ns) {
-        set_error("ct2_backproject: bad view range [%d, %d) of %d", view0, view0 + nviews, d->ns);
-        return CT2_EINVAL;
-    }
+    ViewSet vs;
+    if ((rc = view_set(d, "ct2_backproject", vs))) return rc;
     if (!sino || !fv || !vol) { set_error("ct2_backproject: null pointer"); return CT2_EINVAL; }
     if (d->arith != CT2_ARITH_EXACT && d->arith != CT2_ARITH_FAST) {
         set_error("ct2_backproject: unknown arith %d", d->arith);
         return CT2_EINVAL;
     }
     if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0)
-        return tuned_backproject(d, g, dt, view0, nviews, vol, (const float*)sino, ind, fv, st);
-    return launch_with(d, g, dt, view0, nviews, vol, (const float*)sino, ind, fv, st);
+        return tuned_backproject(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
+    return launch_with(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
 }
 
 // One launch with the brick of *d (0 entries take the defaults).
-static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nviews,
+static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                        void* vol, const float* sn, const int8_t* ind, const float* fv,
                        hipStream_t st) {
     int rc;
     if (d->arith == CT2_ARITH_EXACT)
         rc = d->interp == CT2_LINEAR
-                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, vol, sn, ind, fv, st)
-                 : dispatch_out<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, vol, sn, ind, fv, st);
+                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, vs, vol, sn, ind, fv, st)
+                 : dispatch_out<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, vs, vol, sn, ind, fv, st);
     else if (d->arith == CT2_ARITH_FAST)
         rc = d->interp == CT2_LINEAR
-                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, view0, nviews, vol, sn, ind, fv, st)
-                 : dispatch_out<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, view0, nviews, vol, sn, ind, fv, st);
+                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, vs, vol, sn, ind, fv, st)
+                 : dispatch_out<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, vs, vol, sn, ind, fv, st);
     else {
         set_error("ct2_backproject: unknown arith %d", d->arith);
         return CT2_EINVAL;

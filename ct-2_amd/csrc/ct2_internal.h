@@ -289,6 +289,54 @@ inline Det make_det(int ns, int nu, int nv, float du, float dv, float u0, float 
 int check_grid(const Grid& g);
 int check_det(const Det& d);
 
+// SART residual of one ray and update of one voxel (shared by the
+// single-problem and the batched SART so both compute the same values).
+__device__ __forceinline__ float sart_resid(float b, float ax, float rowsum, bool used) {
+    return (used && rowsum > 0.0f) ? (b - ax) / rowsum : 0.0f;
+}
+__device__ __forceinline__ float sart_update(float x, float lambda, float corr, float colsum) {
+    return colsum > 0.0f ? x + lambda * corr / colsum : x;
+}
+
+// ---- batched plans: one problem as the persistent kernels see it ----
+struct BatchProb {
+    Grid g;
+    Det dt;
+    int zonly;
+    int3 nbr;                  // bricks per axis for the plan's brick shape
+    long long nvox;
+    const float* fv;           // frame vectors [ns][18]
+    const int8_t* ind;         // indicator or null
+    const float* sino;         // measured sinogram
+    float* vol;                // estimate (read by the forward, updated by SART)
+    float* proj;               // forward output (plain projection) or null
+    float* adj;                // adjoint output (plain backprojection) or null
+    float* rowsum;             // A1 per ray
+    float* resid;              // SART residual sinogram
+    float* colsum;             // per subset s: A_s^T 1 at colsum + s*nvox
+};
+enum { BFWD_PLAIN = 0, BFWD_ROWSUM = 1, BFWD_RESID = 2 };
+enum { BBP_PLAIN = 0, BBP_COLSUM = 1, BBP_SART = 2 };
+// number of views p < ns with p = sub (mod nsub)
+inline __host__ __device__ int subset_views(int ns, int sub, int nsub) {
+    return sub < ns ? (ns - sub + nsub - 1) / nsub : 0;
+}
+int launch_batch_project(int interp, int arith, int tu, int mode, const BatchProb* probs,
+                         const long long* off, int nprob, int sub, int nsub, const float* ones,
+                         int grid, hipStream_t st);
+int batch_project_occupancy(int interp, int arith, int tu, int mode, int* blocks_per_cu);
+int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* probs,
+                             const long long* off, int nprob, int sub, int nsub, int3 bdim,
+                             float lambda, int grid, hipStream_t st);
+int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int* blocks_per_cu);
+size_t brick_lds_bytes(int3 bdim);
+
+// View subset of a descriptor: views view0 + j*vstride for j < nviews.
+struct ViewSet {
+    int view0, nviews, vstride;
+};
+int view_set(const ct2_desc* d, const char* who, ViewSet& vs);
+
 // ------------------------------------------------------ launchers (host) ----
 int launch_project(const ct2_desc* desc, void* sino, const int8_t* ind, const float* fv,
                    const void* vol, hipStream_t st);

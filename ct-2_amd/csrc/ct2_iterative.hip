@@ -340,28 +340,35 @@ int launch_em(const ct2_desc* d, const float* sino, const int8_t* ind, const flo
 }
 
 // ------------------------------------------------------------------ SART ----
-// Simultaneous ART over the view block [view0, view0+nviews) (Andersen & Kak):
-//   x_j += lambda * sum_i a_ij (b_i - a_i.x) / (sum_k a_ik)  /  sum_i a_ij
-// with the row sums A1 and column sums A^T 1 of the same block; rows or
+// Simultaneous ART over the view subset V = {view0 + j*vstride} (Andersen &
+// Kak; with vstride > 1, ordered subsets of interleaved views):
+//   x_j += lambda * sum_{i in V} a_ij (b_i - a_i.x) / (sum_k a_ik)  /  sum_{i in V} a_ij
+// with the row sums A1 and column sums A^T 1 of the same subset; rows or
 // columns with a zero sum contribute nothing.  The reference has no SART;
 // it is composed from the pinned forward and exact adjoint.
-__global__ void k_sart_resid(size_t off, size_t n, const float* __restrict__ sino,
-                             const int8_t* __restrict__ ind, const float* __restrict__ rowsum,
-                             float* __restrict__ resid) {
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    q += off;
-    float r = rowsum[q];
-    bool use = (ind == nullptr || ind[q] == 1) && r > 0.0f;
-    resid[q] = use ? (sino[q] - resid[q]) / r : 0.0f;
+// resid[q] <- sart_resid(sino[q], resid[q], rowsum[q]) on the rays of views
+// view0 + blockIdx.y*vstride (resid holds the forward projection on entry)
+__global__ void k_sart_resid(int view0, int vstride, size_t per_view,
+                             const float* __restrict__ sino, const int8_t* __restrict__ ind,
+                             const float* __restrict__ rowsum, float* __restrict__ resid) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_view) return;
+    size_t q = (size_t)(view0 + blockIdx.y * vstride) * per_view + e;
+    resid[q] = sart_resid(sino[q], resid[q], rowsum[q], ind == nullptr || ind[q] == 1);
+}
+
+__global__ void k_fill_views(int view0, int vstride, size_t per_view, float v,
+                             float* __restrict__ p) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_view) return;
+    p[(size_t)(view0 + blockIdx.y * vstride) * per_view + e] = v;
 }
 
 __global__ void k_sart_update(size_t n, float lambda, float* __restrict__ vol,
                               const float* __restrict__ corr, const float* __restrict__ colsum) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    float s = colsum[q];
-    if (s > 0.0f) vol[q] = vol[q] + lambda * corr[q] / s;
+    vol[q] = sart_update(vol[q], lambda, corr[q], colsum[q]);
 }
 
 size_t sart_work_floats(const ct2_desc* d) {
@@ -373,8 +380,9 @@ int launch_sart(const ct2_desc* d, float lambda, const float* sino, const int8_t
                 const float* fv, float* vol, float* work, hipStream_t st) {
     if (!work || !vol || !sino || !fv) { set_error("SART: null pointer"); return CT2_EINVAL; }
     if (d->vol_dtype != CT2_F32) { set_error("SART: updates a float32 volume"); return CT2_EINVAL; }
-    int view0 = d->view0, nviews = d->nviews ? d->nviews : d->ns - d->view0;
-    if (view0 < 0 || nviews <= 0 || view0 + nviews > d->ns) { set_error("SART: bad view range"); return CT2_EINVAL; }
+    int rc;
+    ViewSet vs;
+    if ((rc = view_set(d, "SART", vs))) return rc;
     size_t nsino = (size_t)d->ns * d->nu * d->nv, nvox = (size_t)d->nx * d->ny * d->nz;
     float* resid = work;
     float* rowsum = work + nsino;
@@ -382,16 +390,16 @@ int launch_sart(const ct2_desc* d, float lambda, const float* sino, const int8_t
     float* colsum = corr + nvox;
     float* ones = colsum + nvox;
     size_t per_view = (size_t)d->nu * d->nv;
-    size_t off = (size_t)view0 * per_view, n = (size_t)nviews * per_view;
-    int rc;
+    dim3 vgrid((unsigned)((per_view + 255) / 256), (unsigned)vs.nviews);
     if ((rc = launch_fill(ones, 1.0f, nvox, st))) return rc;
     if ((rc = launch_project(d, rowsum, ind, fv, ones, st))) return rc;
     if ((rc = launch_project(d, resid, ind, fv, vol, st))) return rc;
-    hipLaunchKernelGGL(k_sart_resid, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, off, n,
+    hipLaunchKernelGGL(k_sart_resid, vgrid, dim3(256), 0, st, vs.view0, vs.vstride, per_view,
                        sino, ind, rowsum, resid);
     if ((rc = launch_backproject(d, corr, resid, ind, fv, st))) return rc;
     // column sums: backprojection of ones (reuse rowsum as a ones sinogram)
-    if ((rc = launch_fill(rowsum + off, 1.0f, n, st))) return rc;
+    hipLaunchKernelGGL(k_fill_views, vgrid, dim3(256), 0, st, vs.view0, vs.vstride, per_view,
+                       1.0f, rowsum);
     if ((rc = launch_backproject(d, colsum, rowsum, ind, fv, st))) return rc;
     hipLaunchKernelGGL(k_sart_update, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st,
                        nvox, lambda, vol, corr, colsum);

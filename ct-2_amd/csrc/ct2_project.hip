@@ -79,25 +79,22 @@ __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restri
     return sum * R.trav;
 }
 
+// One lane's ray (view p, detector column j, row k) of a view-indexed
+// projection.  Every lane of the wavefront must call it (the fast path
+// shares the plane range across the wavefront); lanes with use == false
+// return 0.
 template <int INTERP, int ARITH, typename VT>
-__global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0,
-                                                       int zonly, float* __restrict__ sino,
-                                                       const int8_t* __restrict__ ind,
-                                                       const float* __restrict__ fv,
-                                                       const VT* __restrict__ vol) {
-    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int p = view0 + blockIdx.z;
-    const bool active = k < dt.nv && j < dt.nu;
-    const size_t r = ((size_t)p * dt.nu + j) * dt.nv + k;
-    const bool use = active && (ind == nullptr || ind[r] == 1);
+__device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int p, int j, int k,
+                                              bool use, bool zonly,
+                                              const float* __restrict__ fv,
+                                              const VT* __restrict__ vol) {
     float out = 0.0f;
     if (ARITH == CT2_ARITH_EXACT) {
         if (use) {
             float S[3], B[3];
             view_ray(fv, p, j, k, dt, S, B);
             AxisSel A;
-            setup_exact(g, S, B, zonly != 0, A);
+            setup_exact(g, S, B, zonly, A);
             out = march_exact<INTERP, VT>(A, vol, false);
         }
     } else {
@@ -106,7 +103,7 @@ __global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0
         if (use) {
             double S[3], B[3];
             view_ray_f64(fv, p, j, k, dt, S, B);
-            setup_fast(g, S, B, zonly != 0, INTERP == CT2_LINEAR, R);
+            setup_fast(g, S, B, zonly, INTERP == CT2_LINEAR, R);
             // planes where the ray can be inside the volume
             int a0 = 0, a1 = R.na - 1;
             const double hb = INTERP == CT2_LINEAR ? R.nb - 1 : R.nb;
@@ -123,7 +120,131 @@ __global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0
         }
         if (use) out = march_fast<INTERP, VT>(R, vol, false, l0, min(l1, R.na - 1));
     }
+    return out;
+}
+
+template <int INTERP, int ARITH, typename VT>
+__global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0, int vstride,
+                                                       int zonly, float* __restrict__ sino,
+                                                       const int8_t* __restrict__ ind,
+                                                       const float* __restrict__ fv,
+                                                       const VT* __restrict__ vol) {
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = view0 + blockIdx.z * vstride;
+    const bool active = k < dt.nv && j < dt.nu;
+    const size_t r = ((size_t)p * dt.nu + j) * dt.nv + k;
+    const bool use = active && (ind == nullptr || ind[r] == 1);
+    const float out = project_lane<INTERP, ARITH, VT>(g, dt, p, j, k, use, zonly != 0, fv, vol);
     if (active) sino[r] = out;
+}
+
+// ---- batched, persistent forward projection ----
+// Work items are detector tiles (TU columns x 64 rows) of every problem's
+// views in subset `sub` (views sub + pv*nsub), numbered problem by problem:
+// off[p] is the first item of problem p and off[nprob] the total.  The grid
+// is a whole number of waves of resident workgroups (a multiple of 8), and
+// each round of items is cut into 8 contiguous chunks, one per XCD
+// (workgroups are dealt to the XCDs round-robin), so neighbouring tiles of
+// one view share an L2.
+__device__ __forceinline__ int batch_find(const long long* __restrict__ off, int nprob,
+                                          long long it) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= it) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int INTERP, int ARITH, int TU, int MODE>
+__global__ __launch_bounds__(64 * TU) void k_batch_project(const BatchProb* __restrict__ probs,
+                                                           const long long* __restrict__ off,
+                                                           int nprob, int sub, int nsub,
+                                                           const float* __restrict__ ones) {
+    const long long total = off[nprob];
+    const int G = gridDim.x;
+    const int slot = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (long long base = 0;; base += G) {
+        const long long it = base + slot;
+        if (it >= total) break;
+        const int pi = batch_find(off, nprob, it);
+        const BatchProb& P = probs[pi];
+        const long long local = it - off[pi];
+        const int ntv = (P.dt.nv + 63) / 64, ntu = (P.dt.nu + TU - 1) / TU;
+        const int per_view = ntv * ntu;
+        const int pv = (int)(local / per_view);
+        const int rem = (int)(local - (long long)pv * per_view);
+        const int p = sub + pv * nsub;
+        const int j = (rem / ntv) * TU + wave;
+        const int k = (rem % ntv) * 64 + lane;
+        const bool active = k < P.dt.nv && j < P.dt.nu;
+        const size_t r = ((size_t)p * P.dt.nu + j) * P.dt.nv + k;
+        const bool used = active && (P.ind == nullptr || P.ind[r] == 1);
+        const float* vol = MODE == BFWD_ROWSUM ? ones : P.vol;
+        const float out = project_lane<INTERP, ARITH, float>(P.g, P.dt, p, j, k, used,
+                                                             P.zonly != 0, P.fv, vol);
+        if (active) {
+            if (MODE == BFWD_PLAIN) P.proj[r] = out;
+            else if (MODE == BFWD_ROWSUM) P.rowsum[r] = out;
+            else P.resid[r] = sart_resid(P.sino[r], out, P.rowsum[r], used);
+        }
+    }
+}
+
+template <int INTERP, int ARITH, int TU, int MODE>
+static const void* batch_project_kernel() {
+    return (const void*)&k_batch_project<INTERP, ARITH, TU, MODE>;
+}
+
+template <int INTERP, int ARITH, int TU>
+static const void* pick_mode(int mode) {
+    switch (mode) {
+        case BFWD_PLAIN: return batch_project_kernel<INTERP, ARITH, TU, BFWD_PLAIN>();
+        case BFWD_ROWSUM: return batch_project_kernel<INTERP, ARITH, TU, BFWD_ROWSUM>();
+        case BFWD_RESID: return batch_project_kernel<INTERP, ARITH, TU, BFWD_RESID>();
+    }
+    return nullptr;
+}
+
+template <int INTERP, int ARITH>
+static const void* pick_tu(int tu, int mode) {
+    switch (tu) {
+        case 2: return pick_mode<INTERP, ARITH, 2>(mode);
+        case 4: return pick_mode<INTERP, ARITH, 4>(mode);
+        case 8: return pick_mode<INTERP, ARITH, 8>(mode);
+    }
+    return nullptr;
+}
+
+static const void* batch_project_fn(int interp, int arith, int tu, int mode) {
+    if (interp == CT2_LINEAR)
+        return arith == CT2_ARITH_FAST ? pick_tu<CT2_LINEAR, CT2_ARITH_FAST>(tu, mode)
+                                       : pick_tu<CT2_LINEAR, CT2_ARITH_EXACT>(tu, mode);
+    return arith == CT2_ARITH_FAST ? pick_tu<CT2_NEAREST, CT2_ARITH_FAST>(tu, mode)
+                                   : pick_tu<CT2_NEAREST, CT2_ARITH_EXACT>(tu, mode);
+}
+
+int batch_project_occupancy(int interp, int arith, int tu, int mode, int* blocks_per_cu) {
+    const void* fn = batch_project_fn(interp, arith, tu, mode);
+    if (!fn) { set_error("batched projection: no kernel for tile %d, mode %d", tu, mode); return CT2_EINVAL; }
+    CT2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, 64 * tu, 0));
+    return CT2_OK;
+}
+
+int launch_batch_project(int interp, int arith, int tu, int mode, const BatchProb* probs,
+                         const long long* off, int nprob, int sub, int nsub, const float* ones,
+                         int grid, hipStream_t st) {
+    const void* fn = batch_project_fn(interp, arith, tu, mode);
+    if (!fn || grid <= 0 || (grid & 7)) {
+        set_error("batched projection: bad launch (tile %d, mode %d, grid %d)", tu, mode, grid);
+        return CT2_EINVAL;
+    }
+    void* args[] = {(void*)&probs, (void*)&off, (void*)&nprob, (void*)&sub, (void*)&nsub,
+                    (void*)&ones};
+    CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(64 * tu), args, 0, st));
+    return CT2_OK;
 }
 
 template <int INTERP>
@@ -151,23 +272,23 @@ __global__ __launch_bounds__(256) void k_project_rays(Grid g, int nr, int strict
 
 // ---------------------------------------------------------------- host ----
 template <int INTERP, int ARITH, typename VT>
-static void launch_views(const Grid& g, const Det& dt, int view0, int nviews, int zonly,
+static void launch_views(const Grid& g, const Det& dt, const ViewSet& vs, int zonly,
                          float* sino, const int8_t* ind, const float* fv, const void* vol,
                          hipStream_t st) {
-    dim3 grid((dt.nv + 63) / 64, (dt.nu + 3) / 4, nviews);
+    dim3 grid((dt.nv + 63) / 64, (dt.nu + 3) / 4, vs.nviews);
     hipLaunchKernelGGL((k_project_views<INTERP, ARITH, VT>), grid, dim3(256), 0, st, g, dt,
-                       view0, zonly, sino, ind, fv, (const VT*)vol);
+                       vs.view0, vs.vstride, zonly, sino, ind, fv, (const VT*)vol);
 }
 
 template <int INTERP, int ARITH>
-static int dispatch_dtype(const ct2_desc* d, const Grid& g, const Det& dt, int view0, int nv,
+static int dispatch_dtype(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                           float* sino, const int8_t* ind, const float* fv, const void* vol,
                           hipStream_t st) {
     int z = d->axis == CT2_AXIS_Z;
     switch (d->vol_dtype) {
-        case CT2_F32: launch_views<INTERP, ARITH, float>(g, dt, view0, nv, z, sino, ind, fv, vol, st); break;
-        case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, view0, nv, z, sino, ind, fv, vol, st); break;
-        case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, view0, nv, z, sino, ind, fv, vol, st); break;
+        case CT2_F32: launch_views<INTERP, ARITH, float>(g, dt, vs, z, sino, ind, fv, vol, st); break;
+        case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, vs, z, sino, ind, fv, vol, st); break;
+        case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, vs, z, sino, ind, fv, vol, st); break;
         default: set_error("ct2_project: unknown vol_dtype %d", d->vol_dtype); return CT2_EINVAL;
     }
     return CT2_OK;
@@ -179,21 +300,18 @@ int launch_project(const ct2_desc* d, void* sino, const int8_t* ind, const float
     Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
     int rc;
     if ((rc = check_grid(g)) || (rc = check_det(dt))) return rc;
-    int view0 = d->view0, nviews = d->nviews ? d->nviews : d->ns - d->view0;
-    if (view0 < 0 || nviews <= 0 || view0 + nviews > d->ns) {
-        set_error("ct2_project: bad view range [%d, %d) of %d", view0, view0 + nviews, d->ns);
-        return CT2_EINVAL;
-    }
+    ViewSet vs;
+    if ((rc = view_set(d, "ct2_project", vs))) return rc;
     if (!sino || !fv || !vol) { set_error("ct2_project: null pointer"); return CT2_EINVAL; }
     float* s = (float*)sino;
     if (d->arith == CT2_ARITH_EXACT) {
         rc = d->interp == CT2_LINEAR
-                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, s, ind, fv, vol, st)
-                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, view0, nviews, s, ind, fv, vol, st);
+                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, vs, s, ind, fv, vol, st)
+                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, vs, s, ind, fv, vol, st);
     } else if (d->arith == CT2_ARITH_FAST) {
         rc = d->interp == CT2_LINEAR
-                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, view0, nviews, s, ind, fv, vol, st)
-                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, view0, nviews, s, ind, fv, vol, st);
+                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, vs, s, ind, fv, vol, st)
+                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, vs, s, ind, fv, vol, st);
     } else {
         set_error("ct2_project: unknown arith %d", d->arith);
         return CT2_EINVAL;

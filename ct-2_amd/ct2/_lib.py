@@ -32,6 +32,12 @@ SIGNATURES = {
     "ct2_backproject": "PPPPPS",
     "ct2_sart_step": "PfPPPPPS",
     "ct2_convert_volume": "PPiziuS",
+    "ct2_batch_create": "PiPPPPPPPiiS",
+    "ct2_batch_destroy": "P",
+    "ct2_batch_info": "PPP",
+    "ct2_batch_project": "PS",
+    "ct2_batch_backproject": "PS",
+    "ct2_batch_sart": "PfiS",
     "ct2_sinoproj_linear": "PPP" + _V + "P" + _G + "S",
     "ct2_sinoproj_nearest": "PPP" + _V + "P" + _G + "S",
     "ct2_sinobackproject_linear": "PPP" + _V + "P" + _G + "S",
@@ -76,7 +82,7 @@ class ct2_desc(C.Structure):
                 ("x0", C.c_float), ("y0", C.c_float), ("z0", C.c_float),
                 ("interp", C.c_int), ("axis", C.c_int), ("vol_dtype", C.c_int),
                 ("arith", C.c_int), ("view0", C.c_int), ("nviews", C.c_int),
-                ("brick", C.c_int * 3)]
+                ("brick", C.c_int * 3), ("vstride", C.c_int)]
 
 
 _LIB = None

@@ -69,31 +69,35 @@ class ConeBeamOperator:
                    sino.v0, (image.nx, image.ny, image.nz), (image.dx, image.dy, image.dz),
                    (image.x0, image.y0, image.z0), indicator=sino.indicator, **kw)
 
-    def _desc(self, dtype=torch.float32, view0=0, nviews=0):
+    def _desc(self, dtype=torch.float32, view0=0, nviews=0, vstride=1):
         d = ct2_desc.from_buffer_copy(self.d)
         d.vol_dtype = _DT[dtype]
         d.view0 = view0
         d.nviews = nviews
+        d.vstride = vstride
         return d
 
-    def forward(self, vol, out=None, view0=0, nviews=0):
-        """A vol -> sinogram (float32).  vol: (nx,ny,nz) float32/float16/bfloat16."""
+    def forward(self, vol, out=None, view0=0, nviews=0, vstride=1):
+        """A vol -> sinogram (float32).  vol: (nx,ny,nz) float32/float16/bfloat16.
+        With a view subset (views view0 + j*vstride, j < nviews; nviews 0 =
+        all such views) only those views of ``out`` are written."""
         if vol.shape != self.shape or not vol.is_contiguous() or vol.dtype not in _DT:
             raise ValueError(f"volume must be a contiguous {self.shape} float32/16/bf16 tensor")
         if out is None:
             out = torch.empty(self.sino_shape, dtype=torch.float32, device=vol.device)
-        d = self._desc(vol.dtype, view0, nviews)
+        d = self._desc(vol.dtype, view0, nviews, vstride)
         call("ct2_project", C.byref(d), _dev.ptr(out), _dev.ptr(self.ind), _dev.ptr(self.fv),
              _dev.ptr(vol), _dev.stream_handle())
         return out
 
-    def adjoint(self, sino, out=None, dtype=torch.float32, view0=0, nviews=0):
-        """A^T sino -> volume written in `dtype` storage."""
+    def adjoint(self, sino, out=None, dtype=torch.float32, view0=0, nviews=0, vstride=1):
+        """A^T sino -> volume written in `dtype` storage (over a view subset as
+        in ``forward``)."""
         if sino.shape != self.sino_shape or sino.dtype != torch.float32 or not sino.is_contiguous():
             raise ValueError(f"sinogram must be a contiguous float32 {self.sino_shape} tensor")
         if out is None:
             out = torch.empty(self.shape, dtype=dtype, device=sino.device)
-        d = self._desc(out.dtype, view0, nviews)
+        d = self._desc(out.dtype, view0, nviews, vstride)
         call("ct2_backproject", C.byref(d), _dev.ptr(out), _dev.ptr(sino), _dev.ptr(self.ind),
              _dev.ptr(self.fv), _dev.stream_handle())
         return out
@@ -116,12 +120,12 @@ class ConeBeamOperator:
              _dev.ptr(vol), _dev.ptr(work), _dev.ptr(sens), _dev.stream_handle())
         return vol
 
-    def sart_step(self, sino, vol, lam=1.0, view0=0, nviews=0, work=None):
-        """SART update of vol in place over views [view0, view0+nviews)."""
+    def sart_step(self, sino, vol, lam=1.0, view0=0, nviews=0, work=None, vstride=1):
+        """SART update of vol in place over the views view0 + j*vstride."""
         n = int(np.prod(self.sino_shape)) * 2 + int(np.prod(self.shape)) * 3
         if work is None:
             work = torch.empty(n, dtype=torch.float32, device=vol.device)
-        d = self._desc(torch.float32, view0, nviews)
+        d = self._desc(torch.float32, view0, nviews, vstride)
         call("ct2_sart_step", C.byref(d), float(lam), _dev.ptr(sino), _dev.ptr(self.ind),
              _dev.ptr(self.fv), _dev.ptr(vol), _dev.ptr(work), _dev.stream_handle())
         return vol

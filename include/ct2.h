@@ -54,8 +54,10 @@ typedef struct ct2_desc {
     int axis;                          /* CT2_AXIS_FREE | CT2_AXIS_Z */
     int vol_dtype;                     /* storage type of the volume operand */
     int arith;                         /* CT2_ARITH_EXACT | CT2_ARITH_FAST */
-    int view0, nviews;                 /* view subset [view0, view0+nviews); nviews 0 = all */
+    int view0, nviews;                 /* view subset: views view0 + j*vstride, j < nviews; */
+                                       /* nviews 0 = every such view up to ns-1          */
     int brick[3];                      /* adjoint voxel brick (0 = autotuned/default) */
+    int vstride;                       /* view step of the subset (0 or 1 = contiguous) */
 } ct2_desc;
 
 /* ---------------------------------------------------------------- misc -- */
@@ -82,12 +84,45 @@ int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const in
  * unbiased and deterministic.  No counterpart in the reference (all REAL*4). */
 int ct2_convert_volume(const float* src, void* dst, int dtype, size_t n, int rounding,
                        unsigned seed, ct2_stream_t stream);
-/* SART update over the view subset of desc (no counterpart in the reference,
- * which has only sequential ART): x += lambda * A^T((b-Ax)/A1) / A^T1
- * restricted to those views, with work = 2*ns*nu*nv + 3*nx*ny*nz floats of
- * device scratch supplied by the caller. */
+/* SART update over the view subset of desc (views view0 + j*vstride; no
+ * counterpart in the reference, which has only sequential ART):
+ * x += lambda * A^T((b-Ax)/A1) / A^T1 restricted to those views, with
+ * work = 2*ns*nu*nv + 3*nx*ny*nz floats of device scratch supplied by the
+ * caller. */
 int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const int8_t* ind,
                   const float* fv, float* vol, float* work, ct2_stream_t stream);
+
+/* ------------------------------------------------------ batched plans -- */
+/* Many independent problems of uneven size in one persistent launch.  No
+ * reference counterpart (the reference runs one problem per call); the
+ * per-problem results equal those of ct2_project / ct2_backproject /
+ * ct2_sart_step.  A plan binds, per problem p: descriptor descs[p] (all
+ * problems share interp and arith; vol_dtype float32), device frame vectors,
+ * indicator (the array or its entries may be NULL), measured sinogram
+ * sinos[p], volume vols[p] (read by the forward, updated by SART), and
+ * optionally outputs proj_out[p] (forward of vols[p]) and adj_out[p]
+ * (adjoint of sinos[p]); the arrays of pointers are read at creation only.
+ * SART uses nsub ordered subsets of interleaved views (subset s = views s,
+ * s+nsub, ...; the same update as ct2_sart_step with view0 = s, vstride =
+ * nsub).  Creation allocates the row sums, residuals and nsub column-sum
+ * volumes, and with tune != 0 (unless CT2_AUTOTUNE=0) times forward tile
+ * heights {2,4,8} and adjoint bricks to pick the fastest; it synchronises
+ * the stream.  The other entry points only launch kernels on the stream
+ * (graph-capturable).  Calls on one plan must not overlap. */
+typedef struct ct2_batch ct2_batch;
+int ct2_batch_create(ct2_batch** plan, int nprob, const ct2_desc* descs,
+                     const float* const* frame_vectors, const int8_t* const* indicators,
+                     const float* const* sinos, float* const* vols, float* const* proj_out,
+                     float* const* adj_out, int nsub, int tune, ct2_stream_t stream);
+int ct2_batch_destroy(ct2_batch* plan);
+/* info[8] = {forward tile height, forward grid, brick x, y, z, adjoint grid,
+ * nsub, nprob}; *bytes = device memory held by the plan. */
+int ct2_batch_info(const ct2_batch* plan, int* info, size_t* bytes);
+int ct2_batch_project(ct2_batch* plan, ct2_stream_t stream);
+int ct2_batch_backproject(ct2_batch* plan, ct2_stream_t stream);
+/* niter SART iterations (each one pass over the nsub subsets) on every
+ * problem's volume in place. */
+int ct2_batch_sart(ct2_batch* plan, float lambda, int niter, ct2_stream_t stream);
 
 /* ------------------------------------------ view_proj3D.f replacements -- */
 /* view_proj3D.f:703 sinoproj_linear */
