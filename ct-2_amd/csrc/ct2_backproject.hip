@@ -8,7 +8,8 @@
 // touch the brick.  Each candidate ray is set up and sampled with the same
 // device functions as the forward kernel, so the matrix entries are the
 // forward's own, and every sample's corner weights that fall inside the
-// brick are added to LDS (ds_add_f32).  The brick is written to HBM once.
+// brick are added to an FP64 accumulator in LDS.  The brick is written to
+// HBM once.
 //
 // Linear weights per corner are (trav*wb)*wc and the contribution is
 // datum*weight, the reference's order; only the summation order differs.
@@ -31,6 +32,17 @@ struct Brick {
 // wavefront instruction per CU while ds_add_f32 takes ~190 (measured), and
 // the wider sum is also more accurate.
 __device__ __forceinline__ void lds_add(double* p, float v) { atomicAdd(p, (double)v); }
+
+// x / d for x >= 0 and d >= 1, with rd the FP32 reciprocal of d: below 2^22
+// the estimate is off by at most one either way and is corrected exactly;
+// larger x take the integer division.
+__device__ __forceinline__ int div_small(int x, int d, float rd) {
+    if (x >= (1 << 22)) return x / d;
+    int q = (int)((float)x * rd);
+    const int r = x - q * d;
+    q += (r >= d) - (r < 0);
+    return q;
+}
 
 // Detector footprint of an axis-aligned box for view p: rays from S through
 // every corner hit the detector plane D + u e_u + v e_v; returns the bin
@@ -85,9 +97,234 @@ __device__ __forceinline__ int brick_slot(const Brick& br, int x, int y, int z) 
     return (x + 1) * hy * hz + (y + 1) * hz + (z + 1);
 }
 
+// ---- pieces shared by the adjoint's accumulation paths ----
+// Sample points that can touch brick br lie in the box of voxel centres
+// lo-1 .. lo+ext; LDS keeps the brick with a one-voxel halo on every side:
+// local voxel (x,y,z), -1 <= x <= ext, lives at org + x*ls[0] + y*ls[1] + z,
+// so every corner of a sample whose cell touches the brick has a slot (halo
+// slots belong to neighbouring bricks and are dropped at write-back).
+struct BrickCtx {
+    double box_lo[3], box_hi[3];
+    int ls[3];
+    int org, nhalo;
+};
+
+__device__ __forceinline__ BrickCtx brick_ctx(const Grid& g, const Brick& br) {
+    BrickCtx c;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        c.box_lo[q] = (double)g.o[q] + ((double)(br.lo[q] - 1) + 0.5) * g.d[q];
+        c.box_hi[q] = (double)g.o[q] + ((double)(br.lo[q] + br.ext[q]) + 0.5) * g.d[q];
+    }
+    const int hy = br.ext[1] + 2, hz = br.ext[2] + 2;
+    c.ls[0] = hy * hz; c.ls[1] = hz; c.ls[2] = 1;
+    c.org = c.ls[0] + c.ls[1] + 1;
+    c.nhalo = (br.ext[0] + 2) * hy * hz;
+    return c;
+}
+
+// Detector bins [jlo,jhi] x [klo,khi] whose rays can touch the box for view
+// frame f (full detector when the box is not entirely in front of the
+// source).  Whole workgroup; call after a barrier; ends with a barrier, after
+// which s_range holds the range.
+__device__ void view_footprint(const Det& dt, const float* __restrict__ f, const BrickCtx& c,
+                               int* s_range) {
+    __shared__ double s_fp[8][2];
+    __shared__ int s_ok[8];
+    if (threadIdx.x < 8) {
+        int k = threadIdx.x;
+        double P[3] = {(k & 1) ? c.box_hi[0] : c.box_lo[0], (k & 2) ? c.box_hi[1] : c.box_lo[1],
+                       (k & 4) ? c.box_hi[2] : c.box_lo[2]};
+        double u = 0, v = 0;
+        int ok = 0;
+        project_corner(f, P, u, v, ok);
+        s_fp[k][0] = u;
+        s_fp[k][1] = v;
+        s_ok[k] = ok;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int allok = 1;
+        double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+        for (int k = 0; k < 8; ++k) {
+            allok &= s_ok[k];
+            umin = fmin(umin, s_fp[k][0]); umax = fmax(umax, s_fp[k][0]);
+            vmin = fmin(vmin, s_fp[k][1]); vmax = fmax(vmax, s_fp[k][1]);
+        }
+        int jlo = 0, jhi = dt.nu - 1, klo = 0, khi = dt.nv - 1;
+        if (allok) {
+            // bin j has centre u0 + (j+0.5) du; one bin of margin each side
+            double jl = (umin - dt.u0) / dt.du - 0.5, jh = (umax - dt.u0) / dt.du - 0.5;
+            double kl = (vmin - dt.v0) / dt.dv - 0.5, kh = (vmax - dt.v0) / dt.dv - 0.5;
+            if (dt.du < 0) { double t = jl; jl = jh; jh = t; }
+            if (dt.dv < 0) { double t = kl; kl = kh; kh = t; }
+            jl = fmax(jl, -2.0); jh = fmin(jh, (double)dt.nu + 1.0);
+            kl = fmax(kl, -2.0); kh = fmin(kh, (double)dt.nv + 1.0);
+            jlo = max(0, (int)floor(jl) - 1); jhi = min(dt.nu - 1, (int)ceil(jh) + 1);
+            klo = max(0, (int)floor(kl) - 1); khi = min(dt.nv - 1, (int)ceil(kh) + 1);
+        }
+        s_range[0] = jlo; s_range[1] = jhi; s_range[2] = klo; s_range[3] = khi;
+    }
+    __syncthreads();
+}
+
+// Candidate rays of a footprint, in the order lanes take them: every
+// KSTRIDE-th bin of a detector row (then the next phase), so the rays of one
+// wavefront sit >= ~1 voxel apart and their LDS atomics hit distinct words
+// (duplicate addresses in one ds_add instruction serialise, ~2.3x the cost
+// measured at 2 lanes/word).
+constexpr int KSTRIDE = 3;
+struct CandSet {
+    int jlo, klo, khi, row, mk, ncand;
+    float rrow, rmk;
+};
+
+__device__ __forceinline__ CandSet cand_set(const int* s_range) {
+    CandSet cs;
+    cs.jlo = s_range[0];
+    cs.klo = s_range[2];
+    cs.khi = s_range[3];
+    const int nj = s_range[1] - s_range[0] + 1, nk = s_range[3] - s_range[2] + 1;
+    cs.mk = (nk + KSTRIDE - 1) / KSTRIDE;
+    cs.row = KSTRIDE * cs.mk;
+    cs.ncand = (nj > 0 && nk > 0) ? nj * cs.row : 0;
+    cs.rrow = __builtin_amdgcn_rcpf((float)max(cs.row, 1));
+    cs.rmk = __builtin_amdgcn_rcpf((float)max(cs.mk, 1));
+    return cs;
+}
+
+// Candidate cidx of view p: detector bin (j, k), ray index r, datum; live is
+// false past the end, for bins outside the row, unused rays and zero data.
+struct Cand {
+    int j, k;
+    size_t r;
+    float datum;
+    bool live;
+};
+
+__device__ __forceinline__ Cand cand_at(const CandSet& cs, int cidx, int p, const Det& dt,
+                                        const int8_t* __restrict__ ind,
+                                        const float* __restrict__ sino) {
+    Cand c;
+    c.j = 0; c.k = 0; c.r = 0; c.datum = 0.0f;
+    c.live = cidx < cs.ncand;
+    if (c.live) {
+        const int jj = div_small(cidx, cs.row, cs.rrow);
+        const int q = cidx - jj * cs.row;
+        const int kk = div_small(q, cs.mk, cs.rmk);
+        c.j = cs.jlo + jj;
+        c.k = cs.klo + (q - kk * cs.mk) * KSTRIDE + kk;
+        c.live = c.k <= cs.khi;
+    }
+    if (c.live) {
+        c.r = ((size_t)p * dt.nu + c.j) * dt.nv + c.k;
+        c.live = ind == nullptr || ind[c.r] == 1;
+    }
+    if (c.live) {
+        c.datum = sino[c.r];
+        c.live = c.datum != 0.0f;
+    }
+    return c;
+}
+
+// One lane's candidate ray in fast arithmetic, relative to a brick: its
+// brick-relative plane range [t0, t0+tcnt) and the cell windows
+// [bl, bl+cntb) x [cl, cl+cntc) (the brick's cells intersected with the
+// forward's validity window), so one unsigned compare per axis decides a
+// sample; aoff folds the brick corner into the LDS address.
+struct LaneRay {
+    RayFast R;
+    int la0, lsa, lsb, lsc, aoff, t0, tcnt, bl, cntb, cl, cntc;
+};
+
+template <int INTERP>
+__device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const float* __restrict__ fv,
+                                         int p, const Cand& c, bool zonly, const Brick& br,
+                                         const BrickCtx& bc, LaneRay& L) {
+    L.R = RayFast{};
+    L.la0 = L.lsa = L.lsb = L.lsc = L.aoff = 0;
+    L.t0 = L.tcnt = L.bl = L.cntb = L.cl = L.cntc = 0;
+    if (!c.live) return;
+    double Sd[3], Bd[3];
+    view_ray_f64(fv, p, c.j, c.k, dt, Sd, Bd);
+    setup_fast(g, Sd, Bd, zonly, INTERP == CT2_LINEAR, L.R);
+    const RayFast& R = L.R;
+    L.la0 = sel3i(R.a, br.lo[0], br.lo[1], br.lo[2]);
+    const int la1 = L.la0 + sel3i(R.a, br.ext[0], br.ext[1], br.ext[2]);
+    const int lob = sel3i(R.b, br.lo[0], br.lo[1], br.lo[2]);
+    const int lobext = sel3i(R.b, br.ext[0], br.ext[1], br.ext[2]);
+    const int loc = sel3i(R.c, br.lo[0], br.lo[1], br.lo[2]);
+    const int locext = sel3i(R.c, br.ext[0], br.ext[1], br.ext[2]);
+    L.lsa = sel3i(R.a, bc.ls[0], bc.ls[1], bc.ls[2]);
+    L.lsb = sel3i(R.b, bc.ls[0], bc.ls[1], bc.ls[2]);
+    L.lsc = sel3i(R.c, bc.ls[0], bc.ls[1], bc.ls[2]);
+    L.aoff = -(lob * L.lsb + loc * L.lsc);
+    // cells that touch the brick: [lob-1, lob+ext) for linear cells (corners
+    // ib, ib+1), [lob, lob+ext) for nearest
+    const int clo = INTERP == CT2_LINEAR ? -1 : 0;
+    int l0 = L.la0, l1 = la1 - 1;
+    clip_planes_f(R.ub0, R.dub, (float)(lob + clo), (float)(lob + lobext), 0.05f, l0, l1);
+    clip_planes_f(R.uc0, R.duc, (float)(loc + clo), (float)(loc + locext), 0.05f, l0, l1);
+    L.t0 = l0 - L.la0;
+    L.tcnt = max(0, l1 - l0 + 1);
+    // the forward's validity window (march_fast): ib < nb-1 (linear), < nb (nearest)
+    const int hb = INTERP == CT2_LINEAR ? R.nb - 1 : R.nb;
+    const int hc = INTERP == CT2_LINEAR ? R.nc - 1 : R.nc;
+    L.bl = max(0, lob + clo);
+    L.cntb = max(0, min(hb - 1, lob + lobext - 1) - L.bl + 1);
+    L.cl = max(0, loc + clo);
+    L.cntc = max(0, min(hc - 1, loc + locext - 1) - L.cl + 1);
+}
+
+// Union of the wavefront's plane ranges (every lane takes part).
+__device__ __forceinline__ void wave_range(const LaneRay& L, int& w0, int& w1) {
+    w0 = L.tcnt > 0 ? L.t0 : 0x7fffffff;
+    w1 = L.tcnt > 0 ? L.t0 + L.tcnt - 1 : -1;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        w0 = min(w0, __shfl_xor(w0, off, 64));
+        w1 = max(w1, __shfl_xor(w1, off, 64));
+    }
+}
+
+// March the lanes' rays over brick planes [w0, w1] together (same LDS plane
+// per instruction; planes outside a ray's own range are masked) and hand
+// every corner contribution dtr*weight to add(slot, value).  lf is the plane
+// index as a float (exact), as the forward evaluates it.
+template <int INTERP, typename Add>
+__device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, int org, int w0, int w1,
+                                           Add add) {
+    const RayFast& R = L.R;
+    float lf = (float)(L.la0 + w0);
+    int base = org + w0 * L.lsa + L.aoff;
+    for (int t = w0; t <= w1; ++t, lf += 1.0f, base += L.lsa) {
+        const float ub = __builtin_fmaf(lf, R.dub, R.ub0);
+        const float uc = __builtin_fmaf(lf, R.duc, R.uc0);
+        const float fb = floorf(ub), fc = floorf(uc);
+        const int ib = (int)fb, ic = (int)fc;
+        if (!(((unsigned)(t - L.t0) < (unsigned)L.tcnt) &
+              ((unsigned)(ib - L.bl) < (unsigned)L.cntb) &
+              ((unsigned)(ic - L.cl) < (unsigned)L.cntc)))
+            continue;
+        const int a00 = base + __mul24(ib, L.lsb) + __mul24(ic, L.lsc);
+        if (INTERP == CT2_LINEAR) {
+            const float wb = ub - fb, wc = uc - fc;
+            const float tb0 = dtr * (1.0f - wb), tb1 = dtr * wb;
+            const float w1c = 1.0f - wc;
+            // cells in [lob-1, lob+ext): all four corners have a slot
+            add(a00, tb0 * w1c);
+            add(a00 + L.lsb, tb1 * w1c);
+            add(a00 + L.lsc, tb0 * wc);
+            add(a00 + L.lsb + L.lsc, tb1 * wc);
+        } else {
+            add(a00, dtr);
+        }
+    }
+}
+
 // Accumulate A^T sino over views view0 + pv*vstride (pv < nviews) for the
-// voxels of brick br into the LDS accumulator acc (zeroed here).  Called by
-// the whole workgroup; ends with a barrier, after which acc holds the sums.
+// voxels of brick br into the FP64 LDS accumulator acc (zeroed here).  Called
+// by the whole workgroup; ends with a barrier, after which acc holds the sums.
 template <int INTERP, int ARITH>
 __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nviews,
                                  int vstride, bool zonly, const Brick& br, double* acc,
@@ -95,74 +332,33 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
                                  const int8_t* __restrict__ ind,
                                  const float* __restrict__ fv) {
     __shared__ int s_range[4];
-    __shared__ double s_fp[8][2];
-    __shared__ int s_ok[8];
-    // LDS layout with a one-voxel halo on every side: local voxel (x,y,z),
-    // -1 <= x <= ext, lives at org + x*ls[0] + y*ls[1] + z.  Every corner of a
-    // sample whose cell touches the brick then has a slot; halo slots belong
-    // to neighbouring bricks and are dropped at write-back.
-    const int hy = br.ext[1] + 2, hz = br.ext[2] + 2;
-    const int nhalo = (br.ext[0] + 2) * hy * hz;
-    for (int q = threadIdx.x; q < nhalo; q += blockDim.x) acc[q] = 0.0;
-
-    // box containing every sample point that can touch the brick: centres of
-    // voxels lo-1 .. lo+ext (one voxel margin on both sides)
-    double box_lo[3], box_hi[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        box_lo[q] = (double)g.o[q] + ((double)(br.lo[q] - 1) + 0.5) * g.d[q];
-        box_hi[q] = (double)g.o[q] + ((double)(br.lo[q] + br.ext[q]) + 0.5) * g.d[q];
-    }
-    // LDS strides of x, y, z inside the brick
-    const int ls[3] = {hy * hz, hz, 1};
-    const int org = ls[0] + ls[1] + 1;
+    const BrickCtx bc = brick_ctx(g, br);
+    const int* ls = bc.ls;
+    const int org = bc.org;
+    for (int q = threadIdx.x; q < bc.nhalo; q += blockDim.x) acc[q] = 0.0;
+    const double cell_lo = (INTERP == CT2_LINEAR) ? -1.0 : 0.0;
 
     for (int pv = 0; pv < nviews; ++pv) {
         const int p = view0 + pv * vstride;
-        const float* f = fv + (size_t)p * 18;
         __syncthreads();   // previous view's readers of s_range are done
-        if (threadIdx.x < 8) {
-            int c = threadIdx.x;
-            double P[3] = {(c & 1) ? box_hi[0] : box_lo[0], (c & 2) ? box_hi[1] : box_lo[1],
-                           (c & 4) ? box_hi[2] : box_lo[2]};
-            double u = 0, v = 0;
-            int ok = 0;
-            project_corner(f, P, u, v, ok);
-            s_fp[c][0] = u;
-            s_fp[c][1] = v;
-            s_ok[c] = ok;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int allok = 1;
-            double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
-            for (int c = 0; c < 8; ++c) {
-                allok &= s_ok[c];
-                umin = fmin(umin, s_fp[c][0]); umax = fmax(umax, s_fp[c][0]);
-                vmin = fmin(vmin, s_fp[c][1]); vmax = fmax(vmax, s_fp[c][1]);
-            }
-            int jlo = 0, jhi = dt.nu - 1, klo = 0, khi = dt.nv - 1;
-            if (allok) {
-                // bin j has centre u0 + (j+0.5) du; one bin of margin each side
-                double jl = (umin - dt.u0) / dt.du - 0.5, jh = (umax - dt.u0) / dt.du - 0.5;
-                double kl = (vmin - dt.v0) / dt.dv - 0.5, kh = (vmax - dt.v0) / dt.dv - 0.5;
-                if (dt.du < 0) { double t = jl; jl = jh; jh = t; }
-                if (dt.dv < 0) { double t = kl; kl = kh; kh = t; }
-                jl = fmax(jl, -2.0); jh = fmin(jh, (double)dt.nu + 1.0);
-                kl = fmax(kl, -2.0); kh = fmin(kh, (double)dt.nv + 1.0);
-                jlo = max(0, (int)floor(jl) - 1); jhi = min(dt.nu - 1, (int)ceil(jh) + 1);
-                klo = max(0, (int)floor(kl) - 1); khi = min(dt.nv - 1, (int)ceil(kh) + 1);
-            }
-            s_range[0] = jlo; s_range[1] = jhi; s_range[2] = klo; s_range[3] = khi;
-        }
-        __syncthreads();
+        view_footprint(dt, fv + (size_t)p * 18, bc, s_range);
         const int jlo = s_range[0], jhi = s_range[1], klo = s_range[2], khi = s_range[3];
         if (jhi < jlo || khi < klo) continue;
-        // Lanes take every KSTRIDE-th bin of a detector row (then the next
-        // phase), so the rays of one wavefront sit >= ~1 voxel apart and their
-        // LDS atomics hit distinct words: duplicate addresses in one ds_add
-        // instruction serialise (~2.3x the cost measured at 2 lanes/word).
-        constexpr int KSTRIDE = 3;
+        if constexpr (ARITH == CT2_ARITH_FAST) {
+            const CandSet cs = cand_set(s_range);
+            const int lane = threadIdx.x & 63;
+            for (int c0 = threadIdx.x - lane; c0 < cs.ncand; c0 += blockDim.x) {
+                const Cand c = cand_at(cs, c0 + lane, p, dt, ind, sino);
+                LaneRay L;
+                lane_ray<INTERP>(g, dt, fv, p, c, zonly, br, bc, L);
+                int w0, w1;
+                wave_range(L, w0, w1);
+                if (w0 > w1) continue;   // wavefront-uniform
+                lane_march<INTERP>(L, c.datum * L.R.trav, org, w0, w1,
+                                   [&](int a, float v) { lds_add(&acc[a], v); });
+            }
+            continue;   // next view
+        }
         const int nk = khi - klo + 1;
         const int mk = (nk + KSTRIDE - 1) / KSTRIDE;
         const int row = KSTRIDE * mk;
@@ -176,62 +372,6 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
             if (ind != nullptr && ind[r] != 1) continue;
             const float datum = sino[r];
             if (datum == 0.0f) continue;
-            // cell range that touches the brick: [lob-1, lob+ext) for linear
-            // cells (corners ib, ib+1), [lob, lob+ext) for nearest
-            const double cell_lo = (INTERP == CT2_LINEAR) ? -1.0 : 0.0;
-            if constexpr (ARITH == CT2_ARITH_FAST) {
-                double Sd[3], Bd[3];
-                view_ray_f64(fv, p, j, k, dt, Sd, Bd);
-                RayFast R;
-                setup_fast(g, Sd, Bd, zonly, INTERP == CT2_LINEAR, R);
-                const int la0 = sel3i(R.a, br.lo[0], br.lo[1], br.lo[2]);
-                const int la1 = la0 + sel3i(R.a, br.ext[0], br.ext[1], br.ext[2]);
-                const int lob = sel3i(R.b, br.lo[0], br.lo[1], br.lo[2]);
-                const int lobext = sel3i(R.b, br.ext[0], br.ext[1], br.ext[2]);
-                const int loc = sel3i(R.c, br.lo[0], br.lo[1], br.lo[2]);
-                const int locext = sel3i(R.c, br.ext[0], br.ext[1], br.ext[2]);
-                const int lsa = sel3i(R.a, ls[0], ls[1], ls[2]);
-                const int lsb = sel3i(R.b, ls[0], ls[1], ls[2]);
-                const int lsc = sel3i(R.c, ls[0], ls[1], ls[2]);
-                int l0 = la0, l1 = la1 - 1;
-                clip_planes(R.ub0, R.dub, lob + cell_lo, lob + lobext, 0.05, l0, l1);
-                clip_planes(R.uc0, R.duc, loc + cell_lo, loc + locext, 0.05, l0, l1);
-                // the forward's validity window (march_fast)
-                const unsigned hb = (unsigned)(INTERP == CT2_LINEAR ? R.nb - 1 : R.nb);
-                const unsigned hc = (unsigned)(INTERP == CT2_LINEAR ? R.nc - 1 : R.nc);
-                const float dtr = datum * R.trav;
-                // lanes step the brick's planes together (same LDS plane per
-                // instruction); planes outside this ray's range are masked
-                int base = org;
-                for (int l = la0; l < la1; ++l, base += lsa) {
-                    const float ub = __builtin_fmaf((float)l, R.dub, R.ub0);
-                    const float uc = __builtin_fmaf((float)l, R.duc, R.uc0);
-                    const float fb = floorf(ub), fc = floorf(uc);
-                    const int ib = (int)fb, ic = (int)fc;
-                    const int qb = ib - lob, qc = ic - loc;
-                    const bool valid = (l >= l0) & (l <= l1) & ((unsigned)ib < hb) &
-                                       ((unsigned)ic < hc);
-                    if (INTERP == CT2_LINEAR) {
-                        if (!(valid & (qb >= -1) & (qb < lobext) & (qc >= -1) & (qc < locext)))
-                            continue;
-                        const float wb = ub - fb, wc = uc - fc;
-                        const float tb0 = dtr * (1.0f - wb), tb1 = dtr * wb;
-                        const float w1c = 1.0f - wc;
-                        // qb, qc >= -1 and qb+1, qc+1 <= ext: all four corners
-                        // have a slot (interior or halo)
-                        const int a00 = base + __mul24(qb, lsb) + __mul24(qc, lsc);
-                        lds_add(&acc[a00], tb0 * w1c);
-                        lds_add(&acc[a00 + lsb], tb1 * w1c);
-                        lds_add(&acc[a00 + lsc], tb0 * wc);
-                        lds_add(&acc[a00 + lsb + lsc], tb1 * wc);
-                    } else {
-                        if (!(valid & (qb >= 0) & (qb < lobext) & (qc >= 0) & (qc < locext)))
-                            continue;
-                        lds_add(&acc[base + __mul24(qb, lsb) + __mul24(qc, lsc)], dtr);
-                    }
-                }
-                continue;
-            }
             float S[3], B[3];
             view_ray(fv, p, j, k, dt, S, B);
             AxisSel A;
@@ -300,6 +440,9 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
     __syncthreads();
 }
 
+// LDS bytes of a brick accumulator (FP64 slots, one-voxel halo)
+size_t brick_lds_bytes(int3 b) { return (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double); }
+
 template <int INTERP, int ARITH, typename OT>
 __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int view0,
                                                            int nviews, int vstride, int zonly,
@@ -331,6 +474,19 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
 // batched forward.  Modes: BBP_PLAIN adj = A^T sino over all views;
 // BBP_COLSUM colsum_sub = A_sub^T resid (resid holds ones); BBP_SART
 // vol = sart_update(vol, lambda, A_sub^T resid, colsum_sub).
+// Final value c of voxel gi of problem P in batched mode MODE.
+template <int MODE>
+__device__ __forceinline__ void batch_store(const BatchProb& P, size_t gi, int sub, float lambda,
+                                            float c) {
+    if (MODE == BBP_PLAIN) {
+        P.adj[gi] = c;
+    } else if (MODE == BBP_COLSUM) {
+        P.colsum[(size_t)sub * P.nvox + gi] = c;
+    } else {
+        P.vol[gi] = sart_update(P.vol[gi], lambda, c, P.colsum[(size_t)sub * P.nvox + gi]);
+    }
+}
+
 template <int INTERP, int ARITH, int MODE>
 __global__ __launch_bounds__(256) void k_batch_backproject(const BatchProb* __restrict__ probs,
                                                            const long long* __restrict__ off,
@@ -363,20 +519,11 @@ __global__ __launch_bounds__(256) void k_batch_backproject(const BatchProb* __re
             int lx = q / (br.ext[2] * br.ext[1]);
             size_t gi = (size_t)(br.lo[0] + lx) * P.g.s[0] + (size_t)(br.lo[1] + ly) * P.g.s[1] +
                         (size_t)(br.lo[2] + lz) * P.g.s[2];
-            const float c = (float)acc[brick_slot(br, lx, ly, lz)];
-            if (MODE == BBP_PLAIN) {
-                P.adj[gi] = c;
-            } else if (MODE == BBP_COLSUM) {
-                P.colsum[(size_t)sub * P.nvox + gi] = c;
-            } else {
-                P.vol[gi] = sart_update(P.vol[gi], lambda, c, P.colsum[(size_t)sub * P.nvox + gi]);
-            }
+            batch_store<MODE>(P, gi, sub, lambda, (float)acc[brick_slot(br, lx, ly, lz)]);
         }
         __syncthreads();   // write-back done before the next brick clears acc
     }
 }
-
-size_t brick_lds_bytes(int3 b) { return (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double); }
 
 template <int INTERP, int ARITH>
 static const void* bbp_mode(int mode) {
@@ -461,9 +608,9 @@ __global__ __launch_bounds__(256) void k_backproject_rays(Grid g, int nr, int st
 // ---------------------------------------------------------------- host ----
 static int3 default_brick(const ct2_desc* d) {
     int3 b;
-    b.x = d->brick[0] > 0 ? d->brick[0] : 16;
-    b.y = d->brick[1] > 0 ? d->brick[1] : 16;
-    b.z = d->brick[2] > 0 ? d->brick[2] : 12;
+    b.x = d->brick[0] > 0 ? d->brick[0] : kBrickCandidates[0][0];
+    b.y = d->brick[1] > 0 ? d->brick[1] : kBrickCandidates[0][1];
+    b.z = d->brick[2] > 0 ? d->brick[2] : kBrickCandidates[0][2];
     return b;
 }
 
@@ -473,8 +620,9 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
                         hipStream_t st) {
     int3 b = default_brick(d);
     b.x = min(b.x, g.n[0]); b.y = min(b.y, g.n[1]); b.z = min(b.z, g.n[2]);
-    // brick accumulator with a one-voxel halo on every side
-    size_t lds = (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double);
+    // FP64 brick accumulator with a one-voxel halo on every side
+    const int zonly = d->axis == CT2_AXIS_Z ? 1 : 0;
+    size_t lds = brick_lds_bytes(b);
     if (lds > 63 * 1024) {
         set_error("ct2_backproject: brick %dx%dx%d needs %zu B of LDS (max 64512)", b.x, b.y,
                   b.z, lds);
@@ -484,8 +632,7 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
                         (g.n[2] + b.z - 1) / b.z);
     dim3 grid(nb.x * nb.y * nb.z);
     hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(256), lds, st, g, dt,
-                       vs.view0, vs.nviews, vs.vstride, d->axis == CT2_AXIS_Z ? 1 : 0, b, nb,
-                       (OT*)vol, sino, ind, fv);
+                       vs.view0, vs.nviews, vs.vstride, zonly, b, nb, (OT*)vol, sino, ind, fv);
     return CT2_OK;
 }
 
@@ -553,8 +700,7 @@ static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt,
     const char* env = getenv("CT2_AUTOTUNE");
     if (cs != hipStreamCaptureStatusNone || (env && env[0] == '0'))
         return launch_with(&e, g, dt, vs, vol, sn, ind, fv, st);
-    static const int cand[][3] = {{16, 16, 12}, {16, 16, 13}, {12, 12, 14}, {16, 12, 12},
-                                  {14, 14, 14}, {16, 16, 16}, {8, 16, 16}, {16, 16, 8}};
+    const auto& cand = kBrickCandidates;
     const int nc = (int)(sizeof cand / sizeof cand[0]);
     hipEvent_t ev[2 * 8];
     for (int c = 0; c < 2 * nc; ++c) CT2_HIP(hipEventCreate(&ev[c]));

@@ -24,7 +24,7 @@ using namespace ct2;
 struct ct2_batch {
     int nprob = 0, nsub = 1, interp = CT2_LINEAR, arith = CT2_ARITH_EXACT, device = 0;
     int tu = 4, fgrid = 8, bgrid = 8;
-    int3 bdim = make_int3(16, 16, 12);
+    int3 bdim = make_int3(kBrickCandidates[0][0], kBrickCandidates[0][1], kBrickCandidates[0][2]);
     std::vector<BatchProb> h;
     BatchProb* d_probs = nullptr;
     long long* d_foff = nullptr;   // (nsub+1) rows of nprob+1; row nsub = every view
@@ -240,11 +240,9 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
         if (!rc) rc = set_fwd(b, best_tu, st);
         // brick of the adjoint: time the column sums of subset 0
         if (!rc) rc = launch_fill(b->resid, 1.0f, nsino_tot, st);
-        static const int cand[][3] = {{16, 16, 12}, {16, 16, 13}, {12, 12, 14}, {16, 12, 12},
-                                      {14, 14, 14}, {16, 16, 16}, {8, 16, 16}, {16, 16, 8}};
         best = 1e30f;
         int3 best_b = b->bdim;
-        for (const auto& c : cand) {
+        for (const auto& c : kBrickCandidates) {
             if (rc) break;
             int3 bd = make_int3(c[0], c[1], c[2]);
             if ((rc = set_bp(b, bd, st))) break;

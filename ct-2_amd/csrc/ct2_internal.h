@@ -173,40 +173,55 @@ struct RayFast {
     float trav;
 };
 
+// FP64 reciprocal: the FP32 hardware reciprocal (relative error ~1e-7) refined
+// by one Newton-Raphson step r <- r(2 - x r), which squares the relative error
+// (~1e-14), for |x| within the FP32 normal range.
+__device__ __forceinline__ double rcp_f64(double x) {
+    const double r = (double)__builtin_amdgcn_rcpf((float)x);
+    return r * fma(-x, r, 2.0);
+}
+
+// FP64 where cancellation needs it (the entry coordinates ub0, uc0: source
+// distances of hundreds of voxels against a sub-voxel result), FP32 elsewhere;
+// no FP64 division or square root.
 __device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], const double B[3],
                                            bool zonly, bool linear, RayFast& R) {
-    const double dd[3] = {B[0] - S[0], B[1] - S[1], B[2] - S[2]};
-    const double inv[3] = {1.0 / (double)g.d[0], 1.0 / (double)g.d[1], 1.0 / (double)g.d[2]};
+    const double dd0 = B[0] - S[0], dd1 = B[1] - S[1], dd2 = B[2] - S[2];
     int a;
     if (zonly) {
         a = 2;
     } else {
-        double xad = fabs(dd[0]) * inv[0], yad = fabs(dd[1]) * inv[1], zad = fabs(dd[2]) * inv[2];
-        a = ((xad > yad) && (xad > zad)) ? 0 : ((yad > zad) ? 1 : 2);
+        // driving axis: largest |dd_q| / d_q, compared cross-multiplied
+        const float ax = fabsf((float)dd0), ay = fabsf((float)dd1), az = fabsf((float)dd2);
+        const float gx = fabsf(g.d[0]), gy = fabsf(g.d[1]), gz = fabsf(g.d[2]);
+        const bool xy = ax * gy > ay * gx, xz = ax * gz > az * gx, yz = ay * gz > az * gy;
+        a = (xy && xz) ? 0 : (yz ? 1 : 2);
     }
     const int b = (a == 0) ? 1 : 0;
     const int c = (a == 2) ? 1 : 2;
-    const double da = (a == 0) ? g.d[0] : (a == 1 ? g.d[1] : g.d[2]);
-    const double oa = (a == 0) ? g.o[0] : (a == 1 ? g.o[1] : g.o[2]);
-    const double ob = (b == 0) ? g.o[0] : g.o[1];
-    const double oc = (c == 1) ? g.o[1] : g.o[2];
-    const double ib = (b == 0) ? inv[0] : inv[1];
-    const double ic = (c == 1) ? inv[1] : inv[2];
+    const double dda = (a == 0) ? dd0 : (a == 1 ? dd1 : dd2);
+    const double ddb = (b == 0) ? dd0 : dd1;
+    const double ddc = (c == 1) ? dd1 : dd2;
     const double Sa = (a == 0) ? S[0] : (a == 1 ? S[1] : S[2]);
     const double Sb = (b == 0) ? S[0] : S[1];
     const double Sc = (c == 1) ? S[1] : S[2];
-    const double ddb = (b == 0) ? dd[0] : dd[1];
-    const double ddc = (c == 1) ? dd[1] : dd[2];
-    const double dda = (a == 0) ? dd[0] : (a == 1 ? dd[1] : dd[2]);
-    const double ia = 1.0 / dda;
+    const float da = sel3(a, g.d[0], g.d[1], g.d[2]);
+    const float oa = sel3(a, g.o[0], g.o[1], g.o[2]);
+    const float db = (b == 0) ? g.d[0] : g.d[1];
+    const float ob = (b == 0) ? g.o[0] : g.o[1];
+    const float dc = (c == 1) ? g.d[1] : g.d[2];
+    const float oc = (c == 1) ? g.o[1] : g.o[2];
+    const double ia = rcp_f64(dda);
+    const double idb = rcp_f64((double)db), idc = rcp_f64((double)dc);
     const double r1 = ddb * ia, r2 = ddc * ia;
-    const double t0 = oa + 0.5 * da;   // centre of plane 0
+    const double ta = ((double)oa + 0.5 * (double)da) - Sa;   // centre of plane 0 from the source
     const double half = linear ? 0.5 : 0.0;
-    R.ub0 = (float)((Sb + r1 * (t0 - Sa) - ob) * ib - half);
-    R.uc0 = (float)((Sc + r2 * (t0 - Sa) - oc) * ic - half);
-    R.dub = (float)(r1 * da * ib);
-    R.duc = (float)(r2 * da * ic);
-    R.trav = (float)(da * sqrt(1.0 + r1 * r1 + r2 * r2));
+    R.ub0 = (float)((fma(r1, ta, Sb) - (double)ob) * idb - half);
+    R.uc0 = (float)((fma(r2, ta, Sc) - (double)oc) * idc - half);
+    R.dub = (float)(r1 * ((double)da * idb));
+    R.duc = (float)(r2 * ((double)da * idc));
+    const float r1f = (float)r1, r2f = (float)r2;
+    R.trav = fabsf(da) * sqrtf(1.0f + r1f * r1f + r2f * r2f);
     R.a = a; R.b = b; R.c = c;
     R.na = sel3i(a, g.n[0], g.n[1], g.n[2]);
     R.nb = sel3i(b, g.n[0], g.n[1], g.n[2]);
@@ -247,6 +262,24 @@ __device__ __forceinline__ void clip_planes(double u0, double du, double lo, dou
     if (!(tmin <= tmax)) { l1 = l0 - 1; return; }
     l0 = max(l0, (int)floor(tmin) - 1);
     l1 = min(l1, (int)ceil(tmax) + 1);
+}
+
+// FP32 form of clip_planes for the fast arithmetic (same widening; the
+// hardware reciprocal's 1-ulp error is far inside the pad).
+__device__ __forceinline__ void clip_planes_f(float u0, float du, float lo, float hi, float pad,
+                                              int& l0, int& l1) {
+    lo -= pad;
+    hi += pad;
+    if (fabsf(du) < 1e-20f) {
+        if (!(u0 >= lo && u0 < hi)) l1 = l0 - 1;
+        return;
+    }
+    const float idu = __builtin_amdgcn_rcpf(du);
+    const float ta = (lo - u0) * idu, tb = (hi - u0) * idu;
+    const float tmin = fmaxf(fminf(ta, tb), -1e9f), tmax = fminf(fmaxf(ta, tb), 1e9f);
+    if (!(tmin <= tmax)) { l1 = l0 - 1; return; }
+    l0 = max(l0, (int)floorf(tmin) - 1);
+    l1 = min(l1, (int)ceilf(tmax) + 1);
 }
 
 // ------------------------------------------------------- storage types ----
@@ -330,6 +363,10 @@ int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* p
                              float lambda, int grid, hipStream_t st);
 int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int* blocks_per_cu);
 size_t brick_lds_bytes(int3 bdim);
+// adjoint brick shapes the autotuners time (single problems and batched plans)
+inline constexpr int kBrickCandidates[8][3] = {{12, 12, 14}, {14, 14, 14}, {16, 12, 14},
+                                               {12, 16, 14}, {16, 16, 12}, {16, 16, 16},
+                                               {8, 16, 16}, {16, 16, 8}};
 
 // View subset of a descriptor: views view0 + j*vstride for j < nviews.
 struct ViewSet {

@@ -106,10 +106,10 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
             setup_fast(g, S, B, zonly, INTERP == CT2_LINEAR, R);
             // planes where the ray can be inside the volume
             int a0 = 0, a1 = R.na - 1;
-            const double hb = INTERP == CT2_LINEAR ? R.nb - 1 : R.nb;
-            const double hc = INTERP == CT2_LINEAR ? R.nc - 1 : R.nc;
-            clip_planes(R.ub0, R.dub, 0.0, hb, 0.05, a0, a1);
-            clip_planes(R.uc0, R.duc, 0.0, hc, 0.05, a0, a1);
+            const float hb = INTERP == CT2_LINEAR ? R.nb - 1 : R.nb;
+            const float hc = INTERP == CT2_LINEAR ? R.nc - 1 : R.nc;
+            clip_planes_f(R.ub0, R.dub, 0.0f, hb, 0.05f, a0, a1);
+            clip_planes_f(R.uc0, R.duc, 0.0f, hc, 0.05f, a0, a1);
             if (a0 <= a1) { l0 = a0; l1 = a1; }
         }
         // union over the wavefront (every lane takes part in the shuffles)
