@@ -284,17 +284,34 @@ __device__ __forceinline__ void clip_planes_f(float u0, float du, float lo, floa
 
 // ------------------------------------------------------- storage types ----
 template <typename T> struct Store;
+// Element at byte offset off of a uniform base pointer: a 32-bit offset
+// lets the compiler address the load as base register + offset with no
+// 64-bit pointer arithmetic per sample.
+template <typename VT>
+__device__ __forceinline__ const VT* at_byte(const VT* base, unsigned off) {
+    return (const VT*)((const char*)base + off);
+}
+
 template <> struct Store<float> {
     __device__ __forceinline__ static float ld(const float* p, int i) { return p[i]; }
+    __device__ __forceinline__ static float ldb(const float* p, unsigned off) {
+        return *at_byte(p, off);
+    }
     __device__ __forceinline__ static void st(float* p, size_t i, float v) { p[i] = v; }
 };
 template <> struct Store<__half> {
     __device__ __forceinline__ static float ld(const __half* p, int i) { return __half2float(p[i]); }
+    __device__ __forceinline__ static float ldb(const __half* p, unsigned off) {
+        return __half2float(*at_byte(p, off));
+    }
     __device__ __forceinline__ static void st(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
 };
 template <> struct Store<hip_bfloat16> {
     __device__ __forceinline__ static float ld(const hip_bfloat16* p, int i) {
         return static_cast<float>(p[i]);
+    }
+    __device__ __forceinline__ static float ldb(const hip_bfloat16* p, unsigned off) {
+        return static_cast<float>(*at_byte(p, off));
     }
     __device__ __forceinline__ static void st(hip_bfloat16* p, size_t i, float v) {
         p[i] = hip_bfloat16(v);

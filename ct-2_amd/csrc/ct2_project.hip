@@ -11,18 +11,34 @@
 
 namespace ct2 {
 
+// Volumes addressed with 24-bit index multiplies and 32-bit byte offsets:
+// every stride and index below 2^24 and the volume below 4 GiB.  The check
+// is uniform (kernel arguments), so each ray picks its loop once.
+template <typename VT>
+__device__ __forceinline__ bool grid_small(const Grid& g) {
+    return g.s[0] < (1 << 24) && g.n[0] < (1 << 24) &&
+           (unsigned long long)g.n[0] * (unsigned)g.s[0] * sizeof(VT) < (1ull << 32);
+}
+
+// Element offset of cell (ib, ic) of a plane (SMALL: 24-bit multiplies).
+template <bool SMALL>
+__device__ __forceinline__ int cell_offset(int ib, int sb, int ic, int sc) {
+    return SMALL ? __mul24(ib, sb) + __mul24(ic, sc) : ib * sb + ic * sc;
+}
+
 // March one ray and return its sum; reference FP32 order (sequential sum
 // over planes), so the result is bit-identical to the Fortran.
-template <int INTERP, typename VT>
+template <int INTERP, typename VT, bool SMALL>
 __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restrict__ vol,
                                              bool strict) {
 #pragma clang fp contract(off)
     float total = 0.0f;
-    for (int l = 0; l < A.na; ++l) {
+    int base = 0;   // l * sa
+    for (int l = 0; l < A.na; ++l, base += A.sa) {
         Sample sm;
         if (INTERP == CT2_LINEAR) {
             if (!sample_linear_exact(A, l, sm)) continue;
-            int i00 = l * A.sa + sm.ib * A.sb + sm.ic * A.sc;
+            int i00 = base + cell_offset<SMALL>(sm.ib, A.sb, sm.ic, A.sc);
             float v00 = Store<VT>::ld(vol, i00);
             float v10 = Store<VT>::ld(vol, i00 + A.sb);
             float v01 = Store<VT>::ld(vol, i00 + A.sc);
@@ -33,7 +49,7 @@ __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restr
             total = total + A.trav * val;
         } else {
             if (!sample_nearest_exact(A, l, strict, sm)) continue;
-            int i00 = l * A.sa + sm.ib * A.sb + sm.ic * A.sc;
+            int i00 = base + cell_offset<SMALL>(sm.ib, A.sb, sm.ic, A.sc);
             total = total + A.trav * Store<VT>::ld(vol, i00);
         }
     }
@@ -42,7 +58,7 @@ __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restr
 
 // Fast arithmetic: per-ray setup in FP64, the transverse cell coordinate of
 // plane l is one FMA (u(l) = u0 + l*du), FP32 bilinear lerps, compensated sum.
-template <int INTERP, typename VT>
+template <int INTERP, typename VT, bool SMALL>
 __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restrict__ vol,
                                             bool strict, int l0, int l1) {
     float sum = 0.0f, comp = 0.0f;
@@ -50,20 +66,34 @@ __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restri
     const unsigned hc = (unsigned)(INTERP == CT2_LINEAR ? R.nc - 1 : (strict ? R.nc - 1 : R.nc));
     // [l0, l1] is wavefront-uniform: all lanes step the same plane together,
     // so one load instruction touches one plane's neighbourhood
-    for (int l = l0; l <= l1; ++l) {
-        float ub = __builtin_fmaf((float)l, R.dub, R.ub0);
-        float uc = __builtin_fmaf((float)l, R.duc, R.uc0);
+    // lf is the plane index as a float (exact), base its element offset
+    float lf = (float)l0;
+    int base = l0 * R.sa;
+    for (int l = l0; l <= l1; ++l, lf += 1.0f, base += R.sa) {
+        float ub = __builtin_fmaf(lf, R.dub, R.ub0);
+        float uc = __builtin_fmaf(lf, R.duc, R.uc0);
         float fb = floorf(ub), fc = floorf(uc);
         int ib = (int)fb, ic = (int)fc;
         if ((unsigned)ib >= hb || (unsigned)ic >= hc) continue;
-        int i00 = l * R.sa + ib * R.sb + ic * R.sc;
+        int i00 = base + cell_offset<SMALL>(ib, R.sb, ic, R.sc);
         float val;
         if (INTERP == CT2_LINEAR) {
             float wb = ub - fb, wc = uc - fc;
-            float v00 = Store<VT>::ld(vol, i00);
-            float v10 = Store<VT>::ld(vol, i00 + R.sb);
-            float v01 = Store<VT>::ld(vol, i00 + R.sc);
-            float v11 = Store<VT>::ld(vol, i00 + R.sb + R.sc);
+            float v00, v10, v01, v11;
+            if (SMALL) {
+                const unsigned o = (unsigned)i00 * (unsigned)sizeof(VT);
+                const unsigned db = (unsigned)R.sb * (unsigned)sizeof(VT);
+                const unsigned dc = (unsigned)R.sc * (unsigned)sizeof(VT);
+                v00 = Store<VT>::ldb(vol, o);
+                v10 = Store<VT>::ldb(vol, o + db);
+                v01 = Store<VT>::ldb(vol, o + dc);
+                v11 = Store<VT>::ldb(vol, o + db + dc);
+            } else {
+                v00 = Store<VT>::ld(vol, i00);
+                v10 = Store<VT>::ld(vol, i00 + R.sb);
+                v01 = Store<VT>::ld(vol, i00 + R.sc);
+                v11 = Store<VT>::ld(vol, i00 + R.sb + R.sc);
+            }
             float vi1 = __builtin_fmaf(wb, v10 - v00, v00);
             float vi2 = __builtin_fmaf(wb, v11 - v01, v01);
             val = __builtin_fmaf(wc, vi2 - vi1, vi1);
@@ -95,7 +125,8 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
             view_ray(fv, p, j, k, dt, S, B);
             AxisSel A;
             setup_exact(g, S, B, zonly, A);
-            out = march_exact<INTERP, VT>(A, vol, false);
+            out = grid_small<VT>(g) ? march_exact<INTERP, VT, true>(A, vol, false)
+                                    : march_exact<INTERP, VT, false>(A, vol, false);
         }
     } else {
         RayFast R;
@@ -118,7 +149,11 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
             l0 = min(l0, __shfl_xor(l0, off, 64));
             l1 = max(l1, __shfl_xor(l1, off, 64));
         }
-        if (use) out = march_fast<INTERP, VT>(R, vol, false, l0, min(l1, R.na - 1));
+        if (use) {
+            l1 = min(l1, R.na - 1);
+            out = grid_small<VT>(g) ? march_fast<INTERP, VT, true>(R, vol, false, l0, l1)
+                                    : march_fast<INTERP, VT, false>(R, vol, false, l0, l1);
+        }
     }
     return out;
 }
@@ -265,7 +300,8 @@ __global__ __launch_bounds__(256) void k_project_rays(Grid g, int nr, int strict
         }
         AxisSel A;
         setup_exact(g, S, B, false, A);
-        out = march_exact<INTERP, float>(A, vol, strict != 0);
+        out = grid_small<float>(g) ? march_exact<INTERP, float, true>(A, vol, strict != 0)
+                                   : march_exact<INTERP, float, false>(A, vol, strict != 0);
     }
     sino[r] = out;
 }
