@@ -33,13 +33,11 @@ struct Brick {
 // the wider sum is also more accurate.
 __device__ __forceinline__ void lds_add(double* p, float v) { atomicAdd(p, (double)v); }
 
-// x / d for x >= 0 and d >= 1, with rd the FP32 reciprocal of d: below 2^22
-// the estimate is off by at most one either way and is corrected exactly;
-// larger x take the integer division.
+// x / d for 0 <= x < 2^22 and d >= 1, with rd the FP32 reciprocal of d: the
+// estimate is off by at most one either way and is corrected exactly.
 __device__ __forceinline__ int div_small(int x, int d, float rd) {
-    if (x >= (1 << 22)) return x / d;
     int q = (int)((float)x * rd);
-    const int r = x - q * d;
+    const int r = x - __mul24(q, d);
     q += (r >= d) - (r < 0);
     return q;
 }
@@ -169,14 +167,17 @@ __device__ void view_footprint(const Det& dt, const float* __restrict__ f, const
 }
 
 // Candidate rays of a footprint, in the order lanes take them: every
-// KSTRIDE-th bin of a detector row (then the next phase), so the rays of one
-// wavefront sit >= ~1 voxel apart and their LDS atomics hit distinct words
-// (duplicate addresses in one ds_add instruction serialise, ~2.3x the cost
-// measured at 2 lanes/word).
-constexpr int KSTRIDE = 3;
+// KSTRIDE-th bin of a detector row (then the next phase), so neighbouring
+// lanes' rays sit about a voxel apart and their four corners mostly hit
+// distinct LDS words (duplicate addresses in one ds_add instruction
+// serialise).  Measured at 256^3 x 360 views: stride 1 63 ms, 2 50.5 ms,
+// 3 53.7 ms, 4 56.3 ms -- wider strides scatter the sinogram reads of a
+// wavefront over more cache lines.
+constexpr int KSTRIDE = 2;
 struct CandSet {
     int jlo, klo, khi, row, mk, ncand;
     float rrow, rmk;
+    bool small;   // candidate numbers below 2^22: division by reciprocal
 };
 
 __device__ __forceinline__ CandSet cand_set(const int* s_range) {
@@ -190,6 +191,7 @@ __device__ __forceinline__ CandSet cand_set(const int* s_range) {
     cs.ncand = (nj > 0 && nk > 0) ? nj * cs.row : 0;
     cs.rrow = __builtin_amdgcn_rcpf((float)max(cs.row, 1));
     cs.rmk = __builtin_amdgcn_rcpf((float)max(cs.mk, 1));
+    cs.small = cs.ncand < (1 << 22);
     return cs;
 }
 
@@ -209,15 +211,28 @@ __device__ __forceinline__ Cand cand_at(const CandSet& cs, int cidx, int p, cons
     c.j = 0; c.k = 0; c.r = 0; c.datum = 0.0f;
     c.live = cidx < cs.ncand;
     if (c.live) {
-        const int jj = div_small(cidx, cs.row, cs.rrow);
-        const int q = cidx - jj * cs.row;
-        const int kk = div_small(q, cs.mk, cs.rmk);
+        int jj, q, kk;
+        if (cs.small) {   // uniform
+            jj = div_small(cidx, cs.row, cs.rrow);
+            q = cidx - __mul24(jj, cs.row);
+            kk = div_small(q, cs.mk, cs.rmk);
+        } else {
+            jj = cidx / cs.row;
+            q = cidx - jj * cs.row;
+            kk = q / cs.mk;
+        }
         c.j = cs.jlo + jj;
         c.k = cs.klo + (q - kk * cs.mk) * KSTRIDE + kk;
         c.live = c.k <= cs.khi;
     }
     if (c.live) {
-        c.r = ((size_t)p * dt.nu + c.j) * dt.nv + c.k;
+        // ray index: a 24-bit multiply while both factors fit 24 bits and the
+        // sinogram has fewer than 2^32 bins
+        const int rowi = p * dt.nu + c.j;
+        c.r = (rowi < (1 << 24) && dt.nv < (1 << 24) &&
+               (unsigned long long)dt.ns * dt.nu * dt.nv < (1ull << 32))
+                  ? (size_t)(unsigned)__umul24((unsigned)rowi, (unsigned)dt.nv) + (unsigned)c.k
+                  : (size_t)rowi * dt.nv + c.k;
         c.live = ind == nullptr || ind[c.r] == 1;
     }
     if (c.live) {
@@ -258,7 +273,7 @@ __device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const flo
     L.lsa = sel3i(R.a, bc.ls[0], bc.ls[1], bc.ls[2]);
     L.lsb = sel3i(R.b, bc.ls[0], bc.ls[1], bc.ls[2]);
     L.lsc = sel3i(R.c, bc.ls[0], bc.ls[1], bc.ls[2]);
-    L.aoff = -(lob * L.lsb + loc * L.lsc);
+    L.aoff = -(__mul24(lob, L.lsb) + __mul24(loc, L.lsc));
     // cells that touch the brick: [lob-1, lob+ext) for linear cells (corners
     // ib, ib+1), [lob, lob+ext) for nearest
     const int clo = INTERP == CT2_LINEAR ? -1 : 0;
@@ -296,7 +311,7 @@ __device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, int org,
                                            Add add) {
     const RayFast& R = L.R;
     float lf = (float)(L.la0 + w0);
-    int base = org + w0 * L.lsa + L.aoff;
+    int base = org + __mul24(w0, L.lsa) + L.aoff;
     for (int t = w0; t <= w1; ++t, lf += 1.0f, base += L.lsa) {
         const float ub = __builtin_fmaf(lf, R.dub, R.ub0);
         const float uc = __builtin_fmaf(lf, R.duc, R.uc0);

@@ -212,12 +212,15 @@ __device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], con
     const float dc = (c == 1) ? g.d[1] : g.d[2];
     const float oc = (c == 1) ? g.o[1] : g.o[2];
     const double ia = rcp_f64(dda);
-    const double idb = rcp_f64((double)db), idc = rcp_f64((double)dc);
+    // voxel-size reciprocals before the axis selection (loop invariant)
+    const double id0 = rcp_f64((double)g.d[0]), id1 = rcp_f64((double)g.d[1]);
+    const double id2 = rcp_f64((double)g.d[2]);
+    const double idb = (b == 0) ? id0 : id1, idc = (c == 1) ? id1 : id2;
     const double r1 = ddb * ia, r2 = ddc * ia;
-    const double ta = ((double)oa + 0.5 * (double)da) - Sa;   // centre of plane 0 from the source
+    const double ta = fma(0.5, (double)da, (double)oa) - Sa;   // centre of plane 0 from the source
     const double half = linear ? 0.5 : 0.0;
-    R.ub0 = (float)((fma(r1, ta, Sb) - (double)ob) * idb - half);
-    R.uc0 = (float)((fma(r2, ta, Sc) - (double)oc) * idc - half);
+    R.ub0 = (float)fma(fma(r1, ta, Sb) - (double)ob, idb, -half);
+    R.uc0 = (float)fma(fma(r2, ta, Sc) - (double)oc, idc, -half);
     R.dub = (float)(r1 * ((double)da * idb));
     R.duc = (float)(r2 * ((double)da * idc));
     const float r1f = (float)r1, r2f = (float)r2;
@@ -235,12 +238,12 @@ __device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], con
 __device__ __forceinline__ void view_ray_f64(const float* __restrict__ fv, int p, int j, int k,
                                              const Det& dt, double S[3], double B[3]) {
     const float* f = fv + (size_t)p * 18;
-    const double u = (double)dt.u0 + ((double)j + 0.5) * (double)dt.du;
-    const double v = (double)dt.v0 + ((double)k + 0.5) * (double)dt.dv;
+    const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
+    const double v = fma((double)k + 0.5, (double)dt.dv, (double)dt.v0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         S[q] = f[q];
-        B[q] = (double)f[6 + q] + (double)f[9 + q] * u + (double)f[12 + q] * v;
+        B[q] = fma((double)f[12 + q], v, fma((double)f[9 + q], u, (double)f[6 + q]));
     }
 }
 

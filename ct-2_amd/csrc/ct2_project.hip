@@ -7,6 +7,12 @@
 //
 // Reference: emil_code/view_proj3D.f:703-910 (linear), :915-1092 (nearest),
 // view_proj3D_tomo.f:4-148 (z forced), ray_proj3D.f:514-804 (ray lists).
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -158,14 +164,16 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
     return out;
 }
 
-template <int INTERP, int ARITH, typename VT>
-__global__ __launch_bounds__(256) void k_project_views(Grid g, Det dt, int view0, int vstride,
-                                                       int zonly, float* __restrict__ sino,
-                                                       const int8_t* __restrict__ ind,
-                                                       const float* __restrict__ fv,
-                                                       const VT* __restrict__ vol) {
+// A workgroup covers TU detector columns x 64 rows of one view.
+template <int INTERP, int ARITH, typename VT, int TU>
+__global__ __launch_bounds__(64 * TU) void k_project_views(Grid g, Det dt, int view0,
+                                                           int vstride, int zonly,
+                                                           float* __restrict__ sino,
+                                                           const int8_t* __restrict__ ind,
+                                                           const float* __restrict__ fv,
+                                                           const VT* __restrict__ vol) {
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int j = blockIdx.y * TU + (threadIdx.x >> 6);
     const int p = view0 + blockIdx.z * vstride;
     const bool active = k < dt.nv && j < dt.nu;
     const size_t r = ((size_t)p * dt.nu + j) * dt.nv + k;
@@ -307,26 +315,125 @@ __global__ __launch_bounds__(256) void k_project_rays(Grid g, int nr, int strict
 }
 
 // ---------------------------------------------------------------- host ----
+template <int INTERP, int ARITH, typename VT, int TU>
+static void launch_tile(const Grid& g, const Det& dt, const ViewSet& vs, int zonly, float* sino,
+                        const int8_t* ind, const float* fv, const void* vol, hipStream_t st) {
+    dim3 grid((dt.nv + 63) / 64, (dt.nu + TU - 1) / TU, vs.nviews);
+    hipLaunchKernelGGL((k_project_views<INTERP, ARITH, VT, TU>), grid, dim3(64 * TU), 0, st, g,
+                       dt, vs.view0, vs.vstride, zonly, sino, ind, fv, (const VT*)vol);
+}
+
 template <int INTERP, int ARITH, typename VT>
-static void launch_views(const Grid& g, const Det& dt, const ViewSet& vs, int zonly,
+static void launch_views(const Grid& g, const Det& dt, const ViewSet& vs, int zonly, int tu,
                          float* sino, const int8_t* ind, const float* fv, const void* vol,
                          hipStream_t st) {
-    dim3 grid((dt.nv + 63) / 64, (dt.nu + 3) / 4, vs.nviews);
-    hipLaunchKernelGGL((k_project_views<INTERP, ARITH, VT>), grid, dim3(256), 0, st, g, dt,
-                       vs.view0, vs.vstride, zonly, sino, ind, fv, (const VT*)vol);
+    switch (tu) {
+        case 2: launch_tile<INTERP, ARITH, VT, 2>(g, dt, vs, zonly, sino, ind, fv, vol, st); break;
+        case 8: launch_tile<INTERP, ARITH, VT, 8>(g, dt, vs, zonly, sino, ind, fv, vol, st); break;
+        default: launch_tile<INTERP, ARITH, VT, 4>(g, dt, vs, zonly, sino, ind, fv, vol, st); break;
+    }
 }
 
 template <int INTERP, int ARITH>
 static int dispatch_dtype(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
-                          float* sino, const int8_t* ind, const float* fv, const void* vol,
-                          hipStream_t st) {
+                          int tu, float* sino, const int8_t* ind, const float* fv,
+                          const void* vol, hipStream_t st) {
     int z = d->axis == CT2_AXIS_Z;
     switch (d->vol_dtype) {
-        case CT2_F32: launch_views<INTERP, ARITH, float>(g, dt, vs, z, sino, ind, fv, vol, st); break;
-        case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, vs, z, sino, ind, fv, vol, st); break;
-        case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, vs, z, sino, ind, fv, vol, st); break;
+        case CT2_F32: launch_views<INTERP, ARITH, float>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
+        case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
+        case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         default: set_error("ct2_project: unknown vol_dtype %d", d->vol_dtype); return CT2_EINVAL;
     }
+    return CT2_OK;
+}
+
+static int launch_tu(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs, int tu,
+                     float* s, const int8_t* ind, const float* fv, const void* vol,
+                     hipStream_t st) {
+    int rc;
+    if (d->arith == CT2_ARITH_EXACT) {
+        rc = d->interp == CT2_LINEAR
+                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, vs, tu, s, ind, fv, vol, st)
+                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, vs, tu, s, ind, fv, vol, st);
+    } else if (d->arith == CT2_ARITH_FAST) {
+        rc = d->interp == CT2_LINEAR
+                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, vs, tu, s, ind, fv, vol, st)
+                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, vs, tu, s, ind, fv, vol, st);
+    } else {
+        set_error("ct2_project: unknown arith %d", d->arith);
+        return CT2_EINVAL;
+    }
+    if (rc) return rc;
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+// ---- tile autotuner ----
+// The first projection of a geometry times tile heights 2, 4 and 8 (every
+// launch writes the same output, so the winner's launch leaves the right
+// result) and caches the winner per geometry and device; the default of 4 is
+// used while the stream is being captured or with CT2_AUTOTUNE=0.
+struct TileKey {
+    int v[14];
+    bool operator<(const TileKey& o) const { return memcmp(v, o.v, sizeof v) < 0; }
+};
+static std::mutex g_tile_mu;
+static std::map<TileKey, int>& tile_cache() {
+    static std::map<TileKey, int>* m = new std::map<TileKey, int>();
+    return *m;
+}
+
+static int tuned_project(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
+                         float* s, const int8_t* ind, const float* fv, const void* vol,
+                         hipStream_t st) {
+    int dev = 0;
+    CT2_HIP(hipGetDevice(&dev));
+    TileKey key;
+    memset(&key, 0, sizeof key);
+    int i = 0;
+    for (int v : {d->nx, d->ny, d->nz, d->ns, d->nu, d->nv, vs.nviews, d->interp, d->arith,
+                  d->axis, d->vol_dtype, dev})
+        key.v[i++] = v;
+    memcpy(&key.v[i++], &d->dx, sizeof(float));
+    key.v[i++] = ind != nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = tile_cache().find(key);
+        if (it != tile_cache().end()) return launch_tu(d, g, dt, vs, it->second, s, ind, fv, vol, st);
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    CT2_HIP(hipStreamIsCapturing(st, &cs));
+    const char* env = getenv("CT2_AUTOTUNE");
+    if (cs != hipStreamCaptureStatusNone || (env && env[0] == '0'))
+        return launch_tu(d, g, dt, vs, 4, s, ind, fv, vol, st);
+    const int cand[3] = {4, 2, 8};
+    hipEvent_t ev[6];
+    for (int c = 0; c < 6; ++c) CT2_HIP(hipEventCreate(&ev[c]));
+    int rc = CT2_OK;
+    for (int c = 0; c < 3 && rc == CT2_OK; ++c) {
+        hipEventRecord(ev[2 * c], st);
+        rc = launch_tu(d, g, dt, vs, cand[c], s, ind, fv, vol, st);
+        hipEventRecord(ev[2 * c + 1], st);
+    }
+    int best = 4;
+    if (rc == CT2_OK) {
+        hipError_t he = hipEventSynchronize(ev[5]);
+        if (he != hipSuccess) rc = hip_fail(he, "tile autotune");
+        float tbest = 1e30f;
+        for (int c = 0; c < 3 && rc == CT2_OK; ++c) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]);
+            if (ms < tbest) { tbest = ms; best = cand[c]; }
+        }
+    }
+    for (int c = 0; c < 6; ++c) hipEventDestroy(ev[c]);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        tile_cache()[key] = best;
+    }
+    // the last candidate's launch already wrote the output
     return CT2_OK;
 }
 
@@ -339,22 +446,11 @@ int launch_project(const ct2_desc* d, void* sino, const int8_t* ind, const float
     ViewSet vs;
     if ((rc = view_set(d, "ct2_project", vs))) return rc;
     if (!sino || !fv || !vol) { set_error("ct2_project: null pointer"); return CT2_EINVAL; }
-    float* s = (float*)sino;
-    if (d->arith == CT2_ARITH_EXACT) {
-        rc = d->interp == CT2_LINEAR
-                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, vs, s, ind, fv, vol, st)
-                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, vs, s, ind, fv, vol, st);
-    } else if (d->arith == CT2_ARITH_FAST) {
-        rc = d->interp == CT2_LINEAR
-                 ? dispatch_dtype<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, vs, s, ind, fv, vol, st)
-                 : dispatch_dtype<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, vs, s, ind, fv, vol, st);
-    } else {
+    if (d->arith != CT2_ARITH_EXACT && d->arith != CT2_ARITH_FAST) {
         set_error("ct2_project: unknown arith %d", d->arith);
         return CT2_EINVAL;
     }
-    if (rc) return rc;
-    CT2_HIP(hipGetLastError());
-    return CT2_OK;
+    return tuned_project(d, g, dt, vs, (float*)sino, ind, fv, vol, st);
 }
 
 int launch_ray_project(const ct2_desc* d, float* sino, const int8_t* ind, const float* rv,
