@@ -183,6 +183,179 @@ __global__ __launch_bounds__(TV_THREADS) void k_dist_final(int nparts,
     if (threadIdx.x == 0) out[0] = sqrt(t);
 }
 
+// ---- p-variants (totalvar.f:4-472): x**p is powf(x, p), x**2 is x*x ----
+enum { PV_LP = 0, PV_TPV = 1, PV_TPDV = 2 };
+struct PvDims {
+    int nx, ny, nz;
+    float p, eps, dx, dy, dz;
+};
+
+// gradient at voxel (i,j,k): lp_grad3d (:150-215), total_p_var_grad3d
+// (:219-343) and total_p_dvar_grad3d (:347-472), terms in the reference's order
+template <int K>
+__device__ __forceinline__ float pv_grad_at(const float* __restrict__ s, const PvDims& d, int i,
+                                            int j, int k) {
+    const int sx = d.ny * d.nz, sy = d.nz;
+    const size_t q = (size_t)i * sx + (size_t)j * sy + k;
+    const float eps = d.eps, p = d.p, pm2 = d.p - 2.0f;
+    if (K == PV_LP) {
+        const float w14 = s[q];
+        return p * w14 * powf(sqrtf(w14 * w14 + eps), pm2);
+    }
+    const float dx = d.dx, dy = d.dy, dz = d.dz;
+    float g = 0.0f;
+    if (i >= 1 && j >= 1 && k >= 1) {
+        float w5 = s[q - 1], w11 = s[q - sy], w13 = s[q - sx], w14 = s[q];
+        float term1;
+        if (K == PV_TPV) {
+            term1 = (-w11 - w13 + 3.0f * w14 - w5) *
+                    powf(sqrtf((w11 - w14) * (w11 - w14) + (w13 - w14) * (w13 - w14) +
+                               (-w14 + w5) * (-w14 + w5) + eps), pm2);
+        } else {
+            float a = (w11 - w14) / dy, b = (w13 - w14) / dx, c = (-w14 + w5) / dz;
+            term1 = ((w14 - w11) / (dy * dy) + (w14 - w13) / (dx * dx) + (w14 - w5) / (dz * dz)) *
+                    powf(sqrtf(a * a + b * b + c * c + eps), pm2);
+        }
+        g = p * term1;
+    }
+    if (i < d.nx - 1 && j >= 1 && k >= 1) {
+        float w6 = s[q + sx - 1], w12 = s[q + sx - sy], w14 = s[q], w15 = s[q + sx];
+        float t;
+        if (K == PV_TPV) {
+            t = (w14 - w15) * powf(sqrtf((w12 - w15) * (w12 - w15) + (w14 - w15) * (w14 - w15) +
+                                         (-w15 + w6) * (-w15 + w6) + eps), pm2);
+        } else {
+            float a = (w12 - w15) / dy, b = (w14 - w15) / dx, c = (-w15 + w6) / dz;
+            t = ((w14 - w15) / (dx * dx)) * powf(sqrtf(a * a + b * b + c * c + eps), pm2);
+        }
+        g = g + p * t;
+    }
+    if (i >= 1 && j < d.ny - 1 && k >= 1) {
+        float w8 = s[q + sy - 1], w14 = s[q], w16 = s[q - sx + sy], w17 = s[q + sy];
+        float t;
+        if (K == PV_TPV) {
+            t = (w14 - w17) * powf(sqrtf((w14 - w17) * (w14 - w17) + (w16 - w17) * (w16 - w17) +
+                                         (-w17 + w8) * (-w17 + w8) + eps), pm2);
+        } else {
+            float a = (w14 - w17) / dy, b = (w16 - w17) / dx, c = (-w17 + w8) / dz;
+            t = ((w14 - w17) / (dy * dy)) * powf(sqrtf(a * a + b * b + c * c + eps), pm2);
+        }
+        g = g + p * t;
+    }
+    if (i >= 1 && j >= 1 && k < d.nz - 1) {
+        float w14 = s[q], w20 = s[q - sy + 1], w22 = s[q - sx + 1], w23 = s[q + 1];
+        float t;
+        if (K == PV_TPV) {
+            t = (w14 - w23) * powf(sqrtf((w14 - w23) * (w14 - w23) + (w20 - w23) * (w20 - w23) +
+                                         (w22 - w23) * (w22 - w23) + eps), pm2);
+        } else {
+            float a = (w14 - w23) / dz, b = (w20 - w23) / dy, c = (w22 - w23) / dx;
+            t = ((w14 - w23) / (dz * dz)) * powf(sqrtf(a * a + b * b + c * c + eps), pm2);
+        }
+        g = g + p * t;
+    }
+    return g;
+}
+
+template <int K>
+__global__ __launch_bounds__(TV_THREADS) void k_pv_grad(PvDims d, const float* __restrict__ s,
+                                                        float* __restrict__ g,
+                                                        double* __restrict__ partial) {
+    __shared__ double sh[TV_THREADS / 64];
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    double acc = 0.0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        int k = (int)(q % d.nz);
+        int j = (int)((q / d.nz) % d.ny);
+        int i = (int)(q / ((size_t)d.nz * d.ny));
+        float v = pv_grad_at<K>(s, d, i, j, k);
+        g[q] = v;
+        acc += (double)(v * v);
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// values: calc_lp (:65-87, every voxel), total_var3d_p (:91-117) and
+// total_dvar3d_p (:121-146) over voxels with all indices >= 1
+template <int K>
+__global__ __launch_bounds__(TV_THREADS) void k_pv_value(PvDims d, const float* __restrict__ s,
+                                                         double* __restrict__ partial) {
+    __shared__ double sh[TV_THREADS / 64];
+    const size_t n = (size_t)d.nx * d.ny * d.nz;
+    const int sx = d.ny * d.nz, sy = d.nz;
+    double acc = 0.0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        if (K == PV_LP) {
+            acc += (double)powf(sqrtf(d.eps + s[q] * s[q]), d.p);
+            continue;
+        }
+        int k = (int)(q % d.nz);
+        int j = (int)((q / d.nz) % d.ny);
+        int i = (int)(q / ((size_t)d.nz * d.ny));
+        if (i < 1 || j < 1 || k < 1) continue;
+        float c = s[q];
+        float ax = c - s[q - sx], ay = c - s[q - sy], az = c - s[q - 1];
+        if (K == PV_TPDV) { ax = ax / d.dx; ay = ay / d.dy; az = az / d.dz; }
+        acc += (double)powf(sqrtf(d.eps + ax * ax + ay * ay + az * az), d.p);
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// gmi (:4-31): gradient magnitude at voxels with all indices >= 1, 0 elsewhere
+__global__ void k_gmi(int nx, int ny, int nz, float eps, const float* __restrict__ s,
+                      float* __restrict__ out) {
+    const size_t n = (size_t)nx * ny * nz;
+    const int sx = ny * nz, sy = nz;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (size_t)gridDim.x * blockDim.x) {
+        int k = (int)(q % nz);
+        int j = (int)((q / nz) % ny);
+        int i = (int)(q / ((size_t)nz * ny));
+        float v = 0.0f;
+        if (i >= 1 && j >= 1 && k >= 1) {
+            float c = s[q];
+            float ax = c - s[q - sx], ay = c - s[q - sy], az = c - s[q - 1];
+            v = sqrtf(eps + ax * ax + ay * ay + az * az);
+        }
+        out[q] = v;
+    }
+}
+
+static int tv_blocks(int nx, int ny, int nz) {
+    size_t n = (size_t)nx * ny * nz;
+    int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
+    return nb > TV_BLOCKS ? TV_BLOCKS : nb;
+}
+
+template <int K>
+static int pv_grad(const PvDims& d, float* g, const float* s, int normalizeq, float* bobo,
+                   double* work, hipStream_t st) {
+    const int nb = tv_blocks(d.nx, d.ny, d.nz);
+    double* partial = work;
+    float* bscr = (float*)(partial + nb);
+    hipLaunchKernelGGL(k_pv_grad<K>, dim3(nb), dim3(TV_THREADS), 0, st, d, s, g, partial);
+    hipLaunchKernelGGL(k_tv_norm, dim3(1), dim3(TV_THREADS), 0, st, nb, partial, bobo, bscr);
+    if (normalizeq == 1)
+        hipLaunchKernelGGL(k_tv_normalize, dim3(nb), dim3(TV_THREADS), 0, st,
+                           (size_t)d.nx * d.ny * d.nz, g, bscr);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+template <int K>
+static int pv_value(const PvDims& d, const float* s, double shift, double* out, double* work,
+                    hipStream_t st) {
+    const int nb = tv_blocks(d.nx, d.ny, d.nz);
+    hipLaunchKernelGGL(k_pv_value<K>, dim3(nb), dim3(TV_THREADS), 0, st, d, s, work);
+    hipLaunchKernelGGL(k_tv_value_final, dim3(1), dim3(TV_THREADS), 0, st, nb, work, shift, out);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
 static int tv_check(int nx, int ny, int nz) {
     if (nx <= 0 || ny <= 0 || nz <= 0) {
         set_error("TV: volume shape (%d, %d, %d) must be positive", nx, ny, nz);
@@ -279,6 +452,75 @@ int ct2_total_var3d(float ambdala, const float* smat, int nx, int ny, int nz, fl
     if (rc) return rc;
     CT2_HIP(f);
     return CT2_OK;
+}
+
+int ct2_gmi(float* smatout, const float* smat, int nx, int ny, int nz, float eps,
+            ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!smatout || !smat) { set_error("ct2_gmi: null pointer"); return CT2_EINVAL; }
+    hipLaunchKernelGGL(k_gmi, dim3(tv_blocks(nx, ny, nz)), dim3(TV_THREADS), 0,
+                       (hipStream_t)stream, nx, ny, nz, eps, smat, smatout);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int ct2_calc_lp(float p, const float* smat, int nx, int ny, int nz, float eps, double* out,
+                double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!smat || !out || !work) { set_error("ct2_calc_lp: null pointer"); return CT2_EINVAL; }
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    return pv_value<PV_LP>(d, smat, 0.0, out, work, (hipStream_t)stream);
+}
+
+int ct2_total_var3d_p(float p, const float* smat, int nx, int ny, int nz, float eps,
+                      double* out, double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!smat || !out || !work) { set_error("ct2_total_var3d_p: null pointer"); return CT2_EINVAL; }
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    // shift = (nx-1.)*(ny-1.)*(nz-1.)*sqrt(eps)**p, Real*4 (totalvar.f:115)
+    float shift = (float)(nx - 1) * (float)(ny - 1) * (float)(nz - 1) * powf(sqrtf(eps), p);
+    return pv_value<PV_TPV>(d, smat, (double)shift, out, work, (hipStream_t)stream);
+}
+
+int ct2_total_dvar3d_p(float p, const float* smat, int nx, int ny, int nz, float dx, float dy,
+                       float dz, float eps, double* out, double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!smat || !out || !work) { set_error("ct2_total_dvar3d_p: null pointer"); return CT2_EINVAL; }
+    PvDims d{nx, ny, nz, p, eps, dx, dy, dz};
+    return pv_value<PV_TPDV>(d, smat, 0.0, out, work, (hipStream_t)stream);
+}
+
+int ct2_lp_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                  int normalizeq, float eps, float* bobo, double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!tvgmat || !smat || !work) { set_error("ct2_lp_grad3d: null pointer"); return CT2_EINVAL; }
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    return pv_grad<PV_LP>(d, tvgmat, smat, normalizeq, bobo, work, (hipStream_t)stream);
+}
+
+int ct2_total_p_var_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                           int normalizeq, float eps, float* bobo, double* work,
+                           ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!tvgmat || !smat || !work) { set_error("ct2_total_p_var_grad3d: null pointer"); return CT2_EINVAL; }
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    return pv_grad<PV_TPV>(d, tvgmat, smat, normalizeq, bobo, work, (hipStream_t)stream);
+}
+
+int ct2_total_p_dvar_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                            float dx, float dy, float dz, int normalizeq, float eps,
+                            float* bobo, double* work, ct2_stream_t stream) {
+    int rc = tv_check(nx, ny, nz);
+    if (rc) return rc;
+    if (!tvgmat || !smat || !work) { set_error("ct2_total_p_dvar_grad3d: null pointer"); return CT2_EINVAL; }
+    PvDims d{nx, ny, nz, p, eps, dx, dy, dz};
+    return pv_grad<PV_TPDV>(d, tvgmat, smat, normalizeq, bobo, work, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -65,6 +65,13 @@ SIGNATURES = {
     "ct2_tv_value": "fPiiifPPS",
     "ct2_tv_grad": "fPPiiiifPPS",
     "ct2_distance": "PPPzPPS",
+    "ct2_gmi": "PPiiifS",
+    "ct2_calc_lp": "fPiiifPPS",
+    "ct2_total_var3d_p": "fPiiifPPS",
+    "ct2_total_dvar3d_p": "fPiiiffffPPS",
+    "ct2_lp_grad3d": "fPPiiiifPPS",
+    "ct2_total_p_var_grad3d": "fPPiiiifPPS",
+    "ct2_total_p_dvar_grad3d": "fPPiiifffifPPS",
     "ct2_fdk_sino_weight3d": "ffiiiffffffPPS",
     "ct2_fdk_sino_filter3d": "fiiiiPPifS",
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",

@@ -293,6 +293,40 @@ class image3D:
         """emil_code/tomo3D.py:953-958."""
         return float(totalvar.total_var3d(lam1, self.mat, self.nx, self.ny, self.nz))
 
+    def calc_total_var_p(self, p=1.):
+        """emil_code/tomo3D.py:960-965."""
+        return float(totalvar.total_var3d_p(p, self.mat, self.nx, self.ny, self.nz))
+
+    def calc_lp(self, p=1.):
+        """emil_code/tomo3D.py:967-972."""
+        return float(totalvar.calc_lp(p, self.mat, self.nx, self.ny, self.nz))
+
+    def calc_total_dvar_p(self, p=1.):
+        """emil_code/tomo3D.py:975-980."""
+        return float(totalvar.total_dvar3d_p(p, self.mat, self.nx, self.ny, self.nz,
+                                             self.dx, self.dy, self.dz))
+
+    def lp_grad(self, p=1., normalizeq=1):
+        """emil_code/tomo3D.py:982-987: mat <- gradient, returns its norm."""
+        self.mat, bobo = totalvar.lp_grad3d(p, self.mat, self.nx, self.ny, self.nz, normalizeq)
+        return bobo
+
+    def tpv_grad(self, p=1., normalizeq=1):
+        """emil_code/tomo3D.py:990-995."""
+        self.mat, bobo = totalvar.total_p_var_grad3d(p, self.mat, self.nx, self.ny, self.nz,
+                                                     normalizeq)
+        return bobo
+
+    def tpdv_grad(self, p=1., normalizeq=1):
+        """emil_code/tomo3D.py:999-1004."""
+        self.mat, bobo = totalvar.total_p_dvar_grad3d(p, self.mat, self.nx, self.ny, self.nz,
+                                                      self.dx, self.dy, self.dz, normalizeq)
+        return bobo
+
+    def gmi(self):
+        """emil_code/tomo3D.py:1025-1028: mat <- gradient magnitude image."""
+        self.mat = totalvar.gmi(self.mat, self.nx, self.ny, self.nz)
+
     def tv_grad(self, lam1=0., normalizeq=1):
         """emil_code/tomo3D.py:1031-1036: mat <- TV gradient, returns its norm."""
         self.mat, bobo = totalvar.total_var_grad3d(lam1, self.mat, self.nx, self.ny, self.nz,

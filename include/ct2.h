@@ -267,6 +267,35 @@ int ct2_tv_grad(float ambdala, float* tvgmat, const float* smat, int nx, int ny,
 int ct2_distance(const float* a, const float* b, const int8_t* mask, size_t n, double* out,
                  double* work, ct2_stream_t stream);
 
+/* The p-variants of totalvar.f.  Values are written as float64 to the device
+ * pointer out (the reference's Real*4 function result, widened); gradients
+ * write tvgmat and the float32 norm bobo; work holds CT2_TV_WORK_DOUBLES
+ * doubles.  eps is the reference's 1e-9 unless changed. */
+/* totalvar.f:4 gmi: smatout = sqrt(eps + |backward differences|^2) at
+ * voxels with every index >= 1, 0 elsewhere */
+int ct2_gmi(float* smatout, const float* smat, int nx, int ny, int nz, float eps,
+            ct2_stream_t stream);
+/* totalvar.f:65 calc_lp */
+int ct2_calc_lp(float p, const float* smat, int nx, int ny, int nz, float eps, double* out,
+                double* work, ct2_stream_t stream);
+/* totalvar.f:91 total_var3d_p */
+int ct2_total_var3d_p(float p, const float* smat, int nx, int ny, int nz, float eps,
+                      double* out, double* work, ct2_stream_t stream);
+/* totalvar.f:121 total_dvar3d_p */
+int ct2_total_dvar3d_p(float p, const float* smat, int nx, int ny, int nz, float dx, float dy,
+                       float dz, float eps, double* out, double* work, ct2_stream_t stream);
+/* totalvar.f:150 lp_grad3d */
+int ct2_lp_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                  int normalizeq, float eps, float* bobo, double* work, ct2_stream_t stream);
+/* totalvar.f:219 total_p_var_grad3d */
+int ct2_total_p_var_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                           int normalizeq, float eps, float* bobo, double* work,
+                           ct2_stream_t stream);
+/* totalvar.f:347 total_p_dvar_grad3d */
+int ct2_total_p_dvar_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int nz,
+                            float dx, float dy, float dz, int normalizeq, float eps,
+                            float* bobo, double* work, ct2_stream_t stream);
+
 /* --------------------------------------------------- FDK.py replacements -- */
 /* FDK.py:10 fdk_sino_weight3D: out[p,j,k] = F/sqrt((F+D)^2 + vu[j,k]^2 + vv[j,k]^2)
  * * sinomat[p,j,k] with vu[j,k] = u00+(j+uoff)*du, vv[j,k] = v00+(k+voff)*dv. */

@@ -232,6 +232,51 @@ class _Restated:
             C.c_int(normalizeq), self.cr(eps))
         return g, float(bobo)
 
+    # ---- p-variants of totalvar.f ----
+    def gmi(self, smat, nx, ny, nz, eps=1e-9):
+        vol = self._a(smat).reshape(nx, ny, nz)
+        g = np.zeros((nx, ny, nz), self.dt)
+        self._fn("ora_gmi")(_ptr(g), _ptr(vol), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+                            self.cr(eps))
+        return g
+
+    def _pval(self, name, p, smat, nx, ny, nz, *extra, eps=1e-9):
+        vol = self._a(smat).reshape(nx, ny, nz)
+        return float(self._fn(name, C.c_double)(
+            self.cr(p), _ptr(vol), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+            *[self.cr(e) for e in extra], self.cr(eps)))
+
+    def calc_lp(self, p, smat, nx, ny, nz, eps=1e-9):
+        return self._pval("ora_calc_lp", p, smat, nx, ny, nz, eps=eps)
+
+    def total_var3d_p(self, p, smat, nx, ny, nz, eps=1e-9):
+        return self._pval("ora_total_var3d_p", p, smat, nx, ny, nz, eps=eps)
+
+    def total_dvar3d_p(self, p, smat, nx, ny, nz, dx, dy, dz, eps=1e-9):
+        return self._pval("ora_total_dvar3d_p", p, smat, nx, ny, nz, dx, dy, dz, eps=eps)
+
+    def _pgrad(self, name, p, smat, nx, ny, nz, normalizeq, *extra, eps=1e-9):
+        vol = self._a(smat).reshape(nx, ny, nz)
+        g = np.zeros((nx, ny, nz), self.dt)
+        bobo = self._fn(name, C.c_double)(
+            self.cr(p), _ptr(g), _ptr(vol), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+            *[self.cr(e) for e in extra], C.c_int(normalizeq), self.cr(eps))
+        return g, float(bobo)
+
+    def lp_grad3d(self, p, smat, nx, ny, nz, normalizeq, eps=1e-9):
+        return self._pgrad("ora_lp_grad3d", p, smat, nx, ny, nz, normalizeq, eps=eps)
+
+    def total_p_var_grad3d(self, p, smat, nx, ny, nz, normalizeq, eps=1e-9):
+        return self._pgrad("ora_total_p_var_grad3d", p, smat, nx, ny, nz, normalizeq, eps=eps)
+
+    def total_p_dvar_grad3d(self, p, smat, nx, ny, nz, dx, dy, dz, normalizeq, eps=1e-9):
+        vol = self._a(smat).reshape(nx, ny, nz)
+        g = np.zeros((nx, ny, nz), self.dt)
+        bobo = self._fn("ora_total_p_dvar_grad3d", C.c_double)(
+            self.cr(p), _ptr(g), _ptr(vol), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+            self.cr(dx), self.cr(dy), self.cr(dz), C.c_int(normalizeq), self.cr(eps))
+        return g, float(bobo)
+
     # ---- f2py-named facades (reference module/function names) ----
     def modules(self):
         o = self
@@ -264,6 +309,15 @@ class _Restated:
         tv = SimpleNamespace(
             total_var3d=lambda lam, s, nx, ny, nz: o.total_var3d(lam, s, nx, ny, nz),
             total_var_grad3d=lambda lam, s, nx, ny, nz, nq: o.total_var_grad3d(lam, s, nx, ny, nz, nq),
+            gmi=lambda s, nx, ny, nz: o.gmi(s, nx, ny, nz),
+            calc_lp=lambda p, s, nx, ny, nz: o.calc_lp(p, s, nx, ny, nz),
+            total_var3d_p=lambda p, s, nx, ny, nz: o.total_var3d_p(p, s, nx, ny, nz),
+            total_dvar3d_p=lambda p, s, nx, ny, nz, dx, dy, dz:
+                o.total_dvar3d_p(p, s, nx, ny, nz, dx, dy, dz),
+            lp_grad3d=lambda p, s, nx, ny, nz, nq: o.lp_grad3d(p, s, nx, ny, nz, nq),
+            total_p_var_grad3d=lambda p, s, nx, ny, nz, nq: o.total_p_var_grad3d(p, s, nx, ny, nz, nq),
+            total_p_dvar_grad3d=lambda p, s, nx, ny, nz, dx, dy, dz, nq:
+                o.total_p_dvar_grad3d(p, s, nx, ny, nz, dx, dy, dz, nq),
         )
         return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv)
 
@@ -431,6 +485,33 @@ class _Ref:
            self._i(normalizeq), C.byref(bobo))
         return np.ascontiguousarray(g), float(bobo.value)
 
+    def _gmi(self, smat, nx, ny, nz):
+        g = self._F(np.zeros((nx, ny, nz)))
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        fn = self.libs["totalvar"].gmi_
+        fn.restype = None
+        fn(_ptr(g), _ptr(s), self._i(nx), self._i(ny), self._i(nz))
+        return np.ascontiguousarray(g)
+
+    def _pfun(self, name, p, smat, nx, ny, nz, *extra):
+        fn = getattr(self.libs["totalvar"], name)
+        fn.restype = C.c_float
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        return float(fn(self._r(p), _ptr(s), self._i(nx), self._i(ny), self._i(nz),
+                        *[self._r(e) for e in extra]))
+
+    def _pgrad(self, name, p, smat, nx, ny, nz, *rest):
+        """rest = (normalizeq,) or (dx, dy, dz, normalizeq)"""
+        g = self._F(np.zeros((nx, ny, nz)))
+        s = self._F(np.asarray(smat).reshape(nx, ny, nz))
+        bobo = C.c_float(0.0)
+        fn = getattr(self.libs["totalvar"], name)
+        fn.restype = None
+        extra = [self._r(e) for e in rest[:-1]]
+        fn(self._r(p), _ptr(g), _ptr(s), self._i(nx), self._i(ny), self._i(nz), *extra,
+           self._i(rest[-1]), C.byref(bobo))
+        return np.ascontiguousarray(g), float(bobo.value)
+
     def modules(self):
         o = self
         vp = SimpleNamespace(**{
@@ -457,7 +538,14 @@ class _Ref:
             artstep_nearest=lambda *a: o._rart("artstep_nearest", *a),
             emstep_nearest=lambda *a: o._rem("emstep_nearest", *a),
         )
-        tv = SimpleNamespace(total_var3d=o._tv, total_var_grad3d=o._tvg)
+        tv = SimpleNamespace(
+            total_var3d=o._tv, total_var_grad3d=o._tvg, gmi=o._gmi,
+            calc_lp=lambda *a: o._pfun("calc_lp_", *a),
+            total_var3d_p=lambda *a: o._pfun("total_var3d_p_", *a),
+            total_dvar3d_p=lambda *a: o._pfun("total_dvar3d_p_", *a),
+            lp_grad3d=lambda *a: o._pgrad("lp_grad3d_", *a),
+            total_p_var_grad3d=lambda *a: o._pgrad("total_p_var_grad3d_", *a),
+            total_p_dvar_grad3d=lambda *a: o._pgrad("total_p_dvar_grad3d_", *a))
         return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv)
 
 

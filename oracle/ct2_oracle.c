@@ -24,12 +24,14 @@
 #define SQRT(x) sqrtf(x)
 #define FLOOR(x) floorf(x)
 #define ABS(x) fabsf(x)
+#define POW(x, y) powf(x, y)
 #include "ora_body.h"
 #undef REAL
 #undef FN
 #undef SQRT
 #undef FLOOR
 #undef ABS
+#undef POW
 #undef R
 
 #define REAL double
@@ -37,4 +39,5 @@
 #define SQRT(x) sqrt(x)
 #define FLOOR(x) floor(x)
 #define ABS(x) fabs(x)
+#define POW(x, y) pow(x, y)
 #include "ora_body.h"

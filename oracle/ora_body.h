@@ -651,3 +651,178 @@ EXPORT double FN(ora_total_var_grad3d)(REAL lam, REAL *g, const REAL *s, int nx,
         for (size_t q = 0; q < nvox; ++q) g[q] = g[q] / bobo;
     return (double)bobo;
 }
+
+/* ---- the p-variants of totalvar.f.  REAL x**p is pow(x, p); x**2 is x*x.
+ *      Values accumulate Real*4 terms in a Real*8 sum and return Real*4. ---- */
+#define V(i, j, k) s[((long)(i) * ny + (j)) * nz + (k)]
+#define G(i, j, k) g[((long)(i) * ny + (j)) * nz + (k)]
+
+/* gmi (totalvar.f:4-31): gradient magnitude at voxels with all indices >= 1
+ * (1-based >= 2); zero elsewhere (f2py zero-fills the intent(out) array) */
+EXPORT void FN(ora_gmi)(REAL *g, const REAL *s, int nx, int ny, int nz, REAL eps)
+{
+    memset(g, 0, sizeof(REAL) * (size_t)nx * ny * nz);
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL c = V(i, j, k);
+                REAL ax = c - V(i - 1, j, k), ay = c - V(i, j - 1, k), az = c - V(i, j, k - 1);
+                G(i, j, k) = SQRT(eps + ax * ax + ay * ay + az * az);
+            }
+}
+
+/* calc_lp (totalvar.f:65-87): sum of sqrt(eps + x^2)**p over every voxel */
+EXPORT double FN(ora_calc_lp)(REAL p, const REAL *s, int nx, int ny, int nz, REAL eps)
+{
+    double tot = 0.0;
+    size_t n = (size_t)nx * ny * nz;
+    for (size_t q = 0; q < n; ++q) tot = tot + (double)POW(SQRT(eps + s[q] * s[q]), p);
+    return (double)(REAL)tot;
+}
+
+/* total_var3d_p (totalvar.f:91-117): sum of sqrt(eps + |grad-|^2)**p minus
+ * (nx-1)(ny-1)(nz-1) sqrt(eps)**p */
+EXPORT double FN(ora_total_var3d_p)(REAL p, const REAL *s, int nx, int ny, int nz, REAL eps)
+{
+    double tot = 0.0;
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL c = V(i, j, k);
+                REAL ax = c - V(i - 1, j, k), ay = c - V(i, j - 1, k), az = c - V(i, j, k - 1);
+                tot = tot + (double)POW(SQRT(eps + ax * ax + ay * ay + az * az), p);
+            }
+    REAL shift = (REAL)(nx - 1) * (REAL)(ny - 1) * (REAL)(nz - 1) * POW(SQRT(eps), p);
+    return (double)(REAL)(tot - (double)shift);
+}
+
+/* total_dvar3d_p (totalvar.f:121-146): differences divided by the voxel
+ * size; no shift */
+EXPORT double FN(ora_total_dvar3d_p)(REAL p, const REAL *s, int nx, int ny, int nz, REAL dx,
+                                     REAL dy, REAL dz, REAL eps)
+{
+    double tot = 0.0;
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL c = V(i, j, k);
+                REAL ax = (c - V(i - 1, j, k)) / dx, ay = (c - V(i, j - 1, k)) / dy;
+                REAL az = (c - V(i, j, k - 1)) / dz;
+                tot = tot + (double)POW(SQRT(eps + ax * ax + ay * ay + az * az), p);
+            }
+    return (double)(REAL)tot;
+}
+
+/* gradient norm bobo (Real*8 sum of Real*4 squares, |bobo| < 1e-5 -> 1) and
+ * optional normalisation, as in every gradient routine of totalvar.f */
+static REAL FN(tv_norm)(REAL *g, size_t nvox, int normalizeq)
+{
+    double bb = 0.0;
+    for (size_t q = 0; q < nvox; ++q) bb = bb + (double)(g[q] * g[q]);
+    REAL bobo = (REAL)sqrt(bb);
+    if (ABS(bobo) < R(0.00001)) bobo = R(1.0);
+    if (normalizeq == 1)
+        for (size_t q = 0; q < nvox; ++q) g[q] = g[q] / bobo;
+    return bobo;
+}
+
+/* lp_grad3d (totalvar.f:150-215): p*x*sqrt(x^2+eps)**(p-2) */
+EXPORT double FN(ora_lp_grad3d)(REAL p, REAL *g, const REAL *s, int nx, int ny, int nz,
+                                int normalizeq, REAL eps)
+{
+    size_t n = (size_t)nx * ny * nz;
+    for (size_t q = 0; q < n; ++q) {
+        REAL w14 = s[q];
+        g[q] = p * w14 * POW(SQRT(w14 * w14 + eps), p - R(2));
+    }
+    return (double)FN(tv_norm)(g, n, normalizeq);
+}
+
+/* total_p_var_grad3d (totalvar.f:219-343): main term and the three
+ * one-ahead terms, each times sqrt(..)**(p-2), in the reference's order */
+EXPORT double FN(ora_total_p_var_grad3d)(REAL p, REAL *g, const REAL *s, int nx, int ny, int nz,
+                                         int normalizeq, REAL eps)
+{
+    size_t n = (size_t)nx * ny * nz;
+    REAL pm2 = p - R(2);
+    memset(g, 0, sizeof(REAL) * n);
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w5 = V(i, j, k - 1), w11 = V(i, j - 1, k), w13 = V(i - 1, j, k), w14 = V(i, j, k);
+                REAL term1 = (-w11 - w13 + R(3) * w14 - w5)
+                           * POW(SQRT((w11 - w14) * (w11 - w14) + (w13 - w14) * (w13 - w14)
+                                      + (-w14 + w5) * (-w14 + w5) + eps), pm2);
+                G(i, j, k) = p * term1;
+            }
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w6 = V(i + 1, j, k - 1), w12 = V(i + 1, j - 1, k), w14 = V(i, j, k), w15 = V(i + 1, j, k);
+                REAL t = (w14 - w15) * POW(SQRT((w12 - w15) * (w12 - w15) + (w14 - w15) * (w14 - w15)
+                                                + (-w15 + w6) * (-w15 + w6) + eps), pm2);
+                G(i, j, k) = G(i, j, k) + p * t;
+            }
+    for (int i = 1; i < nx; ++i)
+        for (int j = 0; j < ny - 1; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w8 = V(i, j + 1, k - 1), w14 = V(i, j, k), w16 = V(i - 1, j + 1, k), w17 = V(i, j + 1, k);
+                REAL t = (w14 - w17) * POW(SQRT((w14 - w17) * (w14 - w17) + (w16 - w17) * (w16 - w17)
+                                                + (-w17 + w8) * (-w17 + w8) + eps), pm2);
+                G(i, j, k) = G(i, j, k) + p * t;
+            }
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 0; k < nz - 1; ++k) {
+                REAL w14 = V(i, j, k), w20 = V(i, j - 1, k + 1), w22 = V(i - 1, j, k + 1), w23 = V(i, j, k + 1);
+                REAL t = (w14 - w23) * POW(SQRT((w14 - w23) * (w14 - w23) + (w20 - w23) * (w20 - w23)
+                                                + (w22 - w23) * (w22 - w23) + eps), pm2);
+                G(i, j, k) = G(i, j, k) + p * t;
+            }
+    return (double)FN(tv_norm)(g, n, normalizeq);
+}
+
+/* total_p_dvar_grad3d (totalvar.f:347-472): differences over voxel sizes */
+EXPORT double FN(ora_total_p_dvar_grad3d)(REAL p, REAL *g, const REAL *s, int nx, int ny, int nz,
+                                          REAL dx, REAL dy, REAL dz, int normalizeq, REAL eps)
+{
+    size_t n = (size_t)nx * ny * nz;
+    REAL pm2 = p - R(2);
+    memset(g, 0, sizeof(REAL) * n);
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w5 = V(i, j, k - 1), w11 = V(i, j - 1, k), w13 = V(i - 1, j, k), w14 = V(i, j, k);
+                REAL a = (w11 - w14) / dy, b = (w13 - w14) / dx, c = (-w14 + w5) / dz;
+                REAL term1 = ((w14 - w11) / (dy * dy) + (w14 - w13) / (dx * dx) + (w14 - w5) / (dz * dz))
+                           * POW(SQRT(a * a + b * b + c * c + eps), pm2);
+                G(i, j, k) = p * term1;
+            }
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w6 = V(i + 1, j, k - 1), w12 = V(i + 1, j - 1, k), w14 = V(i, j, k), w15 = V(i + 1, j, k);
+                REAL a = (w12 - w15) / dy, b = (w14 - w15) / dx, c = (-w15 + w6) / dz;
+                REAL t = ((w14 - w15) / (dx * dx)) * POW(SQRT(a * a + b * b + c * c + eps), pm2);
+                G(i, j, k) = G(i, j, k) + p * t;
+            }
+    for (int i = 1; i < nx; ++i)
+        for (int j = 0; j < ny - 1; ++j)
+            for (int k = 1; k < nz; ++k) {
+                REAL w8 = V(i, j + 1, k - 1), w14 = V(i, j, k), w16 = V(i - 1, j + 1, k), w17 = V(i, j + 1, k);
+                REAL a = (w14 - w17) / dy, b = (w16 - w17) / dx, c = (-w17 + w8) / dz;
+                REAL t = ((w14 - w17) / (dy * dy)) * POW(SQRT(a * a + b * b + c * c + eps), pm2);
+                G(i, j, k) = G(i, j, k) + p * t;
+            }
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            for (int k = 0; k < nz - 1; ++k) {
+                REAL w14 = V(i, j, k), w20 = V(i, j - 1, k + 1), w22 = V(i - 1, j, k + 1), w23 = V(i, j, k + 1);
+                REAL a = (w14 - w23) / dz, b = (w20 - w23) / dy, c = (w22 - w23) / dx;
+                REAL t = ((w14 - w23) / (dz * dz)) * POW(SQRT(a * a + b * b + c * c + eps), pm2);
+                G(i, j, k) = G(i, j, k) + p * t;
+            }
+    return (double)FN(tv_norm)(g, n, normalizeq);
+}
+#undef V
+#undef G

@@ -148,6 +148,31 @@ def test_tv_value(o32):
         assert abs(a - b) <= 1e-6 * abs(b)
 
 
+@pytest.mark.parametrize("p", [0.8, 1.0, 1.5])
+def test_tv_p_family(o32, p):
+    """totalvar.f p-variants on the device against the FP32 oracle (itself
+    bit-exact with the compiled Fortran).  powf differs between the device
+    and libm by an ulp at most: 1e-6 relative."""
+    from ct2 import totalvar
+    rng = np.random.default_rng(10)
+    n = (17, 19, 23)
+    vol = rng.random(n).astype(np.float32) - 0.3
+    o = o32.totalvar
+    assert np.array_equal(totalvar.gmi(vol, *n), o.gmi(vol, *n))
+    for name, extra in (("calc_lp", ()), ("total_var3d_p", ()),
+                        ("total_dvar3d_p", (0.8, 0.9, 1.3))):
+        a = getattr(totalvar, name)(p, vol, *n, *extra)
+        b = getattr(o, name)(p, vol, *n, *extra)
+        assert abs(a - b) <= 1e-6 * abs(b), name
+    for nq in (0, 1):
+        for name, extra in (("lp_grad3d", ()), ("total_p_var_grad3d", ()),
+                            ("total_p_dvar_grad3d", (0.8, 0.9, 1.3))):
+            g, bobo = getattr(totalvar, name)(p, vol, *n, *extra, nq)
+            go, bo = getattr(o, name)(p, vol, *n, *extra, nq)
+            assert abs(bobo - bo) <= 1e-6 * abs(bo), name
+            assert rel_l2(g, go) <= 1e-6, name
+
+
 def _fdk_setup(nu=64, nv=48, ns=36):
     du = dv = 0.4
     u00 = -nu * du / 2

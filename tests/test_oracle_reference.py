@@ -107,3 +107,24 @@ def test_totalvar(ref, o32):
             ga, ba = ref.totalvar.total_var_grad3d(lam, vol, 9, 11, 13, nq)
             gb, bb = o32.totalvar.total_var_grad3d(lam, vol, 9, 11, 13, nq)
             assert np.array_equal(ga, gb) and ba == bb
+
+
+@pytest.mark.parametrize("p", [0.8, 1.0, 1.5])
+def test_totalvar_p_family(ref, o32, p):
+    """gmi, calc_lp, total_var3d_p, total_dvar3d_p and the three p-gradients
+    (totalvar.f:4-472): the FP32 restatement equals the compiled Fortran."""
+    rng = np.random.default_rng(6)
+    n = (9, 11, 13)
+    vol = rng.random(n).astype(np.float32) - 0.3
+    a, b = ref.totalvar, o32.totalvar
+    assert np.array_equal(a.gmi(vol, *n), b.gmi(vol, *n))
+    assert a.calc_lp(p, vol, *n) == b.calc_lp(p, vol, *n)
+    assert a.total_var3d_p(p, vol, *n) == b.total_var3d_p(p, vol, *n)
+    assert a.total_dvar3d_p(p, vol, *n, 0.8, 0.9, 1.3) == b.total_dvar3d_p(p, vol, *n, 0.8, 0.9, 1.3)
+    for nq in (0, 1):
+        for name, extra in (("lp_grad3d", ()), ("total_p_var_grad3d", ()),
+                            ("total_p_dvar_grad3d", (0.8, 0.9, 1.3))):
+            ga, ba = getattr(a, name)(p, vol, *n, *extra, nq)
+            gb, bb = getattr(b, name)(p, vol, *n, *extra, nq)
+            assert np.array_equal(ga, gb) and ba == bb, name
+
