@@ -72,6 +72,9 @@ SIGNATURES = {
     "ct2_lp_grad3d": "fPPiiiifPPS",
     "ct2_total_p_var_grad3d": "fPPiiiifPPS",
     "ct2_total_p_dvar_grad3d": "fPPiiifffifPPS",
+    "ct2_phantom_project": "PiPiPPPiiiffffiS",
+    "ct2_phantom_project_rays": "PiPiPPPiiS",
+    "ct2_phantom_embed": "PiPiPiiiffffffiS",
     "ct2_fdk_sino_weight3d": "ffiiiffffffPPS",
     "ct2_fdk_sino_filter3d": "fiiiiPPifS",
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",

@@ -78,3 +78,29 @@ def emstep_nearest(sinomat, indsino, ray_vectors, nr, smat, dx, dy, dz, x0, y0, 
     call("ct2_ray_emstep_nearest", _dev.ptr(sino), _dev.ptr(ind), _dev.ptr(rv), nr,
          _dev.ptr(vol), dx, dy, dz, x0, y0, z0, nx, ny, nz, _dev.stream_handle())
     return _dev.out(vol, host)
+
+
+# ---- analytic phantoms (ray_proj3D.f:175-512) ----
+def gen_shape_proj(indsino, ray_vectors, nr, att, nsurf, *surf):
+    """ray_proj3D.f:175-431; called from emil_code/tomo3D.py:2688."""
+    from . import quadric
+    host = _dev.is_host(indsino, ray_vectors)
+    t = quadric.tables_from_f2py(att, nsurf, *surf)
+    ind = _dev.ind_dev(indsino, (nr,))
+    rv = _dev.to_dev(ray_vectors, torch.float32, (nr, 6))
+    sino = torch.empty((nr,), dtype=torch.float32, device=rv.device)
+    quadric.project_rays(t, sino, ind, rv, nr, 0)
+    return _dev.out(sino, host)
+
+
+def ellipsoid_proj(indsino, ray_vectors, nr, ax, ay, az, x0, y0, z0, alpha, beta, att):
+    """ray_proj3D.f:435-512; called from emil_code/tomo3D.py:2891."""
+    from . import quadric
+    host = _dev.is_host(indsino, ray_vectors)
+    t = quadric.Tables()
+    t.add_ellipsoid(att, ax, ay, az, x0, y0, z0, alpha, beta)
+    ind = _dev.ind_dev(indsino, (nr,))
+    rv = _dev.to_dev(ray_vectors, torch.float32, (nr, 6))
+    sino = torch.empty((nr,), dtype=torch.float32, device=rv.device)
+    quadric.project_rays(t, sino, ind, rv, nr, 0)
+    return _dev.out(sino, host)

@@ -18,6 +18,8 @@ import numpy as np
 from . import FDK as _FDK  # noqa: F401  (the FDK call surface lives next to the model)
 from . import configs3D, ray_proj3D, totalvar, view_proj3D
 from . import view_proj3D_tomo as vpt
+from .quadric import (conical, cylindrical, ellipsoid, ellipsoidal, gen_object,  # noqa: F401
+                      phantom3D, planar, shape3D, surface)
 
 pi = np.pi
 default_type = np.float32

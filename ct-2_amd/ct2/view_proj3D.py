@@ -121,3 +121,34 @@ def emstep_linear(*a):
 def emstep_nearest(*a):
     """view_proj3D.f:2834-3075."""
     return _em("ct2_emstep_nearest", *a)
+
+
+# ---- analytic phantoms (view_proj3D.f:11-603) ----
+def gen_shape_proj(indsino, frame_vectors, nu, nv, du, dv, u0, v0, att, nsurf, *surf):
+    """view_proj3D.f:175-455: projection of one general quadric object onto
+    one view; called from emil_code/tomo3D.py:2370, :2536.  surf are the
+    f2py arguments after nsurf (xu1 .. z0, nplane, nellipsoid, ax_ell, ay_ell,
+    az_ell, ncyl, ax_cyl, ay_cyl, ncone, ax_cone, ay_cone)."""
+    from . import quadric
+    host = _dev.is_host(indsino, frame_vectors)
+    t = quadric.tables_from_f2py(att, nsurf, *surf)
+    ind = _dev.ind_dev(indsino, (1, nu, nv))
+    fv = _dev.to_dev(frame_vectors, torch.float32, (1, 18))
+    sino = torch.empty((nu, nv), dtype=torch.float32, device=fv.device)
+    quadric.project_views(t, sino, ind, fv, 1, nu, nv, du, dv, u0, v0, 0)
+    return _dev.out(sino, host)
+
+
+def ellipsoid_proj(indsino, frame_vectors, nu, nv, du, dv, u0, v0, ax, ay, az, x0, y0, z0,
+                   alpha, beta, att):
+    """view_proj3D.f:483-603: chord lengths through a rotated ellipsoid on one
+    view; called from emil_code/tomo3D.py:2829, :2866."""
+    from . import quadric
+    host = _dev.is_host(indsino, frame_vectors)
+    t = quadric.Tables()
+    t.add_ellipsoid(att, ax, ay, az, x0, y0, z0, alpha, beta)
+    ind = _dev.ind_dev(indsino, (1, nu, nv))
+    fv = _dev.to_dev(frame_vectors, torch.float32, (1, 18))
+    sino = torch.empty((nu, nv), dtype=torch.float32, device=fv.device)
+    quadric.project_views(t, sino, ind, fv, 1, nu, nv, du, dv, u0, v0, 0)
+    return _dev.out(sino, host)

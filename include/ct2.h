@@ -318,6 +318,73 @@ int ct2_fdk_backproj_linear3d(int ns, int nu, int nv, float du, float dv, float 
                               float xoff, float yoff, float zoff, const float* sinomat,
                               float* recon, ct2_stream_t stream);
 
+/* ----------------------------------------------------- analytic phantoms -- */
+/* Exact line integrals through quadric objects and their voxel embeddings:
+ * the data generators of emil_code/view_proj3D.f:11-603 (gen_shape_proj,
+ * ellipsoid_proj), ray_proj3D.f:175-512 (the same over a ray list) and
+ * tomo3DfortranAux.f:212-582 (ellipsoid_embed, gen_obj_embed), which
+ * tomo3D.py:1507-2894 calls once per component and view.  One call here
+ * evaluates a whole phantom (every component, every view) and, per bin or
+ * voxel, adds the components in table order, the order of the reference's
+ * Python loop (tomo3D.py:1562-1567).  The shape and surface tables are HOST
+ * arrays; the library stages them in a stream-ordered device allocation. */
+enum { CT2_SHAPE_ELLIPSOID = 0, CT2_SHAPE_GENERAL = 1 };
+enum { CT2_SURF_PLANE = 0, CT2_SURF_ELLIPSOID = 1, CT2_SURF_CYLINDER = 2, CT2_SURF_CONE = 3 };
+
+/* One bounding surface of a general object (tomo3D.py:1704-1832): the
+ * surface frame's x, y and z unit vectors in space-fixed coordinates (the
+ * reference's xu*, yu*, zu*) and its centre.  Plane: the half-space z >= 0 of
+ * the frame.  Ellipsoid: semi-axes ax, ay, az.  Cylinder: elliptic, along
+ * the frame z, semi-axes ax, ay.  Cone: the z >= 0 nappe of
+ * (x/ax)^2 + (y/ay)^2 = z^2. */
+typedef struct ct2_surface {
+    int   kind;
+    float ax, ay, az;
+    float xu[3], yu[3], zu[3];
+    float c[3];
+} ct2_surface;
+
+/* One additive component.
+ * CT2_SHAPE_ELLIPSOID (tomo3D.py:2704-2894): semi-axes ax, ay, az, centre
+ *   x0, y0, z0, rotated by beta about z and then by alpha about the new y.
+ * CT2_SHAPE_GENERAL (tomo3D.py:1835-2701): the intersection of
+ *   surfaces[first .. first+nsurf); they are visited planes, ellipsoids,
+ *   cylinders, cones, each kind in table order, as gen_shape_proj does.
+ * att scales the chord length (projection) or is the value inside (embedding). */
+typedef struct ct2_shape {
+    int   kind;
+    float att;
+    int   first, nsurf;
+    float ax, ay, az;
+    float x0, y0, z0;
+    float alpha, beta;
+} ct2_shape;
+
+/* sinomat[ns][nu][nv] over the views of frame_vectors [ns][18]; rays with
+ * indsino != 1 contribute 0 (indsino NULL: every ray is used).  accumulate = 0: the sum of the components (one
+ * component gives the f2py routine's output, view_proj3D.f:175-455 or
+ * :483-603); accumulate = 1: added to sinomat, which is
+ * phantom3D.project_to / gen_object.project_to (tomo3D.py:1562, :2386). */
+int ct2_phantom_project(const ct2_shape* shapes, int nshape, const ct2_surface* surfaces,
+                        int nsurface, float* sinomat, const int8_t* indsino,
+                        const float* frame_vectors, int ns, int nu, int nv, float du, float dv,
+                        float u0, float v0, int accumulate, ct2_stream_t stream);
+/* The same over a ray list: sinomat[nr], ray_vectors [nr][6]
+ * (ray_proj3D.f:175-512; lino_project_to, tomo3D.py:1570, :2552, :2873). */
+int ct2_phantom_project_rays(const ct2_shape* shapes, int nshape, const ct2_surface* surfaces,
+                             int nsurface, float* sinomat, const int8_t* indsino,
+                             const float* ray_vectors, int nr, int accumulate,
+                             ct2_stream_t stream);
+/* smat[nx][ny][nz] at voxel centres x0 + (i+0.5)dx, ...: an ellipsoid gives
+ * att strictly inside and 0 outside (tomo3DfortranAux.f:212-267), a general
+ * object att times the product of its surfaces' indicator steps
+ * (tomo3DfortranAux.f:420-582); accumulate as above (embed_in,
+ * tomo3D.py:1555, :1967, :2732). */
+int ct2_phantom_embed(const ct2_shape* shapes, int nshape, const ct2_surface* surfaces,
+                      int nsurface, float* smat, int nx, int ny, int nz, float dx, float dy,
+                      float dz, float x0, float y0, float z0, int accumulate,
+                      ct2_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,9 +57,11 @@ def _ora():
     return _ORA
 
 
+_REF_MODULES = ("view_proj3D", "view_proj3D_tomo", "ray_proj3D", "totalvar", "tomo3DfortranAux")
+
+
 def ref_available() -> bool:
-    return all(os.path.exists(os.path.join(HERE, "_ref", f"lib{m}.so"))
-               for m in ("view_proj3D", "view_proj3D_tomo", "ray_proj3D", "totalvar"))
+    return all(os.path.exists(os.path.join(HERE, "_ref", f"lib{m}.so")) for m in _REF_MODULES)
 
 
 # --------------------------------------------------------------------------
@@ -277,6 +279,76 @@ class _Restated:
             self.cr(dx), self.cr(dy), self.cr(dz), C.c_int(normalizeq), self.cr(eps))
         return g, float(bobo)
 
+    # ---- analytic phantoms (view_proj3D.f:11-603, ray_proj3D.f:175-512,
+    #      tomo3DfortranAux.f:212-582) ----
+    def _gen_tables(self, nsurf, rest):
+        """rest = (xu1, xu2, xu3, yu1, yu2, yu3, zu1, zu2, zu3, x0, y0, z0,
+        nplane, nellipsoid, ax_el, ay_el, az_el, ncyl, ax_cyl, ay_cyl,
+        ncone, ax_cone, ay_cone) as the f2py wrappers take them."""
+        cols, (npl, nel, axe, aye, aze, ncy, axc, ayc, nco, axk, ayk) = rest[:12], rest[12:]
+        sf = np.zeros((max(int(nsurf), 1), 12), self.dt)
+        for c, v in enumerate(cols):
+            sf[:int(nsurf), c] = np.asarray(v, self.dt).reshape(-1)[:int(nsurf)]
+        arr = [self._a(np.asarray(a, self.dt).reshape(-1)) for a in (axe, aye, aze, axc, ayc, axk, ayk)]
+        keep = [sf, *arr]
+        args = [_ptr(sf), C.c_int(npl), C.c_int(nel), _ptr(arr[0]), _ptr(arr[1]), _ptr(arr[2]),
+                C.c_int(ncy), _ptr(arr[3]), _ptr(arr[4]), C.c_int(nco), _ptr(arr[5]), _ptr(arr[6])]
+        return keep, args
+
+    def gen_shape_proj(self, indsino, fv, nu, nv, du, dv, u0, v0, att, nsurf, *rest):
+        """One view (fv of 18) as the f2py wrapper, or ns views (fv [ns,18])."""
+        fv = np.ascontiguousarray(fv, np.float32).reshape(-1, 18)
+        ns = fv.shape[0]
+        out = np.zeros((ns, nu, nv), self.dt)
+        ind = self._ind(indsino, (ns, nu, nv))
+        keep, targs = self._gen_tables(nsurf, rest)
+        self._fn("ora_gen_shape_proj")(_ptr(out), _ptr(ind), _ptr(fv), C.c_int(ns), C.c_int(nu),
+                                       C.c_int(nv), *self._geo(du, dv, u0, v0, att), *targs)
+        return out[0] if np.ndim(indsino) == 2 or ns == 1 else out
+
+    def ellipsoid_proj(self, indsino, fv, nu, nv, du, dv, u0, v0, ax, ay, az, x0, y0, z0,
+                       alpha, beta, att):
+        fv = np.ascontiguousarray(fv, np.float32).reshape(-1, 18)
+        ns = fv.shape[0]
+        out = np.zeros((ns, nu, nv), self.dt)
+        ind = self._ind(indsino, (ns, nu, nv))
+        self._fn("ora_ellipsoid_proj")(_ptr(out), _ptr(ind), _ptr(fv), C.c_int(ns), C.c_int(nu),
+                                       C.c_int(nv), *self._geo(du, dv, u0, v0, ax, ay, az, x0, y0,
+                                                               z0, alpha, beta, att))
+        return out[0] if np.ndim(indsino) == 2 or ns == 1 else out
+
+    def ray_gen_shape_proj(self, indsino, rv, nr, att, nsurf, *rest):
+        out = np.zeros((nr,), self.dt)
+        ind = self._ind(indsino, (nr,))
+        rv = np.ascontiguousarray(rv, np.float32).reshape(nr, 6)
+        keep, targs = self._gen_tables(nsurf, rest)
+        self._fn("ora_ray_gen_shape_proj")(_ptr(out), _ptr(ind), _ptr(rv), C.c_int(nr),
+                                           self.cr(att), *targs)
+        return out
+
+    def ray_ellipsoid_proj(self, indsino, rv, nr, ax, ay, az, x0, y0, z0, alpha, beta, att):
+        out = np.zeros((nr,), self.dt)
+        ind = self._ind(indsino, (nr,))
+        rv = np.ascontiguousarray(rv, np.float32).reshape(nr, 6)
+        self._fn("ora_ray_ellipsoid_proj")(_ptr(out), _ptr(ind), _ptr(rv), C.c_int(nr),
+                                           *self._geo(ax, ay, az, x0, y0, z0, alpha, beta, att))
+        return out
+
+    def ellipsoid_embed(self, nx, ny, nz, dx, dy, dz, x0i, y0i, z0i, att, alpha, beta,
+                        x0, y0, z0, ax, ay, az):
+        out = np.zeros((nx, ny, nz), self.dt)
+        self._fn("ora_ellipsoid_embed")(_ptr(out), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+                                        *self._geo(dx, dy, dz, x0i, y0i, z0i, att, alpha, beta,
+                                                   x0, y0, z0, ax, ay, az))
+        return out
+
+    def gen_obj_embed(self, nx, ny, nz, dx, dy, dz, x0i, y0i, z0i, att, nsurf, *rest):
+        out = np.zeros((nx, ny, nz), self.dt)
+        keep, targs = self._gen_tables(nsurf, rest)
+        self._fn("ora_gen_obj_embed")(_ptr(out), C.c_int(nx), C.c_int(ny), C.c_int(nz),
+                                      *self._geo(dx, dy, dz, x0i, y0i, z0i, att), *targs)
+        return out
+
     # ---- f2py-named facades (reference module/function names) ----
     def modules(self):
         o = self
@@ -287,6 +359,8 @@ class _Restated:
             sinobackproject_nearest=lambda *a: o.sinobackproject(NEAREST, 0, *a),
             pd_backproj_linear=lambda *a: o.pd_backproj(LINEAR, *a),
             pd_backproj_nearest=lambda *a: o.pd_backproj(NEAREST, *a),
+            gen_shape_proj=o.gen_shape_proj,
+            ellipsoid_proj=o.ellipsoid_proj,
             artstep_linear=lambda *a: o.artstep(LINEAR, 0, *a),
             artstep_nearest=lambda *a: o.artstep(NEAREST, 0, *a),
             emstep_linear=lambda *a: o.emstep(LINEAR, 0, *a),
@@ -305,7 +379,10 @@ class _Restated:
             artstep_linear=lambda *a: o.ray_artstep(LINEAR, 0, *a),
             artstep_nearest=lambda *a: o.ray_artstep(NEAREST, 1, *a),
             emstep_nearest=lambda *a: o.ray_emstep(NEAREST, 1, *a),
+            gen_shape_proj=o.ray_gen_shape_proj,
+            ellipsoid_proj=o.ray_ellipsoid_proj,
         )
+        aux = SimpleNamespace(ellipsoid_embed=o.ellipsoid_embed, gen_obj_embed=o.gen_obj_embed)
         tv = SimpleNamespace(
             total_var3d=lambda lam, s, nx, ny, nz: o.total_var3d(lam, s, nx, ny, nz),
             total_var_grad3d=lambda lam, s, nx, ny, nz, nq: o.total_var_grad3d(lam, s, nx, ny, nz, nq),
@@ -319,7 +396,8 @@ class _Restated:
             total_p_dvar_grad3d=lambda p, s, nx, ny, nz, dx, dy, dz, nq:
                 o.total_p_dvar_grad3d(p, s, nx, ny, nz, dx, dy, dz, nq),
         )
-        return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv)
+        return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv,
+                               tomo3Dfort=aux)
 
 
 def restated(prec: str = "f32") -> SimpleNamespace:
@@ -337,8 +415,7 @@ def restated_raw(prec: str = "f32") -> _Restated:
 class _Ref:
     def __init__(self):
         d = os.path.join(HERE, "_ref")
-        self.libs = {m: _load(os.path.join(d, f"lib{m}.so"))
-                     for m in ("view_proj3D", "view_proj3D_tomo", "ray_proj3D", "totalvar")}
+        self.libs = {m: _load(os.path.join(d, f"lib{m}.so")) for m in _REF_MODULES}
 
     @staticmethod
     def _F(x, dt=np.float32):
@@ -512,6 +589,70 @@ class _Ref:
            self._i(rest[-1]), C.byref(bobo))
         return np.ascontiguousarray(g), float(bobo.value)
 
+    # ---- analytic phantoms ----
+    def _gen_refargs(self, nsurf, rest):
+        cols, (npl, nel, axe, aye, aze, ncy, axc, ayc, nco, axk, ayk) = rest[:12], rest[12:]
+        n = int(nsurf)
+        arrs = [np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1)[:n]) if n else
+                np.zeros(1, np.float32) for v in cols]
+        ax = [np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1)) for a in
+              (axe, aye, aze, axc, ayc, axk, ayk)]
+        i = self._i
+        args = [i(n), *[_ptr(a) for a in arrs], i(npl), i(nel), _ptr(ax[0]), _ptr(ax[1]), _ptr(ax[2]),
+                i(ncy), _ptr(ax[3]), _ptr(ax[4]), i(nco), _ptr(ax[5]), _ptr(ax[6])]
+        return arrs + ax, args
+
+    def _gen_proj(self, indsino, fv, nu, nv, du, dv, u0, v0, att, nsurf, *rest):
+        out = self._F(np.zeros((nu, nv)))
+        ind = self._ind(indsino, (nu, nv))
+        f = np.ascontiguousarray(fv, np.float32).reshape(18)
+        keep, targs = self._gen_refargs(nsurf, rest)
+        i, r = self._i, self._r
+        self._call("view_proj3D", "gen_shape_proj", _ptr(out), _ptr(ind), _ptr(f), i(nu), i(nv),
+                   r(du), r(dv), r(u0), r(v0), r(att), *targs)
+        return np.ascontiguousarray(out)
+
+    def _ell_proj(self, indsino, fv, nu, nv, du, dv, u0, v0, *p9):
+        out = self._F(np.zeros((nu, nv)))
+        ind = self._ind(indsino, (nu, nv))
+        f = np.ascontiguousarray(fv, np.float32).reshape(18)
+        i, r = self._i, self._r
+        self._call("view_proj3D", "ellipsoid_proj", _ptr(out), _ptr(ind), _ptr(f), i(nu), i(nv),
+                   r(du), r(dv), r(u0), r(v0), *[r(x) for x in p9])
+        return np.ascontiguousarray(out)
+
+    def _rgen_proj(self, indsino, rv, nr, att, nsurf, *rest):
+        out = np.zeros((nr,), np.float32)
+        ind = self._ind(indsino, (nr,))
+        f = self._F(np.asarray(rv, np.float32).reshape(nr, 6))
+        keep, targs = self._gen_refargs(nsurf, rest)
+        self._call("ray_proj3D", "gen_shape_proj", _ptr(out), _ptr(ind), _ptr(f), self._i(nr),
+                   self._r(att), *targs)
+        return out
+
+    def _rell_proj(self, indsino, rv, nr, *p9):
+        out = np.zeros((nr,), np.float32)
+        ind = self._ind(indsino, (nr,))
+        f = self._F(np.asarray(rv, np.float32).reshape(nr, 6))
+        self._call("ray_proj3D", "ellipsoid_proj", _ptr(out), _ptr(ind), _ptr(f), self._i(nr),
+                   *[self._r(x) for x in p9])
+        return out
+
+    def _ell_embed(self, nx, ny, nz, *p15):
+        out = self._F(np.zeros((nx, ny, nz)))
+        i, r = self._i, self._r
+        self._call("tomo3DfortranAux", "ellipsoid_embed", _ptr(out), i(nx), i(ny), i(nz),
+                   *[r(x) for x in p15])
+        return np.ascontiguousarray(out)
+
+    def _gen_embed(self, nx, ny, nz, dx, dy, dz, x0i, y0i, z0i, att, nsurf, *rest):
+        out = self._F(np.zeros((nx, ny, nz)))
+        keep, targs = self._gen_refargs(nsurf, rest)
+        i, r = self._i, self._r
+        self._call("tomo3DfortranAux", "gen_obj_embed", _ptr(out), i(nx), i(ny), i(nz),
+                   r(dx), r(dy), r(dz), r(x0i), r(y0i), r(z0i), r(att), *targs)
+        return np.ascontiguousarray(out)
+
     def modules(self):
         o = self
         vp = SimpleNamespace(**{
@@ -524,6 +665,8 @@ class _Ref:
             setattr(vp, n, (lambda n: (lambda *a: o._art("view_proj3D", n, *a)))(n))
         for n in ("emstep_linear", "emstep_nearest"):
             setattr(vp, n, (lambda n: (lambda *a: o._em("view_proj3D", n, *a)))(n))
+        vp.gen_shape_proj = o._gen_proj
+        vp.ellipsoid_proj = o._ell_proj
         vpt = SimpleNamespace(
             sinoprojz=lambda *a: o._fwd("view_proj3D_tomo", "sinoprojz", *a),
             artstepz=lambda *a: o._art("view_proj3D_tomo", "artstepz", *a),
@@ -537,6 +680,8 @@ class _Ref:
             artstep_linear=lambda *a: o._rart("artstep_linear", *a),
             artstep_nearest=lambda *a: o._rart("artstep_nearest", *a),
             emstep_nearest=lambda *a: o._rem("emstep_nearest", *a),
+            gen_shape_proj=o._rgen_proj,
+            ellipsoid_proj=o._rell_proj,
         )
         tv = SimpleNamespace(
             total_var3d=o._tv, total_var_grad3d=o._tvg, gmi=o._gmi,
@@ -546,7 +691,9 @@ class _Ref:
             lp_grad3d=lambda *a: o._pgrad("lp_grad3d_", *a),
             total_p_var_grad3d=lambda *a: o._pgrad("total_p_var_grad3d_", *a),
             total_p_dvar_grad3d=lambda *a: o._pgrad("total_p_dvar_grad3d_", *a))
-        return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv)
+        aux = SimpleNamespace(ellipsoid_embed=o._ell_embed, gen_obj_embed=o._gen_embed)
+        return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv,
+                               tomo3Dfort=aux)
 
 
 def reference() -> SimpleNamespace:

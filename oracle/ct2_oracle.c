@@ -25,6 +25,8 @@
 #define FLOOR(x) floorf(x)
 #define ABS(x) fabsf(x)
 #define POW(x, y) powf(x, y)
+#define COS(x) cosf(x)
+#define SIN(x) sinf(x)
 #include "ora_body.h"
 #undef REAL
 #undef FN
@@ -32,6 +34,8 @@
 #undef FLOOR
 #undef ABS
 #undef POW
+#undef COS
+#undef SIN
 #undef R
 
 #define REAL double
@@ -40,4 +44,6 @@
 #define FLOOR(x) floor(x)
 #define ABS(x) fabs(x)
 #define POW(x, y) pow(x, y)
+#define COS(x) cos(x)
+#define SIN(x) sin(x)
 #include "ora_body.h"

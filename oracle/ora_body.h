@@ -826,3 +826,314 @@ EXPORT double FN(ora_total_p_dvar_grad3d)(REAL p, REAL *g, const REAL *s, int nx
 }
 #undef V
 #undef G
+
+/* ---- analytic phantoms: line integrals through quadric objects
+ *      (view_proj3D.f:11-603; ray_proj3D.f:11-512 holds identical root_*
+ *      routines) and their voxel embeddings (tomo3DfortranAux.f:212-582).
+ *      Every expression keeps the Fortran's left-to-right association;
+ *      x**2 is x*x. ---- */
+#define ALARGE R(100000000.0)
+
+/* root_plane (view_proj3D.f:11-39): the half-space zs<=0 side of the plane
+ * z=0 in the surface frame, as an interval of the ray parameter t */
+static void FN(root_plane)(REAL *t, REAL zs, REAL zb)
+{
+    if (ABS(zs - zb) < R(0.00000001)) {
+        if (zs >= R(0)) { t[0] = -ALARGE; t[1] = ALARGE; }
+        else            { t[0] = R(1.0);  t[1] = R(-1.0); }
+    } else {
+        REAL root1 = zs / (zs - zb);
+        if (zs < zb) { t[0] = root1; t[1] = ALARGE; }
+        else         { t[0] = -ALARGE; t[1] = root1; }
+    }
+}
+
+/* root_ellipsoid (view_proj3D.f:42-72) */
+static void FN(root_ellipsoid)(REAL *t, REAL xs, REAL ys, REAL zs, REAL xb, REAL yb, REAL zb,
+                               REAL ax, REAL ay, REAL az)
+{
+    REAL ax2 = ax * ax, ay2 = ay * ay, az2 = az * az;
+    REAL dx = xb - xs, dy = yb - ys, dz = zb - zs;
+    REAL c1 = xs * yb - xb * ys, c2 = xs * zb - xb * zs, c3 = ys * zb - yb * zs;
+    REAL den = ax2 * az2 * (dy * dy) + ay2 * (az2 * (dx * dx) + ax2 * (dz * dz));
+    REAL s = -(az2 * (c1 * c1)) + ay2 * (az2 * (dx * dx) + ax2 * (dz * dz) - c2 * c2)
+           + ax2 * (az2 * (dy * dy) - c3 * c3);
+    REAL dist2 = (R(4) * ax2 * ay2 * az2 * s) / (den * den);
+    REAL avet = (ax2 * az2 * ys * (ys - yb) + ay2 * (az2 * xs * (xs - xb) + ax2 * zs * (zs - zb)))
+              / den;
+    if (dist2 < R(0)) { t[0] = R(1.0); t[1] = R(-1.0); }
+    else {
+        t[0] = avet - R(0.5) * SQRT(dist2);
+        t[1] = avet + R(0.5) * SQRT(dist2);
+    }
+}
+
+/* root_cylinder (view_proj3D.f:75-97): elliptic cylinder along the frame z */
+static void FN(root_cylinder)(REAL *t, REAL xs, REAL ys, REAL xb, REAL yb, REAL ax, REAL ay)
+{
+    REAL ax2 = ax * ax, ay2 = ay * ay;
+    REAL dx = xb - xs, dy = yb - ys;
+    REAL c1 = xs * yb - xb * ys;
+    REAL den = ay2 * (dx * dx) + ax2 * (dy * dy);
+    REAL dist2 = (R(4) * ax2 * ay2 * (ay2 * (dx * dx) + ax2 * (dy * dy) - c1 * c1)) / (den * den);
+    REAL avet = (ay2 * xs * (xs - xb) + ax2 * ys * (ys - yb)) / den;
+    if (dist2 < R(0)) { t[0] = R(1.0); t[1] = R(-1.0); }
+    else {
+        t[0] = avet - R(0.5) * SQRT(dist2);
+        t[1] = avet + R(0.5) * SQRT(dist2);
+    }
+}
+
+/* root_cone (view_proj3D.f:100-171): the z>0 nappe of the elliptic cone
+ * (x/ax)^2 + (y/ay)^2 = z^2.  The "one root on the cone" cases run as two
+ * sequential tests, as in the Fortran; t keeps its previous contents when
+ * neither applies. */
+static void FN(root_cone)(REAL *t, REAL xs, REAL ys, REAL zs, REAL xb, REAL yb, REAL zb,
+                          REAL ax, REAL ay)
+{
+    REAL ax2 = ax * ax, ay2 = ay * ay;
+    REAL dy = yb - ys, dz = zb - zs;
+    REAL c3 = ys * zb - yb * zs;
+    REAL s = xs * xs * (-(yb * yb) + ay2 * (zb * zb)) + ax2 * (c3 * c3)
+           + R(2) * xb * xs * (yb * ys - ay2 * zb * zs) + xb * xb * (-(ys * ys) + ay2 * (zs * zs));
+    REAL den = ax2 * (dy * dy) + ay2 * (xb * xb - R(2) * xb * xs + xs * xs - ax2 * (dz * dz));
+    REAL dist2 = (R(4) * ax2 * ay2 * s) / (den * den);
+    REAL avet = (ax2 * ys * (ys - yb) + ay2 * (-(xb * xs) + xs * xs + ax2 * (zb - zs) * zs)) / den;
+    REAL sd2 = SQRT(dist2) * R(0.5);
+    if (dist2 < R(0)) { t[0] = R(1.0); t[1] = R(-1.0); return; }
+    REAL tr1 = avet - sd2, tr2 = avet + sd2;
+    REAL zr1 = zs * (R(1.) - tr1) + zb * tr1;
+    REAL zr2 = zs * (R(1.) - tr2) + zb * tr2;
+    if ((zr1 <= R(0)) && (zr2 <= R(0))) { t[0] = R(1.0); t[1] = R(-1.0); return; }
+    if ((zr1 > R(0)) && (zr2 > R(0))) { t[0] = tr1; t[1] = tr2; return; }
+    if (zr1 >= R(0)) {
+        if (zs < zb) { t[0] = tr1; t[1] = ALARGE; } else { t[0] = -ALARGE; t[1] = tr1; }
+    }
+    if (zr2 >= R(0)) {
+        if (zs < zb) { t[0] = tr2; t[1] = ALARGE; } else { t[0] = -ALARGE; t[1] = tr2; }
+    }
+}
+
+/* Fortran MAX/MIN of two reals as the compiler lowers them (compare, then
+ * select the second operand when the compare fails) */
+static inline REAL FN(fmax2)(REAL a, REAL b) { return (a > b) ? a : b; }
+static inline REAL FN(fmin2)(REAL a, REAL b) { return (a < b) ? a : b; }
+
+/* One object of a general quadric phantom: sf[nsurf][12] holds, per
+ * surface, the frame rows xu1..xu3, yu1..yu3, zu1..zu3 and the centre
+ * x0,y0,z0, ordered planes, ellipsoids, cylinders, cones; the per-kind
+ * counts and semi-axis arrays carry the reference's leading dummy entry
+ * (its loops start at 2). */
+typedef struct {
+    const REAL *sf;
+    int nplane, nell, ncyl, ncone;
+    const REAL *axe, *aye, *aze, *axc, *ayc, *axk, *ayk;
+} FN(genobj_t);
+
+static void FN(to_frame)(const REAL *u, REAL x, REAL y, REAL z, REAL *o)
+{
+    REAL xt = x - u[9], yt = y - u[10], zt = z - u[11];
+    o[0] = u[0] * xt + u[1] * yt + u[2] * zt;
+    o[1] = u[3] * xt + u[4] * yt + u[5] * zt;
+    o[2] = u[6] * xt + u[7] * yt + u[8] * zt;
+}
+
+/* tlen of gen_shape_proj (view_proj3D.f:306-455): the parameter length of
+ * the chord inside every surface, 0 as soon as the interval empties */
+static REAL FN(gen_tlen)(const FN(genobj_t) *ob, const REAL S[3], const REAL B[3], REAL *troot)
+{
+    REAL f1 = -ALARGE, f2 = ALARGE;
+    int counts[4] = {ob->nplane, ob->nell, ob->ncyl, ob->ncone};
+    int is = 0;
+    for (int kind = 0; kind < 4; ++kind)
+        for (int q = 1; q < counts[kind]; ++q, ++is) {
+            const REAL *u = ob->sf + 12 * is;
+            REAL s[3], b[3];
+            FN(to_frame)(u, S[0], S[1], S[2], s);
+            FN(to_frame)(u, B[0], B[1], B[2], b);
+            switch (kind) {
+            case 0: FN(root_plane)(troot, s[2], b[2]); break;
+            case 1: FN(root_ellipsoid)(troot, s[0], s[1], s[2], b[0], b[1], b[2],
+                                       ob->axe[q], ob->aye[q], ob->aze[q]); break;
+            case 2: FN(root_cylinder)(troot, s[0], s[1], b[0], b[1], ob->axc[q], ob->ayc[q]); break;
+            default: FN(root_cone)(troot, s[0], s[1], s[2], b[0], b[1], b[2],
+                                   ob->axk[q], ob->ayk[q]); break;
+            }
+            f1 = FN(fmax2)(f1, troot[0]);
+            f2 = FN(fmin2)(f2, troot[1]);
+            if (f1 >= f2) return R(0.);
+        }
+    return f2 - f1;
+}
+
+static REAL FN(gen_value)(const FN(genobj_t) *ob, const REAL S[3], const REAL B[3], REAL att,
+                          REAL *troot)
+{
+    REAL sbdist = SQRT((S[0] - B[0]) * (S[0] - B[0]) + (S[1] - B[1]) * (S[1] - B[1])
+                       + (S[2] - B[2]) * (S[2] - B[2]));
+    REAL tlen = FN(gen_tlen)(ob, S, B, troot);
+    return sbdist * tlen * att;
+}
+
+/* ellipsoid_proj body (view_proj3D.f:557-597): chord length through the
+ * ellipsoid rotated by beta about z, then alpha about the new y */
+static REAL FN(ell_value)(const REAL S[3], const REAL B[3], REAL ax, REAL ay, REAL az,
+                          REAL x0, REAL y0, REAL z0, REAL alpha, REAL beta, REAL att)
+{
+    REAL ca = COS(alpha), sa = SIN(alpha), cb = COS(beta), sb = SIN(beta);
+    REAL xspp = (S[0] - x0) * cb - (y0 - S[1]) * sb;
+    REAL yspp = (S[1] - y0) * cb + (x0 - S[0]) * sb;
+    REAL zspp = S[2] - z0;
+    REAL xsp = xspp * ca + zspp * sa, ysp = yspp, zsp = -(xspp * sa) + zspp * ca;
+    REAL xbpp = (B[0] - x0) * cb - (y0 - B[1]) * sb;
+    REAL ybpp = (B[1] - y0) * cb + (x0 - B[0]) * sb;
+    REAL zbpp = B[2] - z0;
+    REAL xbp = xbpp * ca + zbpp * sa, ybp = ybpp, zbp = -(xbpp * sa) + zbpp * ca;
+    REAL ax2 = ax * ax, ay2 = ay * ay, az2 = az * az;
+    REAL dx = xbp - xsp, dy = ybp - ysp, dz = zbp - zsp;
+    REAL c1 = xsp * ybp - xbp * ysp, c2 = xsp * zbp - xbp * zsp, c3 = ysp * zbp - ybp * zsp;
+    REAL s = -(az2 * (c1 * c1)) + ay2 * (az2 * (dx * dx) + ax2 * (dz * dz) - c2 * c2)
+           + ax2 * (az2 * (dy * dy) - c3 * c3);
+    REAL den = ax2 * az2 * (dy * dy) + ay2 * (az2 * (dx * dx) + ax2 * (dz * dz));
+    REAL pl2 = (R(4) * ax2 * ay2 * az2 * (dx * dx + dy * dy + dz * dz) * s) / (den * den);
+    if (pl2 < R(0.)) pl2 = R(0.);
+    return SQRT(pl2) * att;
+}
+
+/* gen_shape_proj over ns views (view_proj3D.f:175-455, called once per view
+ * from tomo3D.py:2531-2550); unused bins are zero as f2py leaves them */
+EXPORT void FN(ora_gen_shape_proj)(REAL *sino, const int8_t *ind, const float *fv, int ns, int nu,
+                                   int nv, REAL du, REAL dv, REAL u0, REAL v0, REAL att,
+                                   const REAL *sf, int nplane, int nell, const REAL *axe,
+                                   const REAL *aye, const REAL *aze, int ncyl, const REAL *axc,
+                                   const REAL *ayc, int ncone, const REAL *axk, const REAL *ayk)
+{
+    FN(genobj_t) ob = {sf, nplane, nell, ncyl, ncone, axe, aye, aze, axc, ayc, axk, ayk};
+    REAL troot[2] = {R(0), R(0)};
+    for (int p = 0; p < ns; ++p)
+        for (int j = 0; j < nu; ++j)
+            for (int k = 0; k < nv; ++k) {
+                long q = ((long)p * nu + j) * nv + k;
+                sino[q] = R(0);
+                if (ind[q] != 1) continue;
+                FN(ray_t) ry;
+                FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+                sino[q] = FN(gen_value)(&ob, ry.s, ry.b, att, troot);
+            }
+}
+
+/* ellipsoid_proj over ns views (view_proj3D.f:483-603) */
+EXPORT void FN(ora_ellipsoid_proj)(REAL *sino, const int8_t *ind, const float *fv, int ns, int nu,
+                                   int nv, REAL du, REAL dv, REAL u0, REAL v0, REAL ax, REAL ay,
+                                   REAL az, REAL x0, REAL y0, REAL z0, REAL alpha, REAL beta,
+                                   REAL att)
+{
+    for (int p = 0; p < ns; ++p)
+        for (int j = 0; j < nu; ++j)
+            for (int k = 0; k < nv; ++k) {
+                long q = ((long)p * nu + j) * nv + k;
+                sino[q] = R(0);
+                if (ind[q] != 1) continue;
+                FN(ray_t) ry;
+                FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+                sino[q] = FN(ell_value)(ry.s, ry.b, ax, ay, az, x0, y0, z0, alpha, beta, att);
+            }
+}
+
+/* ray-list forms (ray_proj3D.f:175-431, :435-512) */
+EXPORT void FN(ora_ray_gen_shape_proj)(REAL *sino, const int8_t *ind, const float *rv, int nr,
+                                       REAL att, const REAL *sf, int nplane, int nell,
+                                       const REAL *axe, const REAL *aye, const REAL *aze,
+                                       int ncyl, const REAL *axc, const REAL *ayc, int ncone,
+                                       const REAL *axk, const REAL *ayk)
+{
+    FN(genobj_t) ob = {sf, nplane, nell, ncyl, ncone, axe, aye, aze, axc, ayc, axk, ayk};
+    REAL troot[2] = {R(0), R(0)};
+    for (int r = 0; r < nr; ++r) {
+        sino[r] = R(0);
+        if (ind[r] != 1) continue;
+        FN(ray_t) ry;
+        FN(list_ray)(rv, r, &ry);
+        sino[r] = FN(gen_value)(&ob, ry.s, ry.b, att, troot);
+    }
+}
+
+EXPORT void FN(ora_ray_ellipsoid_proj)(REAL *sino, const int8_t *ind, const float *rv, int nr,
+                                       REAL ax, REAL ay, REAL az, REAL x0, REAL y0, REAL z0,
+                                       REAL alpha, REAL beta, REAL att)
+{
+    for (int r = 0; r < nr; ++r) {
+        sino[r] = R(0);
+        if (ind[r] != 1) continue;
+        FN(ray_t) ry;
+        FN(list_ray)(rv, r, &ry);
+        sino[r] = FN(ell_value)(ry.s, ry.b, ax, ay, az, x0, y0, z0, alpha, beta, att);
+    }
+}
+
+/* ellipsoid_embed (tomo3DfortranAux.f:212-267): att where the voxel centre
+ * is strictly inside, zero elsewhere */
+EXPORT void FN(ora_ellipsoid_embed)(REAL *s, int nx, int ny, int nz, REAL dx, REAL dy, REAL dz,
+                                    REAL x0i, REAL y0i, REAL z0i, REAL att, REAL alpha, REAL beta,
+                                    REAL x0, REAL y0, REAL z0, REAL ax, REAL ay, REAL az)
+{
+    REAL ca = COS(alpha), sa = SIN(alpha), cb = COS(beta), sb = SIN(beta);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                REAL x = x0i + ((REAL)i + R(0.5)) * dx;
+                REAL y = y0i + ((REAL)j + R(0.5)) * dy;
+                REAL z = z0i + ((REAL)k + R(0.5)) * dz;
+                REAL rx = x - x0, ry = y - y0, rz = z - z0;
+                REAL r1x = cb * rx + sb * ry, r1y = -(sb * rx) + cb * ry, r1z = rz;
+                REAL r2x = ca * r1x + sa * r1z, r2y = r1y, r2z = -(sa * r1x) + ca * r1z;
+                REAL qx = r2x / ax, qy = r2y / ay, qz = r2z / az;
+                REAL lhs = qx * qx + qy * qy + qz * qz;
+                s[((long)i * ny + j) * nz + k] = (lhs < R(1.)) ? att : R(0);
+            }
+}
+
+/* gen_obj_embed (tomo3DfortranAux.f:420-582): att times the product of
+ * the surfaces' step functions at the voxel centre */
+EXPORT void FN(ora_gen_obj_embed)(REAL *s, int nx, int ny, int nz, REAL dx, REAL dy, REAL dz,
+                                  REAL x0i, REAL y0i, REAL z0i, REAL att, const REAL *sf,
+                                  int nplane, int nell, const REAL *axe, const REAL *aye,
+                                  const REAL *aze, int ncyl, const REAL *axc, const REAL *ayc,
+                                  int ncone, const REAL *axk, const REAL *ayk)
+{
+    int counts[4] = {nplane, nell, ncyl, ncone};
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                REAL x = x0i + ((REAL)i + R(0.5)) * dx;
+                REAL y = y0i + ((REAL)j + R(0.5)) * dy;
+                REAL z = z0i + ((REAL)k + R(0.5)) * dz;
+                REAL si = R(1.0);
+                int is = 0;
+                for (int kind = 0; kind < 4; ++kind)
+                    for (int q = 1; q < counts[kind]; ++q, ++is) {
+                        REAL f[3];
+                        FN(to_frame)(sf + 12 * is, x, y, z, f);
+                        REAL e;
+                        switch (kind) {
+                        case 0: si = si * (f[2] >= R(0) ? R(1) : R(0)); break;
+                        case 1:
+                            e = (f[0] / axe[q]) * (f[0] / axe[q]) + (f[1] / aye[q]) * (f[1] / aye[q])
+                              + (f[2] / aze[q]) * (f[2] / aze[q]);
+                            si = si * ((R(1.0) - e) >= R(0) ? R(1) : R(0));
+                            break;
+                        case 2:
+                            e = (f[0] / axc[q]) * (f[0] / axc[q]) + (f[1] / ayc[q]) * (f[1] / ayc[q]);
+                            si = si * ((R(1.0) - e) >= R(0) ? R(1) : R(0));
+                            break;
+                        default:
+                            e = (f[0] / axk[q]) * (f[0] / axk[q]) + (f[1] / ayk[q]) * (f[1] / ayk[q]);
+                            si = si * ((f[2] * f[2] - e) >= R(0) ? R(1) : R(0))
+                                    * (f[2] >= R(0) ? R(1) : R(0));
+                            break;
+                        }
+                    }
+                s[((long)i * ny + j) * nz + k] = att * si;
+            }
+}
+#undef ALARGE

@@ -57,3 +57,62 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def quadric_objects(scale=1.0):
+    """Analytic objects of every surface kind, sized for a volume of half
+    width ~16*scale around the origin: a capped and tilted cylinder, a capped
+    cone, an ellipsoid cut by a plane, a box of six planes, and two closed-form
+    ellipsoids."""
+    from ct2 import quadric as q
+    s = scale
+    cyl = q.gen_object(1.0 * s, -0.5 * s, 0.3 * s, att=0.8)
+    cyl.add_surface(q.cylindrical(0., 0., 0., 0., 0., 0., ax=6. * s, ay=4. * s))
+    cyl.add_surface(q.planar(0., 0., -7. * s))                        # z >= -7
+    cyl.add_surface(q.planar(0., 0., 7. * s, 0., np.pi, 0.))          # z <= 7
+    cyl.rotate(0.4, 0.9, -0.3)
+    cone = q.gen_object(-2. * s, 1. * s, -6. * s, att=1.3)
+    cone.add_surface(q.conical(0., 0., 0., 0., 0., 0., ax=0.7, ay=0.5))
+    cone.add_surface(q.planar(0., 0., 11. * s, 0., np.pi, 0.))        # z <= 11
+    cone.rotate(0.2, -0.35, 0.0)
+    cut = q.gen_object(2. * s, 2. * s, -1. * s, att=-0.4)
+    cut.add_surface(q.planar(0.5 * s, 0., 0., 0.3, 1.2, 0.))
+    cut.add_surface(q.ellipsoidal(0., 0., 0., 0.5, 0.2, 0.1, ax=7. * s, ay=5. * s, az=9. * s))
+    box = q.gen_object(0., 0., 0., att=0.25)
+    for axis, sgn in ((0, 1), (0, -1), (1, 1), (1, -1), (2, 1), (2, -1)):
+        # frame z along +-axis: plane through -9s along the normal
+        beta, alpha = {(2, 1): (0., 0.), (2, -1): (np.pi, 0.), (0, 1): (np.pi / 2, 0.),
+                       (0, -1): (-np.pi / 2, 0.), (1, 1): (np.pi / 2, np.pi / 2),
+                       (1, -1): (-np.pi / 2, np.pi / 2)}[(axis, sgn)]
+        c = [0., 0., 0.]
+        c[axis] = -sgn * 9. * s
+        box.add_surface(q.planar(*c, alpha, beta, 0.))
+    e1 = q.ellipsoid(1. * s, -2. * s, 3. * s, att=0.9, ax=8. * s, ay=5. * s, az=11. * s,
+                     alpha=0.3, beta=1.1)
+    e2 = q.ellipsoid(-4. * s, 3. * s, -2. * s, att=0.35, ax=3. * s, ay=3. * s, az=3. * s)
+    return [cyl, cone, cut, box, e1, e2]
+
+
+def f2py_surface_args(obj):
+    """gen_shape_proj's arguments after att for a gen_object, packed as the
+    reference's gen_object.project_to packs them (tomo3D.py:2400-2530)."""
+    cols = [[] for _ in range(12)]
+    ax = {"ellipsoid": ([1.], [1.], [1.]), "cylinder": ([1.], [1.]), "cone": ([1.], [1.])}
+    counts = {"plane": 1, "ellipsoid": 1, "cylinder": 1, "cone": 1}
+    for label in ("plane", "ellipsoid", "cylinder", "cone"):
+        for s in obj.surfaces:
+            if s.label != label:
+                continue
+            for c, v in enumerate((s.xu1, s.xu2, s.xu3, s.yu1, s.yu2, s.yu3, s.zu1, s.zu2, s.zu3,
+                                   s.xc + obj.x0, s.yc + obj.y0, s.zc + obj.z0)):
+                cols[c].append(v)
+            if label != "plane":
+                for lst, v in zip(ax[label], s.axes()):
+                    lst.append(v)
+            counts[label] += 1
+    return (obj.nsurf, *cols, counts["plane"], counts["ellipsoid"], *ax["ellipsoid"],
+            counts["cylinder"], *ax["cylinder"], counts["cone"], *ax["cone"])
+
+
+def ellipsoid_args(e):
+    return (e.ax, e.ay, e.az, e.x0, e.y0, e.z0, e.alpha, e.beta, e.att)

@@ -240,3 +240,110 @@ def test_p_gradients_finite_differences(o64, p):
     # the values themselves: the restated sums agree with the closed forms
     assert o64.total_dvar3d_p(p, x, *n, *vox) == pytest.approx(_tvp64(x, p, vox), rel=1e-6)
 
+
+
+# ---- analytic phantoms: brute-force chords from the objects' definitions ----
+def _inside(ob, P):
+    """Indicator of a phantom object at points P (..., 3), from the definitions:
+    plane z' >= 0, ellipsoid sum (x'/a)^2 <= 1, cylinder (x'/a)^2 + (y'/b)^2 <= 1,
+    cone z' >= 0 and (x'/a)^2 + (y'/b)^2 <= z'^2, in each surface frame."""
+    from ct2 import quadric
+    if isinstance(ob, quadric.ellipsoid):
+        ca, sa, cb, sb = np.cos(ob.alpha), np.sin(ob.alpha), np.cos(ob.beta), np.sin(ob.beta)
+        r = P - np.array([ob.x0, ob.y0, ob.z0])
+        r1x, r1y = cb * r[..., 0] + sb * r[..., 1], -sb * r[..., 0] + cb * r[..., 1]
+        r2x, r2z = ca * r1x + sa * r[..., 2], -sa * r1x + ca * r[..., 2]
+        return (r2x / ob.ax) ** 2 + (r1y / ob.ay) ** 2 + (r2z / ob.az) ** 2 <= 1.0
+    ok = np.ones(P.shape[:-1], bool)
+    for s in ob.surfaces:
+        c = np.array([s.xc + ob.x0, s.yc + ob.y0, s.zc + ob.z0])
+        f = (P - c) @ np.array([[s.xu1, s.xu2, s.xu3], [s.yu1, s.yu2, s.yu3],
+                                [s.zu1, s.zu2, s.zu3]]).T
+        if s.label == "plane":
+            ok &= f[..., 2] >= 0
+        elif s.label == "ellipsoid":
+            ok &= (f[..., 0] / s.ax) ** 2 + (f[..., 1] / s.ay) ** 2 + (f[..., 2] / s.az) ** 2 <= 1
+        elif s.label == "cylinder":
+            ok &= (f[..., 0] / s.ax) ** 2 + (f[..., 1] / s.ay) ** 2 <= 1
+        else:
+            ok &= (f[..., 2] >= 0) & ((f[..., 0] / s.ax) ** 2 + (f[..., 1] / s.ay) ** 2
+                                      <= f[..., 2] ** 2)
+    return ok
+
+
+def test_analytic_chords_brute_force(o64):
+    """Every object's projection equals att times the length of the ray inside
+    it, measured by dense sampling along the ray (step 0.006, so the chord is
+    known to about two steps)."""
+    from cases import ellipsoid_args, f2py_surface_args, quadric_objects
+    from ct2 import quadric
+    ind, fv, geo, _, _ = cone_case(n=32, ns=3, holes=False)
+    ns, nu, nv, du, dv, u0, v0 = geo
+    fv64 = np.asarray(fv, np.float64)
+    j, k = np.meshgrid(np.arange(0, nu, 3), np.arange(0, nv, 3), indexing="ij")
+    nsamp = 10000
+    for ob in quadric_objects():
+        if isinstance(ob, quadric.ellipsoid):
+            proj = o64.ellipsoid_proj(ind, fv, nu, nv, du, dv, u0, v0, *ellipsoid_args(ob))
+        else:
+            proj = o64.gen_shape_proj(ind, fv, nu, nv, du, dv, u0, v0, ob.att,
+                                      *f2py_surface_args(ob))
+        for p in range(ns):
+            S = fv64[p, 0:3]
+            u = u0 + (j + 0.5) * du
+            v = v0 + (k + 0.5) * dv
+            B = fv64[p, 6:9] + u[..., None] * fv64[p, 9:12] + v[..., None] * fv64[p, 12:15]
+            d = B - S
+            L = np.linalg.norm(d, axis=-1)
+            tc = -(S * d).sum(-1) / (L * L)            # closest approach to the origin
+            half = 30.0 / L
+            t = tc[..., None] + half[..., None] * np.linspace(-1, 1, nsamp)
+            P = S + t[..., None] * d[..., None, :]
+            step = 2 * 30.0 / (nsamp - 1)
+            chord = _inside(ob, P).sum(-1) * step
+            np.testing.assert_allclose(proj[p][j, k], ob.att * chord, atol=3 * step * abs(ob.att))
+
+
+def test_analytic_closed_forms(o64):
+    """A ball's chord is 2 sqrt(r^2 - d^2) for a ray at distance d from the
+    centre, through ellipsoid_proj and through a general object with one
+    ellipsoidal surface; a voxel embedding of an axis-aligned box of planes
+    fills exactly the voxels whose centres lie inside."""
+    from cases import f2py_surface_args
+    from ct2 import quadric
+    ind, fv, geo, _, _ = cone_case(n=32, ns=4, holes=False)
+    ns, nu, nv, du, dv, u0, v0 = geo
+    c, r = np.array([1.5, -2.0, 0.5]), 9.0
+    a = o64.ellipsoid_proj(ind, fv, nu, nv, du, dv, u0, v0, r, r, r, *c, 0.7, -0.4, 2.0)
+    ball = quadric.gen_object(*c, att=2.0)
+    ball.add_surface(quadric.ellipsoidal(0., 0., 0., 0.3, 0.2, 0.1, ax=r, ay=r, az=r))
+    b = o64.gen_shape_proj(ind, fv, nu, nv, du, dv, u0, v0, 2.0, *f2py_surface_args(ball))
+    fv64 = np.asarray(fv, np.float64)
+    j, k = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    for p in range(ns):
+        S = fv64[p, 0:3]
+        B = (fv64[p, 6:9] + (u0 + (j + 0.5) * du)[..., None] * fv64[p, 9:12]
+             + (v0 + (k + 0.5) * dv)[..., None] * fv64[p, 12:15])
+        d = (B - S) / np.linalg.norm(B - S, axis=-1)[..., None]
+        w = c - S
+        dist2 = (w * w).sum(-1) - ((w * d).sum(-1)) ** 2
+        want = 2.0 * 2.0 * np.sqrt(np.maximum(r * r - dist2, 0.0))
+        np.testing.assert_allclose(a[p], want, atol=1e-7)
+        np.testing.assert_allclose(b[p], want, atol=1e-7)
+    box = quadric.gen_object(0.3, -0.2, 0.1, att=1.5)
+    for (alpha, beta), cc in (((0., 0.), (0, 0, -4.)), ((0., np.pi), (0, 0, 5.)),
+                              ((0., np.pi / 2), (-6., 0, 0)), ((0., -np.pi / 2), (3., 0, 0)),
+                              ((np.pi / 2, np.pi / 2), (0, -2., 0)),
+                              ((np.pi / 2, -np.pi / 2), (0, 7., 0))):
+        box.add_surface(quadric.planar(*cc, alpha, beta, 0.))
+    g = (24, 22, 20, 0.75, 0.8, 0.9, -9.1, -8.7, -9.05)
+    vol = o64.gen_obj_embed(*g, 1.5, *f2py_surface_args(box))
+    x = -9.1 + (np.arange(24) + 0.5) * 0.75
+    y = -8.7 + (np.arange(22) + 0.5) * 0.8
+    z = -9.05 + (np.arange(20) + 0.5) * 0.9
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    want = 1.5 * ((X >= -5.7) & (X <= 3.3) & (Y >= -2.2) & (Y <= 6.8) & (Z >= -3.9) & (Z <= 5.1))
+    assert np.array_equal(vol, want)
+    e = o64.ellipsoid_embed(*g, 1.5, 0., 0., 0.5, 0.5, 0.5, 5.0, 4.0, 3.0)
+    inside = ((X - 0.5) / 5.0) ** 2 + ((Y - 0.5) / 4.0) ** 2 + ((Z - 0.5) / 3.0) ** 2 < 1.0
+    assert np.array_equal(e, 1.5 * inside)

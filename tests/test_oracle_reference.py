@@ -128,3 +128,47 @@ def test_totalvar_p_family(ref, o32, p):
             gb, bb = getattr(b, name)(p, vol, *n, *extra, nq)
             assert np.array_equal(ga, gb) and ba == bb, name
 
+
+
+def test_analytic_phantoms(ref, o32):
+    """gen_shape_proj / ellipsoid_proj (view and ray-list forms) and the two
+    embedders: the FP32 restatement equals the compiled Fortran bit for bit
+    on objects with every surface kind."""
+    from cases import ellipsoid_args, f2py_surface_args, quadric_objects
+    from ct2 import quadric
+    ind, fv, geo, _, _ = cone_case(n=32, ns=5)
+    ns, nu, nv, du, dv, u0, v0 = geo
+    objs = quadric_objects()
+    nonzero = 0
+    for ob in objs:
+        for p in range(ns):
+            args = (ind[p], fv[p], nu, nv, du, dv, u0, v0)
+            if isinstance(ob, quadric.ellipsoid):
+                a = ref.view_proj3D.ellipsoid_proj(*args, *ellipsoid_args(ob))
+                b = o32.view_proj3D.ellipsoid_proj(*args, *ellipsoid_args(ob))
+            else:
+                a = ref.view_proj3D.gen_shape_proj(*args, ob.att, *f2py_surface_args(ob))
+                b = o32.view_proj3D.gen_shape_proj(*args, ob.att, *f2py_surface_args(ob))
+            assert np.array_equal(a, b), (type(ob).__name__, p)
+            nonzero += int((a != 0).sum())
+    assert nonzero > 2000
+    rind, rv, nr, _, _ = ray_case(nr=800)
+    for ob in objs:
+        if isinstance(ob, quadric.ellipsoid):
+            a = ref.ray_proj3D.ellipsoid_proj(rind, rv, nr, *ellipsoid_args(ob))
+            b = o32.ray_proj3D.ellipsoid_proj(rind, rv, nr, *ellipsoid_args(ob))
+        else:
+            a = ref.ray_proj3D.gen_shape_proj(rind, rv, nr, ob.att, *f2py_surface_args(ob))
+            b = o32.ray_proj3D.gen_shape_proj(rind, rv, nr, ob.att, *f2py_surface_args(ob))
+        assert np.array_equal(a, b)
+    g = (20, 22, 18, 1.6, 1.5, 1.9, -16., -16.5, -17.)
+    for ob in objs:
+        if isinstance(ob, quadric.ellipsoid):
+            e = ellipsoid_args(ob)
+            a = ref.tomo3Dfort.ellipsoid_embed(*g, e[8], e[6], e[7], *e[3:6], *e[0:3])
+            b = o32.tomo3Dfort.ellipsoid_embed(*g, e[8], e[6], e[7], *e[3:6], *e[0:3])
+        else:
+            a = ref.tomo3Dfort.gen_obj_embed(*g, ob.att, *f2py_surface_args(ob))
+            b = o32.tomo3Dfort.gen_obj_embed(*g, ob.att, *f2py_surface_args(ob))
+        assert np.array_equal(a, b) and np.array_equal(np.signbit(a), np.signbit(b))
+        assert (a != 0).any()
