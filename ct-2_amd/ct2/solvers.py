@@ -22,7 +22,6 @@
 """
 from __future__ import annotations
 
-import json
 import os
 
 import numpy as np
@@ -129,36 +128,34 @@ def fdk_em_tv(sino, image, op, n_em=20, n_tv=5, eps=1e-8, graph=True):
 
 # --------------------------------------------------------------- ASD-POCS --
 def _save_checkpoint(path, it, gf, reg_fac, meps, image):
-    os.makedirs(path, exist_ok=True)
-    m = image.mat
-    arr = m.detach().cpu().numpy() if isinstance(m, torch.Tensor) else np.asarray(m)
-    tmp = os.path.join(path, "imageTemp.dat.tmp")
-    arr.astype(np.float32).tofile(tmp)
-    os.replace(tmp, os.path.join(path, "imageTemp.dat"))
-    with open(os.path.join(path, "state.json.tmp"), "w") as f:
-        json.dump({"iter": it, "gf": gf, "reg_fac": reg_fac, "meps": meps}, f)
-    os.replace(os.path.join(path, "state.json.tmp"), os.path.join(path, "state.json"))
+    """The reference's run state: imageTemp.dat, iternum.txt, parms.txt
+    (examples/sphericalRadon.py:171-183)."""
+    from . import Utils
+    Utils.write_state(path, it, gf, reg_fac, meps, image.mat)
 
 
 def _load_checkpoint(path, image):
-    with open(os.path.join(path, "state.json")) as f:
-        s = json.load(f)
-    arr = np.fromfile(os.path.join(path, "imageTemp.dat"), dtype=np.float32)
-    arr = arr.reshape(image.nx, image.ny, image.nz)
+    from . import Utils
+    it, gf, reg_fac, meps = Utils.read_state(path)
+    arr = Utils.read_dat(os.path.join(path, "imageTemp.dat"), (image.nx, image.ny, image.nz))
     if isinstance(image.mat, torch.Tensor):
         image.mat = torch.from_numpy(arr).to(image.mat.device)
     else:
         image.mat = arr
-    return s["iter"], s["gf"], s["reg_fac"], s["meps"]
+    return it, gf, reg_fac, meps
 
 
 def asd_pocs(image, sino, work_sino, max_iter, nart=1, ngradmax=20, gf=0.2, reg_fac=1.0,
              red_fact=0.98, red_reg=0.995, conv=0.99, iter_eps=500, art_alg='art_linear',
-             checkpoint_dir=None, resume=False, traffic_file=None, truth=None, log=None):
+             checkpoint_dir=None, resume=False, traffic_file=None, truth=None, log=None,
+             bang_num=None):
     """ASD-POCS of emil_code/examples/jawPhantomCCB.py:191-327 on tomo3D
     objects (numpy or device arrays).  Returns the list of per-iteration
-    records (iter, data distance, TV, image error) that the reference appends
-    to derrEvol.txt."""
+    records (iter, data distance, TV, image error).  With checkpoint_dir the
+    records are also appended to derrEvol.txt there, the run state is saved
+    after every iteration in the reference's files (imageTemp.dat,
+    iternum.txt, parms.txt), and with bang_num a snapshot image<iter>.dat is
+    written every bang_num iterations (jawPhantomCCB.py:257-268)."""
     meps = -1.0
     it = 0
     if resume:
@@ -190,6 +187,12 @@ def asd_pocs(image, sino, work_sino, max_iter, nart=1, ngradmax=20, gf=0.2, reg_
             records.append((it, ddist, posttv, ierr))
             if log is not None:
                 log(it, ddist, posttv, ierr)
+            if checkpoint_dir is not None:
+                from . import Utils
+                Utils.append_derr(checkpoint_dir, it, ddist, posttv, ierr)
+        if checkpoint_dir is not None and bang_num and it % bang_num == 0:
+            from . import Utils
+            Utils.write_dat(os.path.join(checkpoint_dir, f"image{it}.dat"), image.mat)
         image.copy_to(pre)
         ngrad = 0
         while ngrad < ngradmax and not done:

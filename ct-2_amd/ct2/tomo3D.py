@@ -127,6 +127,21 @@ class sinogram3D:
         return (np.array(c[0:3]), np.array(c[3:6]), np.array(c[6:9]), np.array(c[9:12]),
                 np.array(c[12:15]), np.array(c[15:18]))
 
+    def read(self, file_name, data_type=default_type):
+        """tomo3D.py:202-208 / :876-880: raw array of the current shape; a
+        device array stays on the device."""
+        shape = tuple(self.mat.shape)
+        a = np.fromfile(file_name, dtype=data_type).reshape(shape)
+        if _is_tensor(self.mat):
+            self.mat = _torch().from_numpy(a).to(self.mat.device)
+        else:
+            self.mat = a
+
+    def write_file(self, file_name):
+        """tomo3D.py:254-257 / :883-885: raw native bytes in C order."""
+        from . import Utils
+        Utils.write_dat(file_name, self.mat)
+
     def duplicate(self):
         new = sinogram3D(self.config_name, self.parms, (self.ns, self.nu, self.nv), self.slen,
                          self.ulen, self.vlen, self.s0, self.u0, self.v0, self.s_include_endpoints)
@@ -240,6 +255,21 @@ class image3D:
 
     def _grid(self):
         return (self.dx, self.dy, self.dz, self.x0, self.y0, self.z0, self.nx, self.ny, self.nz)
+
+    def read(self, file_name, data_type=default_type):
+        """tomo3D.py:202-208 / :876-880: raw array of the current shape; a
+        device array stays on the device."""
+        shape = tuple(self.mat.shape)
+        a = np.fromfile(file_name, dtype=data_type).reshape(shape)
+        if _is_tensor(self.mat):
+            self.mat = _torch().from_numpy(a).to(self.mat.device)
+        else:
+            self.mat = a
+
+    def write_file(self, file_name):
+        """tomo3D.py:254-257 / :883-885: raw native bytes in C order."""
+        from . import Utils
+        Utils.write_dat(file_name, self.mat)
 
     def duplicate(self):
         new = image3D((self.nx, self.ny, self.nz), self.xlen, self.ylen, self.zlen,

@@ -113,6 +113,11 @@ def test_asd_pocs_resume_continues_trajectory(tmp_path):
                       checkpoint_dir=str(tmp_path), resume=True)
     assert [r[0] for r in rec_b1 + rec_b2] == [1, 2, 3, 4]
     assert rel_l2(c.mat.cpu().numpy(), a.mat.cpu().numpy()) < 1e-6
+    # the reference's state files and log
+    from ct2 import Utils
+    assert (tmp_path / "iternum.txt").read_text() == "4"
+    log = Utils.read_derr(str(tmp_path / "derrEvol.txt"))
+    np.testing.assert_array_equal(log, np.array(rec_b1 + rec_b2, np.float64))
     for ra, rb in zip(rec_a, rec_b1 + rec_b2):
         assert abs(ra[1] - rb[1]) <= 1e-5 * abs(ra[1])
     # data distance falls
