@@ -75,6 +75,9 @@ SIGNATURES = {
     "ct2_phantom_project": "PiPiPPPiiiffffiS",
     "ct2_phantom_project_rays": "PiPiPPPiiS",
     "ct2_phantom_embed": "PiPiPiiiffffffiS",
+    "ct2_sinoscale": "PPiiiffffS",
+    "ct2_sinoave": "PiiiP" + "i" * 9 + "S",
+    "ct2_sinodist": "PiiiP" + "i" * 9 + "S",
     "ct2_fdk_sino_weight3d": "ffiiiffffffPPS",
     "ct2_fdk_sino_filter3d": "fiiiiPPifS",
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",

@@ -183,12 +183,45 @@ class sinogram3D:
         else:
             self.mat = p
 
+    def homoscale(self):
+        """Divides each ray by its length (tomo3D.py:395-399)."""
+        from . import tomo3Dfort
+        self.mat = tomo3Dfort.sinoscale(self.mat, self.frame_vectors, self.ns, self.nu, self.nv,
+                                        self.du, self.dv, self.u0, self.v0)
+
     def backproject_to(self, image, alg='adj_linear'):
         """emil_code/tomo3D.py:432-472."""
         bp = _pick(bp_algs_dict, alg, "backprojection")
         image.mat = bp(self.mat, self.indicator, self.frame_vectors, self.ns, self.nu, self.nv,
                        self.du, self.dv, self.u0, self.v0, image.dx, image.dy, image.dz,
                        image.x0, image.y0, image.z0, image.nx, image.ny, image.nz)
+
+
+class raybunch(sinogram3D):
+    """A coarse sinogram of averaged ray windows (tomo3D.py:484-534): coarse bin
+    (i, j, k) is the mean of the fine window [i*nds, i*nds+nws) x [j*ndu,
+    j*ndu+nwu) x [k*ndv, k*ndv+nwv)."""
+
+    def __init__(self, sino, nds, ndu, ndv, nws, nwu, nwv):
+        self.nviews = sino.ns
+        self.nds, self.ndu, self.ndv = nds, ndu, ndv
+        self.nws, self.nwu, self.nwv = nws, nwu, nwv
+        sinogram3D.__init__(self, config_name=sino.config_name, parms=sino.parms,
+                            shape=(sino.ns // nds, sino.nu // ndu, sino.nv // ndv),
+                            slen=sino.slen, ulen=sino.ulen, vlen=sino.vlen, s0=sino.s0,
+                            u0=sino.u0, v0=sino.v0, s_include_endpoints=sino.s_include_endpoints)
+
+    def _n9(self):
+        return (self.ns, self.nu, self.nv, self.nds, self.ndu, self.ndv,
+                self.nws, self.nwu, self.nwv)
+
+    def get_data(self, sino):
+        from . import tomo3Dfort
+        self.mat = tomo3Dfort.sinoave(sino.mat, sino.ns, sino.nu, sino.nv, *self._n9())
+
+    def put_data(self, sino):
+        from . import tomo3Dfort
+        sino.mat = tomo3Dfort.sinodist(self.mat, sino.ns, sino.nu, sino.nv, *self._n9())
 
 
 class linogram3D:

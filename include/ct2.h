@@ -385,6 +385,24 @@ int ct2_phantom_embed(const ct2_shape* shapes, int nshape, const ct2_surface* su
                       float dz, float x0, float y0, float z0, int accumulate,
                       ct2_stream_t stream);
 
+/* -------------------------------------------------------- ray bunching -- */
+/* tomo3DfortranAux.f:1-67 sinoscale (sinogram3D.homoscale, tomo3D.py:395):
+ * sinomat[p,j,k] /= |source_p - bin centre (j,k)|, in place. */
+int ct2_sinoscale(float* sinomat, const float* frame_vectors, int ns, int nu, int nv, float du,
+                  float dv, float u0, float v0, ct2_stream_t stream);
+/* tomo3DfortranAux.f:71-122 sinoave (raybunch.get_data, tomo3D.py:504):
+ * sinoout[i,j,k] = mean of sinomat over [i*nds, i*nds+nws) x [j*ndu,
+ * j*ndu+nwu) x [k*ndv, k*ndv+nwv); every window must lie inside sinomat. */
+int ct2_sinoave(const float* sinomat, int ns, int nu, int nv, float* sinoout, int nss, int nus,
+                int nvs, int nds, int ndu, int ndv, int nws, int nwu, int nwv,
+                ct2_stream_t stream);
+/* tomo3DfortranAux.f:125-176 sinodist (raybunch.put_data, tomo3D.py:520):
+ * every window of sinomat takes its coarse value from sinoin (the last
+ * window in loop order wins where windows overlap); other bins are 0. */
+int ct2_sinodist(float* sinomat, int ns, int nu, int nv, const float* sinoin, int nss, int nus,
+                 int nvs, int nds, int ndu, int ndv, int nws, int nwu, int nwv,
+                 ct2_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ Two backends expose the reference's f2py call signatures (hidden
   ``ora_body.h``.  ``prec="f32"`` is Fortran-faithful FP32, ``prec="f64"``
   the same algorithm in FP64.
 * ``oracle.reference()``: the reference Fortran itself
-  (``emil_code/{view_proj3D,view_proj3D_tomo,ray_proj3D,totalvar}.f``)
+  (``emil_code/{view_proj3D,view_proj3D_tomo,ray_proj3D,totalvar,tomo3DfortranAux}.f``)
   compiled by ``oracle/Makefile`` into ``oracle/_ref/``, called through the
   Fortran ABI (every argument by reference, arrays column-major).  Arrays are
   handed over as F-ordered copies of the same logical arrays, and outputs are
@@ -349,6 +349,28 @@ class _Restated:
                                       *self._geo(dx, dy, dz, x0i, y0i, z0i, att), *targs)
         return out
 
+    # ---- ray bunching (tomo3DfortranAux.f:1-176) ----
+    def sinoscale(self, sinomat, fv, ns, nu, nv, du, dv, u0, v0):
+        out = self._a(sinomat).reshape(ns, nu, nv).copy()
+        fv = np.ascontiguousarray(fv, np.float32).reshape(ns, 18)
+        self._fn("ora_sinoscale")(_ptr(out), _ptr(fv), C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                                  *self._geo(du, dv, u0, v0))
+        return out
+
+    def sinoave(self, sinomat, ns, nu, nv, nss, nus, nvs, nds, ndu, ndv, nws, nwu, nwv):
+        s = self._a(sinomat).reshape(ns, nu, nv)
+        out = np.zeros((nss, nus, nvs), self.dt)
+        self._fn("ora_sinoave")(_ptr(s), *[C.c_int(v) for v in (ns, nu, nv)], _ptr(out),
+                                *[C.c_int(v) for v in (nss, nus, nvs, nds, ndu, ndv, nws, nwu, nwv)])
+        return out
+
+    def sinodist(self, sinoin, ns, nu, nv, nss, nus, nvs, nds, ndu, ndv, nws, nwu, nwv):
+        src = self._a(sinoin).reshape(nss, nus, nvs)
+        out = np.zeros((ns, nu, nv), self.dt)
+        self._fn("ora_sinodist")(_ptr(out), *[C.c_int(v) for v in (ns, nu, nv)], _ptr(src),
+                                 *[C.c_int(v) for v in (nss, nus, nvs, nds, ndu, ndv, nws, nwu, nwv)])
+        return out
+
     # ---- f2py-named facades (reference module/function names) ----
     def modules(self):
         o = self
@@ -382,7 +404,8 @@ class _Restated:
             gen_shape_proj=o.ray_gen_shape_proj,
             ellipsoid_proj=o.ray_ellipsoid_proj,
         )
-        aux = SimpleNamespace(ellipsoid_embed=o.ellipsoid_embed, gen_obj_embed=o.gen_obj_embed)
+        aux = SimpleNamespace(ellipsoid_embed=o.ellipsoid_embed, gen_obj_embed=o.gen_obj_embed,
+                              sinoscale=o.sinoscale, sinoave=o.sinoave, sinodist=o.sinodist)
         tv = SimpleNamespace(
             total_var3d=lambda lam, s, nx, ny, nz: o.total_var3d(lam, s, nx, ny, nz),
             total_var_grad3d=lambda lam, s, nx, ny, nz, nq: o.total_var_grad3d(lam, s, nx, ny, nz, nq),
@@ -653,6 +676,30 @@ class _Ref:
                    r(dx), r(dy), r(dz), r(x0i), r(y0i), r(z0i), r(att), *targs)
         return np.ascontiguousarray(out)
 
+    def _sinoscale(self, sinomat, fv, ns, nu, nv, du, dv, u0, v0):
+        s = self._F(np.asarray(sinomat).reshape(ns, nu, nv)).copy(order="F")
+        f = self._F(np.asarray(fv, np.float32).reshape(ns, 18))
+        i, r = self._i, self._r
+        self._call("tomo3DfortranAux", "sinoscale", _ptr(s), _ptr(f), i(ns), i(nu), i(nv),
+                   r(du), r(dv), r(u0), r(v0))
+        return np.ascontiguousarray(s)
+
+    def _sinoave(self, sinomat, ns, nu, nv, *n9):
+        s = self._F(np.asarray(sinomat).reshape(ns, nu, nv))
+        out = self._F(np.zeros(tuple(n9[:3])))
+        i = self._i
+        self._call("tomo3DfortranAux", "sinoave", _ptr(s), _ptr(out), i(ns), i(nu), i(nv),
+                   *[i(v) for v in n9])
+        return np.ascontiguousarray(out)
+
+    def _sinodist(self, sinoin, ns, nu, nv, *n9):
+        src = self._F(np.asarray(sinoin).reshape(tuple(n9[:3])))
+        out = self._F(np.zeros((ns, nu, nv)))
+        i = self._i
+        self._call("tomo3DfortranAux", "sinodist", _ptr(out), _ptr(src), i(ns), i(nu), i(nv),
+                   *[i(v) for v in n9])
+        return np.ascontiguousarray(out)
+
     def modules(self):
         o = self
         vp = SimpleNamespace(**{
@@ -691,7 +738,8 @@ class _Ref:
             lp_grad3d=lambda *a: o._pgrad("lp_grad3d_", *a),
             total_p_var_grad3d=lambda *a: o._pgrad("total_p_var_grad3d_", *a),
             total_p_dvar_grad3d=lambda *a: o._pgrad("total_p_dvar_grad3d_", *a))
-        aux = SimpleNamespace(ellipsoid_embed=o._ell_embed, gen_obj_embed=o._gen_embed)
+        aux = SimpleNamespace(ellipsoid_embed=o._ell_embed, gen_obj_embed=o._gen_embed,
+                              sinoscale=o._sinoscale, sinoave=o._sinoave, sinodist=o._sinodist)
         return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv,
                                tomo3Dfort=aux)
 

@@ -1,6 +1,7 @@
 /*
  * ct2_oracle.c -- CPU restatement of ct-2's reconstruction hot path
- * (emil_code/view_proj3D.f, view_proj3D_tomo.f, ray_proj3D.f, totalvar.f).
+ * (emil_code/view_proj3D.f, view_proj3D_tomo.f, ray_proj3D.f, totalvar.f,
+ * tomo3DfortranAux.f).
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP kernels and the CPU
  * baseline leg of bench.py.  Nothing in the product imports or links it.

@@ -1137,3 +1137,60 @@ EXPORT void FN(ora_gen_obj_embed)(REAL *s, int nx, int ny, int nz, REAL dx, REAL
             }
 }
 #undef ALARGE
+
+/* ---- ray bunching (tomo3DfortranAux.f:1-176; tomo3D.py:395-399, :484-534) ---- */
+
+/* sinoscale (tomo3DfortranAux.f:1-67): each bin divided by the distance
+ * from the source to the bin centre */
+EXPORT void FN(ora_sinoscale)(REAL *sino, const float *fv, int ns, int nu, int nv, REAL du,
+                              REAL dv, REAL u0, REAL v0)
+{
+    for (int p = 0; p < ns; ++p)
+        for (int j = 0; j < nu; ++j)
+            for (int k = 0; k < nv; ++k) {
+                FN(ray_t) ry;
+                FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+                REAL d = SQRT((ry.s[0] - ry.b[0]) * (ry.s[0] - ry.b[0])
+                              + (ry.s[1] - ry.b[1]) * (ry.s[1] - ry.b[1])
+                              + (ry.s[2] - ry.b[2]) * (ry.s[2] - ry.b[2]));
+                long q = ((long)p * nu + j) * nv + k;
+                sino[q] = sino[q] / d;
+            }
+}
+
+/* sinoave (tomo3DfortranAux.f:71-122): out[i,j,k] = mean of the window
+ * [i*nds, i*nds+nws) x [j*ndu, +nwu) x [k*ndv, +nwv), summed in the
+ * Fortran's ii, jj, kk order */
+EXPORT void FN(ora_sinoave)(const REAL *s, int ns, int nu, int nv, REAL *out, int nss, int nus,
+                            int nvs, int nds, int ndu, int ndv, int nws, int nwu, int nwv)
+{
+    (void)ns;
+    for (int i = 0; i < nss; ++i)
+        for (int j = 0; j < nus; ++j)
+            for (int k = 0; k < nvs; ++k) {
+                REAL acc = R(0);
+                for (int ii = 0; ii < nws; ++ii)
+                    for (int jj = 0; jj < nwu; ++jj)
+                        for (int kk = 0; kk < nwv; ++kk)
+                            acc = acc + s[((long)(i * nds + ii) * nu + (j * ndu + jj)) * nv
+                                          + (k * ndv + kk)];
+                out[((long)i * nus + j) * nvs + k] = acc / (REAL)(nws * nwu * nwv);
+            }
+}
+
+/* sinodist (tomo3DfortranAux.f:125-176): every window of the fine
+ * sinogram gets its coarse value; where windows overlap the last write in
+ * the Fortran's loop order wins; uncovered bins are zero */
+EXPORT void FN(ora_sinodist)(REAL *s, int ns, int nu, int nv, const REAL *in, int nss, int nus,
+                             int nvs, int nds, int ndu, int ndv, int nws, int nwu, int nwv)
+{
+    memset(s, 0, sizeof(REAL) * (size_t)ns * nu * nv);
+    for (int i = 0; i < nss; ++i)
+        for (int j = 0; j < nus; ++j)
+            for (int k = 0; k < nvs; ++k)
+                for (int ii = 0; ii < nws; ++ii)
+                    for (int jj = 0; jj < nwu; ++jj)
+                        for (int kk = 0; kk < nwv; ++kk)
+                            s[((long)(i * nds + ii) * nu + (j * ndu + jj)) * nv + (k * ndv + kk)]
+                                = in[((long)i * nus + j) * nvs + k];
+}

@@ -347,3 +347,29 @@ def test_analytic_closed_forms(o64):
     e = o64.ellipsoid_embed(*g, 1.5, 0., 0., 0.5, 0.5, 0.5, 5.0, 4.0, 3.0)
     inside = ((X - 0.5) / 5.0) ** 2 + ((Y - 0.5) / 4.0) ** 2 + ((Z - 0.5) / 3.0) ** 2 < 1.0
     assert np.array_equal(e, 1.5 * inside)
+
+
+def test_raybunch_invariants(o64):
+    """sinoscale times the source-to-bin distance (numpy, FP64) gives the
+    input back; sinoave of non-overlapping windows is the block mean, and
+    sinodist of it spreads each mean over its block, so sinoave(sinodist(y))
+    is y."""
+    ind, fv, geo, _, _ = cone_case(n=32, ns=6, nu=12, nv=10)
+    ns, nu, nv, du, dv, u0, v0 = geo
+    rng = np.random.default_rng(8)
+    s = rng.random((ns, nu, nv)) + 0.5
+    sc = o64.sinoscale(s, fv, ns, nu, nv, du, dv, u0, v0)
+    f = np.asarray(fv, np.float64)
+    u = u0 + (np.arange(nu) + 0.5) * du
+    v = v0 + (np.arange(nv) + 0.5) * dv
+    B = (f[:, None, None, 6:9] + u[None, :, None, None] * f[:, None, None, 9:12]
+         + v[None, None, :, None] * f[:, None, None, 12:15])
+    d = np.linalg.norm(B - f[:, None, None, 0:3], axis=-1)
+    np.testing.assert_allclose(sc * d, s, rtol=1e-12)
+    n9 = (3, 4, 5, 2, 3, 2, 2, 3, 2)
+    a = o64.sinoave(s, ns, nu, nv, *n9)
+    np.testing.assert_allclose(a, s.reshape(3, 2, 4, 3, 5, 2).mean(axis=(1, 3, 5)), rtol=1e-12)
+    y = rng.random((3, 4, 5))
+    spread = o64.sinodist(y, ns, nu, nv, *n9)
+    np.testing.assert_array_equal(spread, np.repeat(np.repeat(np.repeat(y, 2, 0), 3, 1), 2, 2))
+    np.testing.assert_allclose(o64.sinoave(spread, ns, nu, nv, *n9), y, rtol=1e-12)

@@ -172,3 +172,23 @@ def test_analytic_phantoms(ref, o32):
             b = o32.tomo3Dfort.gen_obj_embed(*g, ob.att, *f2py_surface_args(ob))
         assert np.array_equal(a, b) and np.array_equal(np.signbit(a), np.signbit(b))
         assert (a != 0).any()
+
+
+@pytest.mark.parametrize("win", [(1, 2, 2, 1, 2, 2), (2, 3, 2, 3, 4, 2), (1, 4, 4, 1, 4, 4),
+                                 (2, 2, 3, 1, 1, 1)])
+def test_raybunch(ref, o32, win):
+    """sinoscale, sinoave and sinodist (tomo3DfortranAux.f:1-176): FP32
+    restatement equals the compiled Fortran, overlapping windows included."""
+    ind, fv, geo, _, _ = cone_case(n=32, ns=9, nu=20, nv=18)
+    ns, nu, nv, du, dv, u0, v0 = geo
+    rng = np.random.default_rng(7)
+    s = rng.standard_normal((ns, nu, nv)).astype(np.float32)
+    assert np.array_equal(ref.tomo3Dfort.sinoscale(s, fv, ns, nu, nv, du, dv, u0, v0),
+                          o32.tomo3Dfort.sinoscale(s, fv, ns, nu, nv, du, dv, u0, v0))
+    nds, ndu, ndv, nws, nwu, nwv = win
+    nss, nus, nvs = (ns - nws) // nds + 1, (nu - nwu) // ndu + 1, (nv - nwv) // ndv + 1
+    n9 = (nss, nus, nvs, nds, ndu, ndv, nws, nwu, nwv)
+    a = ref.tomo3Dfort.sinoave(s, ns, nu, nv, *n9)
+    assert np.array_equal(a, o32.tomo3Dfort.sinoave(s, ns, nu, nv, *n9))
+    assert np.array_equal(ref.tomo3Dfort.sinodist(a, ns, nu, nv, *n9),
+                          o32.tomo3Dfort.sinodist(a, ns, nu, nv, *n9))
