@@ -159,8 +159,8 @@ __global__ __launch_bounds__(64) void k_art_rays(Grid g, int nr, int strict, flo
 // distance t from S, so t >= tmin (nearest volume point) bounds the gap.
 // Rays whose bins differ by more than r in j or k are |B1-B2| >= (r+1)*pmin
 // apart.  Returns -1 when the bound does not apply (source inside the box).
-static int conflict_radius(const Grid& g, const Grid* g2, const Det& dt, const float* fv_host,
-                           int view0, int nviews) {
+int conflict_radius(const Grid& g, const Grid* g2, const Det& dt, const float* fv_host,
+                    int view0, int nviews) {
     double dmax = fmax(fmax(g.d[0], g.d[1]), g.d[2]);
     double lo[3], hi[3];
     for (int q = 0; q < 3; ++q) {

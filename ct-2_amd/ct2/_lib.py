@@ -78,6 +78,9 @@ SIGNATURES = {
     "ct2_sinoscale": "PPiiiffffS",
     "ct2_sinoave": "PiiiP" + "i" * 9 + "S",
     "ct2_sinodist": "PiiiP" + "i" * 9 + "S",
+    "ct2_stencil_project": "PPPPPPPS",
+    "ct2_stencil_weights": "PPPPPPPPS",
+    "ct2_stencil_art": "PPPfPPPPPS",
     "ct2_fdk_sino_weight3d": "ffiiiffffffPPS",
     "ct2_fdk_sino_filter3d": "fiiiiPPifS",
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",
@@ -96,6 +99,11 @@ class ct2_desc(C.Structure):
                 ("interp", C.c_int), ("axis", C.c_int), ("vol_dtype", C.c_int),
                 ("arith", C.c_int), ("view0", C.c_int), ("nviews", C.c_int),
                 ("brick", C.c_int * 3), ("vstride", C.c_int)]
+
+
+class ct2_stencil(C.Structure):
+    _fields_ = [("ws", C.c_int), ("wu", C.c_int), ("wv", C.c_int),
+                ("nss", C.c_int), ("nsu", C.c_int), ("nsv", C.c_int)]
 
 
 _LIB = None

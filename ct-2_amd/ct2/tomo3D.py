@@ -401,6 +401,28 @@ class image3D:
     def scale(self, factor):
         self.mat = factor * self.mat
 
+    # weighted ray-bunch and derivative ART (tests/cweighted_art_test.py,
+    # tests/cderiv_art_test.py); launch-shape arguments are ignored
+    def cweighted_project_to(self, data, rayweights, nss, nsu, nsv, *launch):
+        from . import weighted
+        weighted.cweighted_project_to(self, data, rayweights, nss, nsu, nsv)
+
+    def put_weights(self, data, rayweights, nss, nsu, nsv, *launch):
+        from . import weighted
+        weighted.put_weights(self, data, rayweights, nss, nsu, nsv)
+
+    def c_ss_weighted_art_step(self, data, rayweights, nss, nsu, nsv, *launch, reg_fac=1.0):
+        from . import weighted
+        weighted.c_ss_weighted_art_step(self, data, rayweights, nss, nsu, nsv, reg_fac=reg_fac)
+
+    def get_deriv_weights(self, data, *launch):
+        from . import weighted
+        weighted.get_deriv_weights(self, data)
+
+    def cderiv_art_step(self, data, *launch, reg_fac=1.0):
+        from . import weighted
+        weighted.cderiv_art_step(self, data, reg_fac=reg_fac)
+
     def project_to(self, sino, alg='proj_linear'):
         """emil_code/tomo3D.py:1127-1168."""
         proj = _pick(proj_algs_dict, alg, "projection")

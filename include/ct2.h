@@ -403,6 +403,43 @@ int ct2_sinodist(float* sinomat, int ns, int nu, int nv, const float* sinoin, in
                  int nvs, int nds, int ndu, int ndv, int nws, int nwu, int nwv,
                  ct2_stream_t stream);
 
+/* ------------------------------------------ weighted and derivative ART -- */
+/* The weighted ray-bunch and derivative measurements whose call sites and
+ * checks the reference keeps (tests/cweighted_art_test.py:80-140,
+ * tests/cderiv_art_test.py:106-140) while their GPU sources are missing.
+ * Coarse bin (I,J,K) of [NS][NU][NV], NS = ceil(ns/nss), NU = ceil(nu/nsu),
+ * NV = ceil(nv/nsv), measures m = sum_q w_q (A x)[fine ray q], where kernel
+ * entry q = (a,c,d) of w[ws][wu][wv] (device memory) is the fine ray
+ * (I*nss + a - os, J*nsu + c - ou, K*nsv + d - ov), os = ws-1-ws/2 (likewise
+ * ou, ov), rays outside the sinogram skipped: scipy.ndimage.convolve(proj,
+ * w[::-1,::-1,::-1], mode='constant')[::nss, ::nsu, ::nsv], the tests' own
+ * check.  A bin is used when indsino at its anchor (I*nss, J*nsu, K*nsv) is 1
+ * (NULL: every bin).  The derivative data are w = [-1, +1] along u with unit
+ * strides.  desc: float32 volume, EXACT arithmetic, linear or nearest, free or
+ * z axis; the view range fields are ignored. */
+typedef struct ct2_stencil {
+    int ws, wu, wv;      /* kernel extents */
+    int nss, nsu, nsv;   /* coarse strides */
+} ct2_stencil;
+/* small[NS][NU][NV] = the measurements of vol (cweighted_project_to);
+ * fine[ns][nu][nv] is work space for the fine projection. */
+int ct2_stencil_project(const ct2_desc* desc, const ct2_stencil* st, const float* w,
+                        const float* vol, const float* frame_vectors, float* fine, float* small,
+                        ct2_stream_t stream);
+/* weights[b] = |r_b|^2 of every used bin's row r_b = sum_q w_q a_q (0 for
+ * unused bins) and image += sum of the used rows (put_weights,
+ * get_deriv_weights).  scratch: nx*ny*nz floats, zero on entry, zero again
+ * on return. */
+int ct2_stencil_weights(const ct2_desc* desc, const ct2_stencil* st, const float* w,
+                        const int8_t* indsino, const float* frame_vectors, float* weights,
+                        float* image, float* scratch, ct2_stream_t stream);
+/* One Kaczmarz sweep over the used bins with weights > 0, in (I,J,K) order:
+ * vol += reg_fac (small[b] - r_b.vol) r_b / weights[b]
+ * (c_ss_weighted_art_step, cderiv_art_step). */
+int ct2_stencil_art(const ct2_desc* desc, const ct2_stencil* st, const float* w, float reg_fac,
+                    const float* small, const float* weights, const int8_t* indsino,
+                    const float* frame_vectors, float* vol, ct2_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

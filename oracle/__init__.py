@@ -371,6 +371,52 @@ class _Restated:
                                  *[C.c_int(v) for v in (nss, nus, nvs, nds, ndu, ndv, nws, nwu, nwv)])
         return out
 
+    # ---- weighted ray-bunch ("stencil") measurements and ART ----
+    def _w(self, w):
+        w = self._a(w)
+        if w.ndim != 3:
+            raise ValueError("stencil weights must be 3-D [ws, wu, wv]")
+        return w
+
+    def stencil_correlate(self, fine, w, nss, nsu, nsv):
+        fine = self._a(fine)
+        ns, nu, nv = fine.shape
+        w = self._w(w)
+        out = np.zeros((-(-ns // nss), -(-nu // nsu), -(-nv // nsv)), self.dt)
+        self._fn("ora_stencil_correlate")(_ptr(fine), C.c_int(ns), C.c_int(nu), C.c_int(nv),
+                                          _ptr(w), *[C.c_int(v) for v in (*w.shape, nss, nsu, nsv)],
+                                          _ptr(out))
+        return out
+
+    def stencil_weights(self, interp, zonly, w, nss, nsu, nsv, indsino, fv, ns, nu, nv, du, dv,
+                        u0, v0, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        w = self._w(w)
+        ind = None if indsino is None else self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32).reshape(ns, 18)
+        wts = np.zeros((-(-ns // nss), -(-nu // nsu), -(-nv // nsv)), self.dt)
+        img = np.zeros((nx, ny, nz), self.dt)
+        self._fn("ora_stencil_weights")(
+            C.c_int(interp), C.c_int(zonly), _ptr(w), *[C.c_int(v) for v in (*w.shape, nss, nsu, nsv)],
+            None if ind is None else _ptr(ind), _ptr(fv), C.c_int(ns), C.c_int(nu), C.c_int(nv),
+            *self._geo(du, dv, u0, v0, dx, dy, dz, x0, y0, z0), C.c_int(nx), C.c_int(ny),
+            C.c_int(nz), _ptr(wts), _ptr(img))
+        return wts, img
+
+    def stencil_art(self, interp, zonly, reg_fac, w, nss, nsu, nsv, small, weights, indsino, fv,
+                    ns, nu, nv, du, dv, u0, v0, smat, dx, dy, dz, x0, y0, z0, nx, ny, nz):
+        w = self._w(w)
+        ind = None if indsino is None else self._ind(indsino, (ns, nu, nv))
+        fv = np.ascontiguousarray(fv, np.float32).reshape(ns, 18)
+        small, weights = self._a(small), self._a(weights)
+        vol = self._a(smat).reshape(nx, ny, nz).copy()
+        self._fn("ora_stencil_art")(
+            C.c_int(interp), C.c_int(zonly), self.cr(reg_fac), _ptr(w),
+            *[C.c_int(v) for v in (*w.shape, nss, nsu, nsv)], _ptr(small), _ptr(weights),
+            None if ind is None else _ptr(ind), _ptr(fv), C.c_int(ns), C.c_int(nu), C.c_int(nv),
+            *self._geo(du, dv, u0, v0), _ptr(vol), *self._geo(dx, dy, dz, x0, y0, z0),
+            C.c_int(nx), C.c_int(ny), C.c_int(nz))
+        return vol
+
     # ---- f2py-named facades (reference module/function names) ----
     def modules(self):
         o = self

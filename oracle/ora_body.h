@@ -1194,3 +1194,148 @@ EXPORT void FN(ora_sinodist)(REAL *s, int ns, int nu, int nv, const REAL *in, in
                             s[((long)(i * nds + ii) * nu + (j * ndu + jj)) * nv + (k * ndv + kk)]
                                 = in[((long)i * nus + j) * nvs + k];
 }
+
+/* ---- weighted ray-bunch ("stencil") measurements and their ART: the
+ *      semantics of the reference's cweighted_project_to / put_weights /
+ *      c_ss_weighted_art_step and get_deriv_weights / cderiv_art_step
+ *      (tests/cweighted_art_test.py:80-140, tests/cderiv_art_test.py:106-140;
+ *      the GPU sources are missing).  Kernel entry q = (a,c,d) of
+ *      w[ws][wu][wv] belongs to fine ray (I*nss+a-os, J*nsu+c-ou, K*nsv+d-ov),
+ *      os = ws-1-ws/2; rays outside the sinogram are skipped. ---- */
+typedef struct {
+    const REAL *w;
+    int ws, wu, wv, nss, nsu, nsv, NS, NU, NV;
+} FN(stencil_t);
+
+static void FN(stencil_init)(FN(stencil_t) *s, const REAL *w, int ws, int wu, int wv, int nss,
+                             int nsu, int nsv, int ns, int nu, int nv)
+{
+    s->w = w; s->ws = ws; s->wu = wu; s->wv = wv;
+    s->nss = nss; s->nsu = nsu; s->nsv = nsv;
+    s->NS = (ns + nss - 1) / nss; s->NU = (nu + nsu - 1) / nsu; s->NV = (nv + nsv - 1) / nsv;
+}
+
+static int FN(stencil_ray)(const FN(stencil_t) *s, int ns, int nu, int nv, int I, int J, int K,
+                           int q, int *p, int *j, int *k)
+{
+    int d = q % s->wv, t = q / s->wv, c = t % s->wu, a = t / s->wu;
+    *p = I * s->nss + a - (s->ws - 1 - s->ws / 2);
+    *j = J * s->nsu + c - (s->wu - 1 - s->wu / 2);
+    *k = K * s->nsv + d - (s->wv - 1 - s->wv / 2);
+    return *p >= 0 && *p < ns && *j >= 0 && *j < nu && *k >= 0 && *k < nv;
+}
+
+/* small[I,J,K] = sum_q w_q fine[ray q], in kernel order */
+EXPORT void FN(ora_stencil_correlate)(const REAL *fine, int ns, int nu, int nv, const REAL *w,
+                                      int ws, int wu, int wv, int nss, int nsu, int nsv,
+                                      REAL *small)
+{
+    FN(stencil_t) s;
+    FN(stencil_init)(&s, w, ws, wu, wv, nss, nsu, nsv, ns, nu, nv);
+    int R = ws * wu * wv;
+    for (int I = 0; I < s.NS; ++I)
+        for (int J = 0; J < s.NU; ++J)
+            for (int K = 0; K < s.NV; ++K) {
+                REAL acc = R(0);
+                for (int q = 0; q < R; ++q) {
+                    int p, j, k;
+                    if (!FN(stencil_ray)(&s, ns, nu, nv, I, J, K, q, &p, &j, &k)) continue;
+                    acc = acc + w[q] * fine[((long)p * nu + j) * nv + k];
+                }
+                small[((long)I * s.NU + J) * s.NV + K] = acc;
+            }
+}
+
+/* weights[b] = |r_b|^2 (FP64 sum, each voxel once), image += r_b, for every
+ * bin whose anchor indicator is 1 (others: weight 0) */
+EXPORT void FN(ora_stencil_weights)(int interp, int zonly, const REAL *w, int ws, int wu, int wv,
+                                    int nss, int nsu, int nsv, const int8_t *ind, const float *fv,
+                                    int ns, int nu, int nv, REAL du, REAL dv, REAL u0, REAL v0,
+                                    REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0,
+                                    int nx, int ny, int nz, REAL *weights, REAL *image)
+{
+    FN(stencil_t) s;
+    FN(stencil_init)(&s, w, ws, wu, wv, nss, nsu, nsv, ns, nu, nv);
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    REAL *scratch = (REAL *)calloc((size_t)nx * ny * nz, sizeof(REAL));
+    FN(list_t) L = {0, 0, NULL, NULL};
+    int R = ws * wu * wv;
+    for (int I = 0; I < s.NS; ++I)
+        for (int J = 0; J < s.NU; ++J)
+            for (int K = 0; K < s.NV; ++K) {
+                long b = ((long)I * s.NU + J) * s.NV + K;
+                long anchor = ((long)(I * nss) * nu + J * nsu) * nv + K * nsv;
+                if (ind && ind[anchor] != 1) { weights[b] = R(0); continue; }
+                for (int pass = 0; pass < 2; ++pass) {
+                    double acc = 0.0;
+                    for (int q = 0; q < R; ++q) {
+                        int p, j, k;
+                        if (!FN(stencil_ray)(&s, ns, nu, nv, I, J, K, q, &p, &j, &k)) continue;
+                        FN(ray_t) ry;
+                        FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+                        L.n = 0;
+                        FN(march)(&ry, &g, zonly, interp, 0, NULL, &L, NULL);
+                        for (long e = 0; e < L.n; ++e) {
+                            long i = L.idx[e];
+                            if (pass == 0) {
+                                scratch[i] = scratch[i] + w[q] * L.w[e];
+                            } else if (scratch[i] != R(0)) {
+                                REAL v = scratch[i];
+                                acc += (double)v * (double)v;
+                                image[i] = image[i] + v;
+                                scratch[i] = R(0);
+                            }
+                        }
+                    }
+                    if (pass == 1) weights[b] = (REAL)acc;
+                }
+            }
+    free(L.idx); free(L.w); free(scratch);
+}
+
+/* Kaczmarz over the used bins with weights > 0 in (I,J,K) order:
+ * vol += reg_fac (small[b] - sum_q w_q (a_q . vol)) (w_q a_q) / weights[b] */
+EXPORT void FN(ora_stencil_art)(int interp, int zonly, REAL reg_fac, const REAL *w, int ws, int wu,
+                                int wv, int nss, int nsu, int nsv, const REAL *small,
+                                const REAL *weights, const int8_t *ind, const float *fv, int ns,
+                                int nu, int nv, REAL du, REAL dv, REAL u0, REAL v0, REAL *vol,
+                                REAL dx, REAL dy, REAL dz, REAL x0, REAL y0, REAL z0, int nx,
+                                int ny, int nz)
+{
+    FN(stencil_t) s;
+    FN(stencil_init)(&s, w, ws, wu, wv, nss, nsu, nsv, ns, nu, nv);
+    FN(grid_t) g;
+    FN(grid_init)(&g, dx, dy, dz, x0, y0, z0, nx, ny, nz);
+    FN(list_t) L = {0, 0, NULL, NULL};
+    int R = ws * wu * wv;
+    for (int I = 0; I < s.NS; ++I)
+        for (int J = 0; J < s.NU; ++J)
+            for (int K = 0; K < s.NV; ++K) {
+                long b = ((long)I * s.NU + J) * s.NV + K;
+                long anchor = ((long)(I * nss) * nu + J * nsu) * nv + K * nsv;
+                if (ind && ind[anchor] != 1) continue;
+                REAL norm = weights[b];
+                if (!(norm > R(0))) continue;
+                REAL rsum = R(0);
+                for (int q = 0; q < R; ++q) {
+                    int p, j, k;
+                    if (!FN(stencil_ray)(&s, ns, nu, nv, I, J, K, q, &p, &j, &k)) continue;
+                    FN(ray_t) ry;
+                    FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+                    rsum = rsum + w[q] * FN(march)(&ry, &g, zonly, interp, 0, vol, NULL, NULL);
+                }
+                REAL resid = reg_fac * (small[b] - rsum);
+                for (int q = 0; q < R; ++q) {
+                    int p, j, k;
+                    if (!FN(stencil_ray)(&s, ns, nu, nv, I, J, K, q, &p, &j, &k)) continue;
+                    FN(ray_t) ry;
+                    FN(view_ray)(fv, p, j, k, du, dv, u0, v0, &ry);
+                    L.n = 0;
+                    FN(march)(&ry, &g, zonly, interp, 0, NULL, &L, NULL);
+                    for (long e = 0; e < L.n; ++e)
+                        vol[L.idx[e]] = vol[L.idx[e]] + resid * (w[q] * L.w[e]) / norm;
+                }
+            }
+    free(L.idx); free(L.w);
+}

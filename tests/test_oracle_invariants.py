@@ -373,3 +373,91 @@ def test_raybunch_invariants(o64):
     spread = o64.sinodist(y, ns, nu, nv, *n9)
     np.testing.assert_array_equal(spread, np.repeat(np.repeat(np.repeat(y, 2, 0), 3, 1), 2, 2))
     np.testing.assert_allclose(o64.sinoave(spread, ns, nu, nv, *n9), y, rtol=1e-12)
+
+
+# ---- weighted ray-bunch and derivative measurements (stencils) ----
+def _stencil_setup(ns=4, nu=14, nv=12, n=12):
+    ind, fv, geo, vol, g2 = cone_case(n=n, ns=ns, nu=nu, nv=nv, phantom="random", holes=False,
+                                      du=2.0, dv=2.0)
+    return ind, fv, geo, vol.astype(np.float64), g2
+
+
+@pytest.mark.parametrize("w,strides", [
+    (np.ones((1, 4, 4)) / 16.0, (1, 4, 4)),          # tests/cweighted_art_test.py:81
+    (np.array([[[-1.0], [1.0]]]), (1, 1, 1)),          # the derivative kernel
+    (np.arange(1, 19, dtype=float).reshape(2, 3, 3) / 7.0, (2, 2, 3)),
+])
+def test_stencil_measurements_and_rows(o64, w, strides):
+    """Measurements are the reference test's scipy correlation of the
+    projection; put_weights' image is the adjoint of the used stencils; each
+    weight is the squared norm of its row, and <row, x> is the measurement
+    (tests/cweighted_art_test.py:92-120)."""
+    from scipy import ndimage
+    ind, fv, geo, vol, g2 = _stencil_setup()
+    ns, nu, nv = geo[:3]
+    nss, nsu, nsv = strides
+    proj = o64.sinoproj(0, 0, None, fv, *geo, vol, *g2)
+    small = o64.stencil_correlate(proj, w, *strides)
+    want = ndimage.convolve(proj, w[::-1, ::-1, ::-1], mode="constant")[::nss, ::nsu, ::nsv]
+    np.testing.assert_allclose(small, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+    rng = np.random.default_rng(3)
+    cind = np.zeros((ns, nu, nv), np.int8)
+    used = [(rng.integers(small.shape[0]), rng.integers(small.shape[1]), rng.integers(small.shape[2]))
+            for _ in range(3)]
+    for (I, J, K) in used:
+        cind[I * nss, J * nsu, K * nsv] = 1
+    wts, img = o64.stencil_weights(0, 0, w, *strides, cind, fv, *geo, *g2)
+    # the fine sinogram holding every used bin's stencil, and its adjoint
+    S = np.zeros((ns, nu, nv))
+    os_ = [e - 1 - e // 2 for e in w.shape]
+    for (I, J, K) in set(used):
+        for a in range(w.shape[0]):
+            for c in range(w.shape[1]):
+                for d in range(w.shape[2]):
+                    p, j, k = I * nss + a - os_[0], J * nsu + c - os_[1], K * nsv + d - os_[2]
+                    if 0 <= p < ns and 0 <= j < nu and 0 <= k < nv:
+                        S[p, j, k] += w[a, c, d]
+    np.testing.assert_allclose(img, o64.sinobackproject(0, 0, S, None, fv, *geo, *g2),
+                               rtol=1e-12, atol=1e-12 * np.abs(img).max())
+    for (I, J, K) in set(used):
+        Sb = np.zeros((ns, nu, nv))
+        for a in range(w.shape[0]):
+            for c in range(w.shape[1]):
+                for d in range(w.shape[2]):
+                    p, j, k = I * nss + a - os_[0], J * nsu + c - os_[1], K * nsv + d - os_[2]
+                    if 0 <= p < ns and 0 <= j < nu and 0 <= k < nv:
+                        Sb[p, j, k] += w[a, c, d]
+        row = o64.sinobackproject(0, 0, Sb, None, fv, *geo, *g2)
+        assert wts[I, J, K] == pytest.approx(float((row * row).sum()), rel=1e-12)
+        assert float((row * vol).sum()) == pytest.approx(small[I, J, K], rel=1e-10, abs=1e-12)
+    assert np.count_nonzero(wts) <= len(set(used))
+
+
+def test_stencil_art_kaczmarz(o64):
+    """From zero with one used bin, the step is row * m / |row|^2 (the
+    reference test's check 4); after a sweep with reg_fac 1 the last bin's
+    equation holds exactly, and sweeps reduce the coarse residual."""
+    w = np.ones((1, 2, 2)) / 4.0
+    strides = (1, 2, 2)
+    ind, fv, geo, vol, g2 = _stencil_setup()
+    ns, nu, nv = geo[:3]
+    nx, ny, nz = g2[6:]
+    proj = o64.sinoproj(0, 0, None, fv, *geo, vol, *g2)
+    small = o64.stencil_correlate(proj, w, *strides)
+    one = np.zeros((ns, nu, nv), np.int8)
+    one[2, 6, 4] = 1
+    wts, row = o64.stencil_weights(0, 0, w, *strides, one, fv, *geo, *g2)
+    x = o64.stencil_art(0, 0, 1.0, w, *strides, small, wts, one, fv, *geo, np.zeros((nx, ny, nz)),
+                        *g2)
+    np.testing.assert_allclose(x, row * small[2, 3, 2] / wts[2, 3, 2], rtol=1e-12, atol=1e-15)
+    allb = np.ones((ns, nu, nv), np.int8)
+    wts, _ = o64.stencil_weights(0, 0, w, *strides, allb, fv, *geo, *g2)
+    x = np.zeros((nx, ny, nz))
+    res = []
+    for _ in range(3):
+        x = o64.stencil_art(0, 0, 1.0, w, *strides, small, wts, allb, fv, *geo, x, *g2)
+        m = o64.stencil_correlate(o64.sinoproj(0, 0, None, fv, *geo, x, *g2), w, *strides)
+        res.append(np.linalg.norm(m - small))
+        last = tuple(np.array(np.nonzero(wts > 0))[:, -1])
+        assert m[last] == pytest.approx(small[last], rel=1e-9)
+    assert res[2] < res[1] < res[0] < 0.5 * np.linalg.norm(small)
