@@ -506,8 +506,10 @@ template <int INTERP, int ARITH, int MODE>
 __global__ __launch_bounds__(256) void k_batch_backproject(const BatchProb* __restrict__ probs,
                                                            const long long* __restrict__ off,
                                                            int nprob, int sub, int nsub,
-                                                           int3 bdim, float lambda) {
+                                                           int3 bdim, float lambda,
+                                                           const float* __restrict__ lambda_dev) {
     extern __shared__ double acc[];
+    if (MODE == BBP_SART && lambda_dev != nullptr) lambda = *lambda_dev;
     const long long total = off[nprob];
     const int G = gridDim.x;
     const int slot = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
@@ -568,7 +570,7 @@ int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int*
 
 int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* probs,
                              const long long* off, int nprob, int sub, int nsub, int3 bdim,
-                             float lambda, int grid, hipStream_t st) {
+                             float lambda, const float* lambda_dev, int grid, hipStream_t st) {
     const void* fn = batch_backproject_fn(interp, arith, mode);
     size_t lds = brick_lds_bytes(bdim);
     if (!fn || grid <= 0 || (grid & 7) || lds > 63 * 1024) {
@@ -577,7 +579,7 @@ int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* p
         return CT2_EINVAL;
     }
     void* args[] = {(void*)&probs, (void*)&off, (void*)&nprob, (void*)&sub, (void*)&nsub,
-                    (void*)&bdim, (void*)&lambda};
+                    (void*)&bdim, (void*)&lambda, (void*)&lambda_dev};
     CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, st));
     return CT2_OK;
 }

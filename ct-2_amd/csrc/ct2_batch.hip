@@ -135,11 +135,12 @@ int fwd(ct2_batch* b, int mode, int sub, const float* ones, hipStream_t st) {
                                 all ? 1 : b->nsub, ones, b->fgrid, st);
 }
 
-int bp(ct2_batch* b, int mode, int sub, float lambda, hipStream_t st) {
+int bp(ct2_batch* b, int mode, int sub, float lambda, hipStream_t st,
+       const float* lambda_dev = nullptr) {
     const bool all = mode == BBP_PLAIN;
     return launch_batch_backproject(b->interp, b->arith, mode, b->d_probs,
                                     brow(b, all ? b->nsub : sub), b->nprob, all ? 0 : sub,
-                                    all ? 1 : b->nsub, b->bdim, lambda, b->bgrid, st);
+                                    all ? 1 : b->nsub, b->bdim, lambda, lambda_dev, b->bgrid, st);
 }
 
 // time one call of fn on the stream, in ms
@@ -328,18 +329,28 @@ int ct2_batch_backproject(ct2_batch* plan, ct2_stream_t stream) {
     return CT2_OK;
 }
 
-int ct2_batch_sart(ct2_batch* plan, float lambda, int niter, ct2_stream_t stream) {
-    if (!plan) { set_error("ct2_batch_sart: null plan"); return CT2_EINVAL; }
-    if (niter < 0) { set_error("ct2_batch_sart: niter < 0"); return CT2_EINVAL; }
+static int batch_sart(const char* who, ct2_batch* plan, float lambda, const float* lambda_dev,
+                      int niter, ct2_stream_t stream) {
+    if (!plan) { set_error("%s: null plan", who); return CT2_EINVAL; }
+    if (niter < 0) { set_error("%s: niter < 0", who); return CT2_EINVAL; }
     hipStream_t st = (hipStream_t)stream;
     for (int it = 0; it < niter; ++it)
         for (int s = 0; s < plan->nsub; ++s) {
             int rc = fwd(plan, BFWD_RESID, s, nullptr, st);
-            if (!rc) rc = bp(plan, BBP_SART, s, lambda, st);
+            if (!rc) rc = bp(plan, BBP_SART, s, lambda, st, lambda_dev);
             if (rc) return rc;
         }
     CT2_HIP(hipGetLastError());
     return CT2_OK;
+}
+
+int ct2_batch_sart(ct2_batch* plan, float lambda, int niter, ct2_stream_t stream) {
+    return batch_sart("ct2_batch_sart", plan, lambda, nullptr, niter, stream);
+}
+
+int ct2_batch_sart_dev(ct2_batch* plan, const float* lambda, int niter, ct2_stream_t stream) {
+    if (!lambda) { set_error("ct2_batch_sart_dev: null relaxation"); return CT2_EINVAL; }
+    return batch_sart("ct2_batch_sart_dev", plan, 0.0f, lambda, niter, stream);
 }
 
 }  // extern "C"

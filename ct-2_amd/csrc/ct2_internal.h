@@ -378,9 +378,10 @@ int launch_batch_project(int interp, int arith, int tu, int mode, const BatchPro
                          const long long* off, int nprob, int sub, int nsub, const float* ones,
                          int grid, hipStream_t st);
 int batch_project_occupancy(int interp, int arith, int tu, int mode, int* blocks_per_cu);
+// lambda_dev (BBP_SART only, may be null): relaxation read at run time
 int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* probs,
                              const long long* off, int nprob, int sub, int nsub, int3 bdim,
-                             float lambda, int grid, hipStream_t st);
+                             float lambda, const float* lambda_dev, int grid, hipStream_t st);
 int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int* blocks_per_cu);
 size_t brick_lds_bytes(int3 bdim);
 // adjoint brick shapes the autotuners time (single problems and batched plans)

@@ -211,6 +211,7 @@ def read_state(base_dir):
 
 def append_derr(base_dir, it, ddist, tv, ierr):
     """One line of derrEvol.txt (jawPhantomCCB.py:257-261)."""
+    os.makedirs(base_dir, exist_ok=True)
     with open(os.path.join(base_dir, "derrEvol.txt"), "a") as f:
         f.write(f"{int(it)}\t{float(ddist)!r}\t{float(tv)!r}\t{float(ierr)!r}\n")
 

@@ -38,6 +38,7 @@ SIGNATURES = {
     "ct2_batch_project": "PS",
     "ct2_batch_backproject": "PS",
     "ct2_batch_sart": "PfiS",
+    "ct2_batch_sart_dev": "PPiS",
     "ct2_sinoproj_linear": "PPP" + _V + "P" + _G + "S",
     "ct2_sinoproj_nearest": "PPP" + _V + "P" + _G + "S",
     "ct2_sinobackproject_linear": "PPP" + _V + "P" + _G + "S",

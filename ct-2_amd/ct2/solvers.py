@@ -29,7 +29,7 @@ import torch
 
 from . import _dev
 from ._lib import CT2_TV_WORK_DOUBLES, call
-from .autograd import project, total_variation, tv_grad_dev
+from .autograd import project, total_variation, tv_grad_dev, tv_value_dev
 
 
 def distance(a, b=None, mask=None, out=None, work=None):
@@ -212,6 +212,154 @@ def asd_pocs(image, sino, work_sino, max_iter, nart=1, ngradmax=20, gf=0.2, reg_
         if checkpoint_dir is not None:
             _save_checkpoint(checkpoint_dir, it, gf, reg_fac, meps, image)
     return records
+
+
+# ------------------------------------------------ ASD-POCS in a HIP graph --
+def _zero_edges_(x):
+    """Two-voxel border of a device volume set to zero (tomo3D.py:928-943)."""
+    for sl in ((slice(0, 2),), (slice(-2, None),), (slice(None), slice(0, 2)),
+               (slice(None), slice(-2, None)), (slice(None), slice(None), slice(0, 2)),
+               (slice(None), slice(None), slice(-2, None))):
+        x[sl].zero_()
+
+
+class AsdPocs:
+    """ASD-POCS (examples/jawPhantomCCB.py:191-327) on device tensors with the
+    data step done by ordered-subset SART over a one-problem BatchPlan, and
+    every scalar of the adaptive schedule (gf, reg_fac, meps, the distances,
+    the iteration counter) kept on the device, so that one outer iteration
+    contains no host synchronisation and is captured into a HIP graph.
+
+    One iteration: SART sweep over nsub subsets with relaxation reg_fac;
+    zero_edges, positivity; adist = |x - x_pre|; gf *= adist on the first
+    iteration; data distance |A x - b| over used rays and TV (the record the
+    reference writes to derrEvol.txt); ngrad normalised TV-gradient steps of
+    size gf, each followed by zero_edges and positivity; gdist = |x - x_pre|;
+    gf *= red_fact when gdist >= conv*adist and the data distance is above
+    meps (the distance at iteration iter_eps); reg_fac *= red_reg.  The
+    records accumulate in a device log; ``run`` writes them and the run
+    state (the reference's files) between replays and checks the traffic
+    file there."""
+
+    def __init__(self, op, sino, x, nsub=10, ngrad=20, gf=0.2, reg_fac=1.0, red_fact=0.98,
+                 red_reg=0.995, conv=0.99, iter_eps=500, max_iter=1000, truth=None, eps=1e-9):
+        from .batch import BatchPlan
+        if x.dtype != torch.float32 or not x.is_contiguous() or not x.is_cuda:
+            raise ValueError("x must be a contiguous float32 device tensor")
+        self.op, self.b, self.x, self.truth = op, sino, x, truth
+        self.ngrad, self.eps = int(ngrad), float(eps)
+        self.red_fact, self.red_reg, self.conv = float(red_fact), float(red_reg), float(conv)
+        self.iter_eps = int(iter_eps)
+        dev = x.device
+        self.plan = BatchPlan([op], [sino], [x], nsub=nsub)
+        self.pre = torch.empty_like(x)
+        self.g = torch.empty_like(x)
+        self.proj = torch.empty(op.sino_shape, dtype=torch.float32, device=dev)
+        self.bobo = torch.empty((1,), dtype=torch.float32, device=dev)
+        self.rwork = torch.empty(CT2_TV_WORK_DOUBLES, dtype=torch.float64, device=dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.adist, self.gdist = torch.zeros((1,), **f64), torch.zeros((1,), **f64)
+        self.ddist, self.tv = torch.zeros((1,), **f64), torch.zeros((1,), **f64)
+        self.ierr = torch.full((1,), float("nan"), **f64)
+        self.meps = torch.full((1,), -1.0, **f64)
+        self.gf = torch.full((1,), float(gf), dtype=torch.float32, device=dev)
+        self.reg_fac = torch.full((1,), float(reg_fac), dtype=torch.float32, device=dev)
+        self.started = torch.zeros((1,), dtype=torch.float32, device=dev)
+        self.it = torch.zeros((1,), dtype=torch.int64, device=dev)
+        self.log = torch.zeros((int(max_iter) + 1, 4), **f64)
+        self.row = torch.zeros((1, 4), **f64)
+        self._graph = None
+
+    def iterate(self):
+        x = self.x
+        self.it.add_(1)
+        self.pre.copy_(x)
+        call("ct2_batch_sart_dev", self.plan._h, _dev.ptr(self.reg_fac), 1, _dev.stream_handle())
+        _zero_edges_(x)
+        x.clamp_(min=0.0)
+        distance(x, self.pre, out=self.adist, work=self.rwork)
+        self.gf.copy_(torch.where(self.started > 0, self.gf,
+                                  self.gf * self.adist.to(torch.float32)))
+        self.started.fill_(1.0)
+        tv_value_dev(x, 0.0, self.eps, out=self.tv, work=self.rwork)
+        self.op.forward(x, out=self.proj)
+        distance(self.proj, self.b, mask=self.op.ind, out=self.ddist, work=self.rwork)
+        self.meps.copy_(torch.where(self.it == self.iter_eps, self.ddist, self.meps))
+        if self.truth is not None:
+            distance(x, self.truth, out=self.ierr, work=self.rwork)
+        self.row[:, 0].copy_(self.it.to(torch.float64))
+        self.row[:, 1].copy_(self.ddist)
+        self.row[:, 2].copy_(self.tv)
+        self.row[:, 3].copy_(self.ierr)
+        self.log.index_copy_(0, self.it.clamp(max=self.log.shape[0] - 1), self.row)
+        self.pre.copy_(x)
+        for _ in range(self.ngrad):
+            tv_grad_dev(x, 0.0, self.eps, normalizeq=1, out=self.g, bobo=self.bobo,
+                        work=self.rwork)
+            x.sub_(self.gf * self.g)
+            _zero_edges_(x)
+            x.clamp_(min=0.0)
+        distance(x, self.pre, out=self.gdist, work=self.rwork)
+        shrink = (self.gdist >= self.conv * self.adist) & (self.ddist > self.meps)
+        self.gf.mul_(torch.where(shrink, self.red_fact, 1.0).to(torch.float32))
+        self.reg_fac.mul_(self.red_reg)
+
+    def state(self):
+        """(iteration, gf, reg_fac, meps) as host numbers (synchronises)."""
+        return (int(self.it.item()), float(self.gf.item()), float(self.reg_fac.item()),
+                float(self.meps.item()))
+
+    def set_state(self, it, gf, reg_fac, meps):
+        self.it.fill_(int(it))
+        self.gf.fill_(float(gf))
+        self.reg_fac.fill_(float(reg_fac))
+        self.meps.fill_(float(meps))
+        self.started.fill_(1.0 if it > 0 else 0.0)
+
+    def records(self, first, last):
+        """Log rows of iterations first..last (inclusive) as a host array."""
+        return self.log[first:last + 1].cpu().numpy()
+
+    def run(self, n_iter, graph=True, checkpoint_dir=None, save_every=10, traffic_file=None):
+        """n_iter more iterations (replays of one captured iteration after an
+        eager first one).  Every save_every iterations, and at the end, the
+        new log rows are appended to derrEvol.txt and the run state is saved
+        in checkpoint_dir; the traffic file (``stop``) is read there too."""
+        from . import Utils
+        it0 = self.state()[0]
+        done_it = it0
+        stopped = False
+        for k in range(n_iter):
+            if k == 0 or not graph:
+                self.iterate()
+                if graph and self._graph is None and n_iter > 1:
+                    self._graph = capture(self.iterate)
+                    # capture runs nothing; the captured iteration is replayed below
+            else:
+                self._graph.replay()
+            last = k == n_iter - 1
+            if (k + 1) % max(int(save_every), 1) == 0 or last:
+                torch.cuda.synchronize()
+                it = self.state()[0]
+                if checkpoint_dir is not None:
+                    for r in self.records(done_it + 1, it):
+                        Utils.append_derr(checkpoint_dir, int(r[0]), r[1], r[2], r[3])
+                    Utils.write_state(checkpoint_dir, *self.state(), self.x)
+                done_it = it
+                if traffic_file is not None and os.path.exists(traffic_file):
+                    with open(traffic_file) as f:
+                        if f.read().strip() == "stop":
+                            stopped = True
+                            break
+        return self.records(it0 + 1, self.state()[0]), stopped
+
+    def resume(self, checkpoint_dir):
+        """Loads the reference's run-state files into x and the schedule."""
+        from . import Utils
+        it, gf, reg_fac, meps = Utils.read_state(checkpoint_dir)
+        arr = Utils.read_dat(os.path.join(checkpoint_dir, "imageTemp.dat"), tuple(self.x.shape))
+        self.x.copy_(torch.from_numpy(arr).to(self.x.device))
+        self.set_state(it, gf, reg_fac, meps)
 
 
 # ------------------------------------------------ TV-regularised LS (Adam) --

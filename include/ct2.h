@@ -123,6 +123,11 @@ int ct2_batch_backproject(ct2_batch* plan, ct2_stream_t stream);
 /* niter SART iterations (each one pass over the nsub subsets) on every
  * problem's volume in place. */
 int ct2_batch_sart(ct2_batch* plan, float lambda, int niter, ct2_stream_t stream);
+/* ct2_batch_sart with the relaxation read from device memory (one float)
+ * when the kernels run, so a captured graph of the iteration can change it
+ * between replays (the ASD-POCS driver's reg_fac schedule,
+ * examples/jawPhantomCCB.py:325).  No counterpart in the reference. */
+int ct2_batch_sart_dev(ct2_batch* plan, const float* lambda, int niter, ct2_stream_t stream);
 
 /* ------------------------------------------ view_proj3D.f replacements -- */
 /* view_proj3D.f:703 sinoproj_linear */

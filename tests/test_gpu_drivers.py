@@ -147,3 +147,54 @@ def test_fdk_then_em_tv():
     assert r0 == 0.0
     w.mat = op.forward(x)
     assert sino.dist_to(w) < 0.5 * float(torch.linalg.vector_norm(sino.mat.double()))
+
+
+def test_asd_pocs_graph_matches_eager_and_resumes(tmp_path):
+    """The graph-captured ASD-POCS iteration (SART data step, device-side gf /
+    reg_fac / meps schedule) replays like the eager iteration, logs the
+    reference's records, reduces the data distance, and a run cut by a
+    checkpoint and resumed from the reference's state files continues the
+    same trajectory."""
+    from ct2 import Utils
+    from ct2.solvers import AsdPocs
+    op, truth, b = small_problem(n=40, ns=36, nd=64)
+    x0 = torch.zeros(op.shape, device="cuda")
+    kw = dict(nsub=6, ngrad=5, iter_eps=3, max_iter=20, truth=truth)
+    xe = x0.clone()
+    eager = AsdPocs(op, b, xe, **kw)
+    rec_e, _ = eager.run(6, graph=False)
+    xg = x0.clone()
+    gr = AsdPocs(op, b, xg, **kw)
+    rec_g, stopped = gr.run(6, graph=True, checkpoint_dir=str(tmp_path), save_every=2)
+    assert not stopped and rec_g.shape == (6, 4)
+    assert rel_l2(xg.cpu().numpy(), xe.cpu().numpy()) < 1e-5
+    np.testing.assert_allclose(rec_g, rec_e, rtol=1e-5)
+    assert list(rec_g[:, 0]) == [1, 2, 3, 4, 5, 6]
+    assert rec_g[-1, 1] < 0.5 * rec_g[0, 1] and rec_g[-1, 3] < rec_g[0, 3]
+    np.testing.assert_allclose(Utils.read_derr(str(tmp_path / "derrEvol.txt")), rec_g, rtol=1e-12)
+    it, gf, reg_fac, meps = Utils.read_state(str(tmp_path))
+    assert it == 6 and reg_fac == pytest.approx(0.995 ** 6, rel=1e-6)
+    assert meps == pytest.approx(rec_g[2, 1], rel=1e-12)       # distance at iter_eps
+    # cut after 3 iterations, resume from the files, 3 more
+    d2 = tmp_path / "cut"
+    xa = x0.clone()
+    a = AsdPocs(op, b, xa, **kw)
+    a.run(3, graph=True, checkpoint_dir=str(d2), save_every=3)
+    xb = torch.full(op.shape, 7.0, device="cuda")
+    c = AsdPocs(op, b, xb, **kw)
+    c.resume(str(d2))
+    rec_c, _ = c.run(3, graph=True, checkpoint_dir=str(d2), save_every=3)
+    assert list(rec_c[:, 0]) == [4, 5, 6]
+    assert rel_l2(xb.cpu().numpy(), xg.cpu().numpy()) < 1e-5
+    assert Utils.read_derr(str(d2 / "derrEvol.txt")).shape == (6, 4)
+
+
+def test_asd_pocs_graph_traffic_stop(tmp_path):
+    from ct2.solvers import AsdPocs
+    op, truth, b = small_problem(n=32, ns=24, nd=48)
+    x = torch.zeros(op.shape, device="cuda")
+    stop = tmp_path / "traffic.txt"
+    stop.write_text("stop\n")
+    st = AsdPocs(op, b, x, nsub=4, ngrad=3, max_iter=50)
+    rec, stopped = st.run(20, graph=True, save_every=4, traffic_file=str(stop))
+    assert stopped and rec.shape[0] == 4
