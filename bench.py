@@ -73,10 +73,12 @@ def to_storage(vol32, vol16, stochastic, seed=20260107):
     return convert(vol32, vol16.dtype, "stochastic" if stochastic else "nearest", seed, out=vol16)
 
 
-def time_path(op, vol32, vol_store, sino, recon, steps, warmup):
-    """Time `steps` steps of [FP32 -> storage conversion (mixed only), forward,
-    adjoint]; per-kernel durations from HIP events on the stream the kernels
-    run on (torch's current stream)."""
+def time_path(op, vol32, vol_store, sino, data, recon, steps, warmup):
+    """Time `steps` steps of [FP32 -> storage conversion (mixed only), forward
+    of the volume into `sino`, adjoint of the measured data `data` (Poisson-
+    noised, so no ray is zero and the adjoint skips none)]; per-kernel
+    durations from HIP events on the stream the kernels run on (torch's
+    current stream)."""
     import torch
 
     def step(ev=None):
@@ -87,7 +89,7 @@ def time_path(op, vol32, vol_store, sino, recon, steps, warmup):
         if ev: ev[1].record()
         op.forward(v, out=sino)
         if ev: ev[2].record()
-        op.adjoint(sino, out=recon)
+        op.adjoint(data, out=recon)
         if ev: ev[3].record()
 
     for _ in range(warmup):
@@ -103,6 +105,16 @@ def time_path(op, vol32, vol_store, sino, recon, steps, warmup):
     t_fwd = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
     t_bp = sum(e[2].elapsed_time(e[3]) for e in ev) / steps
     return wall, t_cvt, t_fwd, t_bp
+
+
+def noisy_data(op, vol32, photons=1e5, seed=20260102):
+    """Poisson-noised projections p_hat = -ln(max(I,1)/N), I ~ Poisson(N e^-p),
+    of the phantom (attenuation 0.02/mm per unit density)."""
+    import torch
+    p = op.forward(vol32 * 0.02)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    counts = torch.poisson(photons * torch.exp(-p.double()), generator=g)
+    return (-torch.log(torch.clamp(counts, min=1.0) / photons)).float().contiguous()
 
 
 def check_views(ns, count=4):
@@ -489,12 +501,13 @@ def main():
     sino = torch.empty(op_mixed.sino_shape, dtype=torch.float32, device="cuda")
     recon = torch.empty(op_mixed.shape, dtype=torch.float32, device="cuda")
     nupd = 2 * op_mixed.updates_per_direction()
+    data = noisy_data(op_fp32, vol32)
 
     if args.profile_once:
         for _ in range(2):
             to_storage(vol32, vol16, stochastic=True)
             op_mixed.forward(vol16, out=sino)
-            op_mixed.adjoint(sino, out=recon)
+            op_mixed.adjoint(data, out=recon)
         src = torch.zeros(128 * 1024 * 1024, dtype=torch.float32, device="cuda")   # 512 MiB
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
@@ -504,13 +517,13 @@ def main():
         return
 
     log("timing mixed path (FP16 storage with stochastic rounding, fast arithmetic)")
-    wall_m, tc_m, tf_m, tb_m = time_path(op_mixed, vol32, vol16, sino, recon, args.steps,
+    wall_m, tc_m, tf_m, tb_m = time_path(op_mixed, vol32, vol16, sino, data, recon, args.steps,
                                          args.warmup)
     log(f"  convert {tc_m:.3f} ms  fwd {tf_m:.2f} ms  bp {tb_m:.2f} ms  "
         f"wall/step {1e3 * wall_m / args.steps:.2f} ms")
     log("timing FP32 path (FP32 storage, reference operation order)")
     sino32 = torch.empty_like(sino)
-    wall_f, _, tf_f, tb_f = time_path(op_fp32, vol32, None, sino32, recon,
+    wall_f, _, tf_f, tb_f = time_path(op_fp32, vol32, None, sino32, data, recon,
                                       max(1, args.steps // 2), 1)
     log(f"  fwd {tf_f:.2f} ms  bp {tb_f:.2f} ms")
     sub = check_views(NS)
@@ -539,7 +552,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "storage": "f16 volume (forward), f32 sinogram, f32 volume out (adjoint)",
-        "data": "synthetic: seeded 40-ellipsoid phantom, views of a circular cone-beam scan",
+        "data": "synthetic: seeded 40-ellipsoid phantom, views of a circular cone-beam scan; "
+                "the adjoint is applied to its Poisson-noised projections (1e5 photons/ray, "
+                "seed 20260102), where no ray is zero",
         "config": {"workload": "Joseph forward + exact-adjoint backprojection, 256^3 volume "
                                "(0.8 mm), 360 views, 512x512 detector (0.8 mm), R=500 mm, "
                                "SDD=1000 mm",
