@@ -321,6 +321,30 @@ template <> struct Store<hip_bfloat16> {
     }
 };
 
+// Division by a runtime-invariant divisor with one multiply-high (Granlund
+// and Montgomery, "Division by invariant integers using multiplication",
+// PLDI 1994): q = (t + ((n - t) >> s1)) >> s2 with t = umulhi(m, n), exact
+// for every 32-bit n and 1 <= d < 2^32.
+struct Div32 {
+    unsigned m;
+    int s1, s2;
+};
+
+inline Div32 make_div32(unsigned d) {
+    int l = 0;
+    while ((1ull << l) < d) ++l;   // ceil(log2 d)
+    Div32 v;
+    v.m = (unsigned)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    v.s1 = l < 1 ? l : 1;
+    v.s2 = l - 1 > 0 ? l - 1 : 0;
+    return v;
+}
+
+__device__ __forceinline__ unsigned div32(unsigned n, const Div32& v) {
+    const unsigned t = __umulhi(v.m, n);
+    return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
 // ------------------------------------------------------- host utilities ----
 inline Grid make_grid(int nx, int ny, int nz, float dx, float dy, float dz, float x0, float y0,
                       float z0) {

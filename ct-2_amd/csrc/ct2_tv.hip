@@ -21,7 +21,7 @@ namespace ct2 {
 
 constexpr int TV_BLOCKS = 1024;
 static_assert(TV_BLOCKS + 1 <= CT2_TV_WORK_DOUBLES, "TV workspace too small");
-constexpr int TV_THREADS = 256;
+constexpr int TV_THREADS = 1024;
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
 #pragma unroll
@@ -39,7 +39,18 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 struct TvDims {
     int nx, ny, nz;
     float lam, eps;
+    Div32 divz, divy;   // voxel index -> (i, j, k) without 64-bit division
 };
+
+// (i, j, k) of flat index q < 2^31 of a [nx][ny][nz] volume
+__device__ __forceinline__ void voxel_ijk(unsigned q, int ny, int nz, const Div32& divz,
+                                          const Div32& divy, int& i, int& j, int& k) {
+    const unsigned t = div32(q, divz);
+    k = (int)(q - t * (unsigned)nz);
+    const unsigned ii = div32(t, divy);
+    j = (int)(t - ii * (unsigned)ny);
+    i = (int)ii;
+}
 
 __device__ __forceinline__ float tv_grad_at(const float* __restrict__ s, const TvDims& d, int i,
                                             int j, int k) {
@@ -88,9 +99,8 @@ __global__ __launch_bounds__(TV_THREADS) void k_tv_grad(TvDims d, const float* _
     double acc = 0.0;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
-        int k = (int)(q % d.nz);
-        int j = (int)((q / d.nz) % d.ny);
-        int i = (int)(q / ((size_t)d.nz * d.ny));
+        int i, j, k;
+        voxel_ijk((unsigned)q, d.ny, d.nz, d.divz, d.divy, i, j, k);
         float v = tv_grad_at(s, d, i, j, k);
         g[q] = v;
         acc += (double)(v * v);   // Real*4 square accumulated in Real*8 (totalvar.f:580)
@@ -131,9 +141,8 @@ __global__ __launch_bounds__(TV_THREADS) void k_tv_value(TvDims d, const float* 
     double acc = 0.0;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
-        int k = (int)(q % d.nz);
-        int j = (int)((q / d.nz) % d.ny);
-        int i = (int)(q / ((size_t)d.nz * d.ny));
+        int i, j, k;
+        voxel_ijk((unsigned)q, d.ny, d.nz, d.divz, d.divy, i, j, k);
         if (i < 1 || j < 1 || k < 1) continue;
         float c = s[q];
         float ax = c - s[q - sx], ay = c - s[q - sy], az = c - s[q - 1];
@@ -188,6 +197,7 @@ enum { PV_LP = 0, PV_TPV = 1, PV_TPDV = 2 };
 struct PvDims {
     int nx, ny, nz;
     float p, eps, dx, dy, dz;
+    Div32 divz, divy;
 };
 
 // gradient at voxel (i,j,k): lp_grad3d (:150-215), total_p_var_grad3d
@@ -266,9 +276,8 @@ __global__ __launch_bounds__(TV_THREADS) void k_pv_grad(PvDims d, const float* _
     double acc = 0.0;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
-        int k = (int)(q % d.nz);
-        int j = (int)((q / d.nz) % d.ny);
-        int i = (int)(q / ((size_t)d.nz * d.ny));
+        int i, j, k;
+        voxel_ijk((unsigned)q, d.ny, d.nz, d.divz, d.divy, i, j, k);
         float v = pv_grad_at<K>(s, d, i, j, k);
         g[q] = v;
         acc += (double)(v * v);
@@ -292,9 +301,8 @@ __global__ __launch_bounds__(TV_THREADS) void k_pv_value(PvDims d, const float* 
             acc += (double)powf(sqrtf(d.eps + s[q] * s[q]), d.p);
             continue;
         }
-        int k = (int)(q % d.nz);
-        int j = (int)((q / d.nz) % d.ny);
-        int i = (int)(q / ((size_t)d.nz * d.ny));
+        int i, j, k;
+        voxel_ijk((unsigned)q, d.ny, d.nz, d.divz, d.divy, i, j, k);
         if (i < 1 || j < 1 || k < 1) continue;
         float c = s[q];
         float ax = c - s[q - sx], ay = c - s[q - sy], az = c - s[q - 1];
@@ -306,15 +314,14 @@ __global__ __launch_bounds__(TV_THREADS) void k_pv_value(PvDims d, const float* 
 }
 
 // gmi (:4-31): gradient magnitude at voxels with all indices >= 1, 0 elsewhere
-__global__ void k_gmi(int nx, int ny, int nz, float eps, const float* __restrict__ s,
-                      float* __restrict__ out) {
+__global__ void k_gmi(int nx, int ny, int nz, float eps, Div32 divz, Div32 divy,
+                      const float* __restrict__ s, float* __restrict__ out) {
     const size_t n = (size_t)nx * ny * nz;
     const int sx = ny * nz, sy = nz;
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
          q += (size_t)gridDim.x * blockDim.x) {
-        int k = (int)(q % nz);
-        int j = (int)((q / nz) % ny);
-        int i = (int)(q / ((size_t)nz * ny));
+        int i, j, k;
+        voxel_ijk((unsigned)q, ny, nz, divz, divy, i, j, k);
         float v = 0.0f;
         if (i >= 1 && j >= 1 && k >= 1) {
             float c = s[q];
@@ -377,7 +384,7 @@ int ct2_tv_grad(float ambdala, float* tvgmat, const float* smat, int nx, int ny,
     if (rc) return rc;
     if (!tvgmat || !smat || !work) { set_error("ct2_tv_grad: null pointer"); return CT2_EINVAL; }
     hipStream_t st = (hipStream_t)stream;
-    TvDims d{nx, ny, nz, ambdala, eps};
+    TvDims d{nx, ny, nz, ambdala, eps, make_div32((unsigned)nz), make_div32((unsigned)ny)};
     size_t n = (size_t)nx * ny * nz;
     int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
     if (nb > TV_BLOCKS) nb = TV_BLOCKS;
@@ -398,7 +405,7 @@ int ct2_tv_value(float ambdala, const float* smat, int nx, int ny, int nz, float
     if (rc) return rc;
     if (!smat || !tv_out || !work) { set_error("ct2_tv_value: null pointer"); return CT2_EINVAL; }
     hipStream_t st = (hipStream_t)stream;
-    TvDims d{nx, ny, nz, ambdala, eps};
+    TvDims d{nx, ny, nz, ambdala, eps, make_div32((unsigned)nz), make_div32((unsigned)ny)};
     size_t n = (size_t)nx * ny * nz;
     int nb = (int)((n + TV_THREADS - 1) / TV_THREADS);
     if (nb > TV_BLOCKS) nb = TV_BLOCKS;
@@ -460,7 +467,8 @@ int ct2_gmi(float* smatout, const float* smat, int nx, int ny, int nz, float eps
     if (rc) return rc;
     if (!smatout || !smat) { set_error("ct2_gmi: null pointer"); return CT2_EINVAL; }
     hipLaunchKernelGGL(k_gmi, dim3(tv_blocks(nx, ny, nz)), dim3(TV_THREADS), 0,
-                       (hipStream_t)stream, nx, ny, nz, eps, smat, smatout);
+                       (hipStream_t)stream, nx, ny, nz, eps, make_div32((unsigned)nz),
+                       make_div32((unsigned)ny), smat, smatout);
     CT2_HIP(hipGetLastError());
     return CT2_OK;
 }
@@ -470,7 +478,8 @@ int ct2_calc_lp(float p, const float* smat, int nx, int ny, int nz, float eps, d
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!smat || !out || !work) { set_error("ct2_calc_lp: null pointer"); return CT2_EINVAL; }
-    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f, make_div32((unsigned)nz),
+             make_div32((unsigned)ny)};
     return pv_value<PV_LP>(d, smat, 0.0, out, work, (hipStream_t)stream);
 }
 
@@ -479,7 +488,8 @@ int ct2_total_var3d_p(float p, const float* smat, int nx, int ny, int nz, float 
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!smat || !out || !work) { set_error("ct2_total_var3d_p: null pointer"); return CT2_EINVAL; }
-    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f, make_div32((unsigned)nz),
+             make_div32((unsigned)ny)};
     // shift = (nx-1.)*(ny-1.)*(nz-1.)*sqrt(eps)**p, Real*4 (totalvar.f:115)
     float shift = (float)(nx - 1) * (float)(ny - 1) * (float)(nz - 1) * powf(sqrtf(eps), p);
     return pv_value<PV_TPV>(d, smat, (double)shift, out, work, (hipStream_t)stream);
@@ -490,7 +500,8 @@ int ct2_total_dvar3d_p(float p, const float* smat, int nx, int ny, int nz, float
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!smat || !out || !work) { set_error("ct2_total_dvar3d_p: null pointer"); return CT2_EINVAL; }
-    PvDims d{nx, ny, nz, p, eps, dx, dy, dz};
+    PvDims d{nx, ny, nz, p, eps, dx, dy, dz, make_div32((unsigned)nz),
+             make_div32((unsigned)ny)};
     return pv_value<PV_TPDV>(d, smat, 0.0, out, work, (hipStream_t)stream);
 }
 
@@ -499,7 +510,8 @@ int ct2_lp_grad3d(float p, float* tvgmat, const float* smat, int nx, int ny, int
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!tvgmat || !smat || !work) { set_error("ct2_lp_grad3d: null pointer"); return CT2_EINVAL; }
-    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f, make_div32((unsigned)nz),
+             make_div32((unsigned)ny)};
     return pv_grad<PV_LP>(d, tvgmat, smat, normalizeq, bobo, work, (hipStream_t)stream);
 }
 
@@ -509,7 +521,8 @@ int ct2_total_p_var_grad3d(float p, float* tvgmat, const float* smat, int nx, in
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!tvgmat || !smat || !work) { set_error("ct2_total_p_var_grad3d: null pointer"); return CT2_EINVAL; }
-    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f};
+    PvDims d{nx, ny, nz, p, eps, 1.0f, 1.0f, 1.0f, make_div32((unsigned)nz),
+             make_div32((unsigned)ny)};
     return pv_grad<PV_TPV>(d, tvgmat, smat, normalizeq, bobo, work, (hipStream_t)stream);
 }
 
@@ -519,7 +532,8 @@ int ct2_total_p_dvar_grad3d(float p, float* tvgmat, const float* smat, int nx, i
     int rc = tv_check(nx, ny, nz);
     if (rc) return rc;
     if (!tvgmat || !smat || !work) { set_error("ct2_total_p_dvar_grad3d: null pointer"); return CT2_EINVAL; }
-    PvDims d{nx, ny, nz, p, eps, dx, dy, dz};
+    PvDims d{nx, ny, nz, p, eps, dx, dy, dz, make_div32((unsigned)nz),
+             make_div32((unsigned)ny)};
     return pv_grad<PV_TPDV>(d, tvgmat, smat, normalizeq, bobo, work, (hipStream_t)stream);
 }
 
