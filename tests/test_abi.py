@@ -73,3 +73,45 @@ def test_error_path_without_device():
               1.0, 1.0, 1.0, -4.0, -4.0, -4.0)
     assert d.arith == _lib.CT2_ARITH_EXACT and d.interp == _lib.CT2_LINEAR
     assert _lib.lib().ct2_version() >= 100
+
+
+def test_argument_errors_without_gpu():
+    """Invalid sizes and null pointers are rejected with CT2_EINVAL and a
+    message before any HIP call, so this runs without a device; the Python
+    binding raises CT2Error carrying that message."""
+    import ctypes as C
+    from ct2 import _lib
+    from ct2._lib import CT2Error, call, ct2_desc, ct2_stencil
+    EINVAL = -1
+    L = _lib.lib()
+
+    def err(name, *args):
+        rc = getattr(L, name)(*args)
+        return rc, L.ct2_last_error().decode()
+
+    rc, msg = err("ct2_sinoave", None, 0, 4, 4, None, 1, 1, 1, 1, 1, 1, 1, 1, 1, None)
+    assert rc == EINVAL and "positive" in msg
+    rc, msg = err("ct2_sinodist", None, 8, 8, 8, None, 3, 3, 3, 3, 3, 3, 3, 3, 3, None)
+    assert rc == EINVAL and "outside" in msg          # last window ends at 9 > 8
+    rc, msg = err("ct2_sinoscale", None, None, 0, 4, 4, 1.0, 1.0, 0.0, 0.0, None)
+    assert rc == EINVAL and "shape" in msg
+    rc, msg = err("ct2_phantom_project", None, 0, None, 0, None, None, None, 0, 4, 4,
+                  1.0, 1.0, 0.0, 0.0, 0, None)
+    assert rc == EINVAL and "shape" in msg
+    rc, msg = err("ct2_phantom_embed", None, 0, None, 0, None, 4, 4, 4, 1.0, 1.0, 1.0,
+                  0.0, 0.0, 0.0, 0, None)
+    assert rc == EINVAL and "null" in msg
+    rc, msg = err("ct2_total_var_grad3d", 0.0, None, None, 0, 4, 4, 1, 1e-9, None, None)
+    assert rc == EINVAL and "positive" in msg
+    rc, msg = err("ct2_batch_sart_dev", None, None, 1, None)
+    assert rc == EINVAL and "null relaxation" in msg
+    d = ct2_desc()
+    call("ct2_desc_init", C.byref(d), 4, 8, 8, 1.0, 1.0, -4.0, -4.0, 8, 8, 8,
+         1.0, 1.0, 1.0, -4.0, -4.0, -4.0)
+    st = ct2_stencil(1, 0, 1, 1, 1, 1)                 # wu = 0
+    with pytest.raises(CT2Error, match="stencil extents"):
+        call("ct2_stencil_project", C.byref(d), C.byref(st), None, None, None, None, None, None)
+    st = ct2_stencil(1, 2, 1, 1, 1, 1)
+    with pytest.raises(CT2Error, match="null pointer"):
+        call("ct2_stencil_art", C.byref(d), C.byref(st), None, 1.0, None, None, None, None, None,
+             None)
