@@ -166,6 +166,49 @@ __device__ void view_footprint(const Det& dt, const float* __restrict__ f, const
     __syncthreads();
 }
 
+// Footprints of FP_BATCH views at once: thread t projects corner t%8 of view
+// t/8 of the batch, the eight corners of a view reduce by lane shuffles, and
+// s_range[b] receives view b's detector range as view_footprint computes it.
+// Whole workgroup (256 threads); call after a barrier; ends with a barrier.
+constexpr int FP_BATCH = 32;
+__device__ void view_footprints(const Det& dt, const float* __restrict__ fv, int view0, int pv0,
+                                int nb, int vstride, const BrickCtx& c, int (*s_range)[4]) {
+    const int b = threadIdx.x >> 3, k = threadIdx.x & 7;
+    double u = 0, v = 0;
+    int ok = 0;
+    if (b < nb) {
+        const int p = view0 + (pv0 + b) * vstride;
+        double P[3] = {(k & 1) ? c.box_hi[0] : c.box_lo[0], (k & 2) ? c.box_hi[1] : c.box_lo[1],
+                       (k & 4) ? c.box_hi[2] : c.box_lo[2]};
+        project_corner(fv + (size_t)p * 18, P, u, v, ok);
+    }
+    double umin = u, umax = u, vmin = v, vmax = v;
+    int allok = ok;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        umin = fmin(umin, __shfl_xor(umin, off, 64));
+        umax = fmax(umax, __shfl_xor(umax, off, 64));
+        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+        allok &= __shfl_xor(allok, off, 64);
+    }
+    if (b < nb && k == 0) {
+        int jlo = 0, jhi = dt.nu - 1, klo = 0, khi = dt.nv - 1;
+        if (allok) {
+            double jl = (umin - dt.u0) / dt.du - 0.5, jh = (umax - dt.u0) / dt.du - 0.5;
+            double kl = (vmin - dt.v0) / dt.dv - 0.5, kh = (vmax - dt.v0) / dt.dv - 0.5;
+            if (dt.du < 0) { double t = jl; jl = jh; jh = t; }
+            if (dt.dv < 0) { double t = kl; kl = kh; kh = t; }
+            jl = fmax(jl, -2.0); jh = fmin(jh, (double)dt.nu + 1.0);
+            kl = fmax(kl, -2.0); kh = fmin(kh, (double)dt.nv + 1.0);
+            jlo = max(0, (int)floor(jl) - 1); jhi = min(dt.nu - 1, (int)ceil(jh) + 1);
+            klo = max(0, (int)floor(kl) - 1); khi = min(dt.nv - 1, (int)ceil(kh) + 1);
+        }
+        s_range[b][0] = jlo; s_range[b][1] = jhi; s_range[b][2] = klo; s_range[b][3] = khi;
+    }
+    __syncthreads();
+}
+
 // Candidate rays of a footprint, in the order lanes take them: every
 // KSTRIDE-th bin of a detector row (then the next phase), so neighbouring
 // lanes' rays sit about a voxel apart and their four corners mostly hit
@@ -353,27 +396,41 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
     for (int q = threadIdx.x; q < bc.nhalo; q += blockDim.x) acc[q] = 0.0;
     const double cell_lo = (INTERP == CT2_LINEAR) ? -1.0 : 0.0;
 
+    if constexpr (ARITH == CT2_ARITH_FAST) {
+        // footprints a batch of views at a time: between batches no barrier,
+        // so a wavefront moves on to the next view as soon as its own
+        // candidates are done (the LDS accumulation is atomic)
+        __shared__ int s_ranges[FP_BATCH][4];
+        const int lane = threadIdx.x & 63;
+        for (int pv0 = 0; pv0 < nviews; pv0 += FP_BATCH) {
+            const int nb = min(FP_BATCH, nviews - pv0);
+            __syncthreads();   // the previous batch's readers of s_ranges are done
+            view_footprints(dt, fv, view0, pv0, nb, vstride, bc, s_ranges);
+            for (int b = 0; b < nb; ++b) {
+                const int p = view0 + (pv0 + b) * vstride;
+                if (s_ranges[b][1] < s_ranges[b][0] || s_ranges[b][3] < s_ranges[b][2]) continue;
+                const CandSet cs = cand_set(s_ranges[b]);
+                for (int c0 = threadIdx.x - lane; c0 < cs.ncand; c0 += blockDim.x) {
+                    const Cand c = cand_at(cs, c0 + lane, p, dt, ind, sino);
+                    LaneRay L;
+                    lane_ray<INTERP>(g, dt, fv, p, c, zonly, br, bc, L);
+                    int w0, w1;
+                    wave_range(L, w0, w1);
+                    if (w0 > w1) continue;   // wavefront-uniform
+                    lane_march<INTERP>(L, c.datum * L.R.trav, org, w0, w1,
+                                       [&](int a, float v) { lds_add(&acc[a], v); });
+                }
+            }
+        }
+        __syncthreads();
+        return;
+    }
     for (int pv = 0; pv < nviews; ++pv) {
         const int p = view0 + pv * vstride;
         __syncthreads();   // previous view's readers of s_range are done
         view_footprint(dt, fv + (size_t)p * 18, bc, s_range);
         const int jlo = s_range[0], jhi = s_range[1], klo = s_range[2], khi = s_range[3];
         if (jhi < jlo || khi < klo) continue;
-        if constexpr (ARITH == CT2_ARITH_FAST) {
-            const CandSet cs = cand_set(s_range);
-            const int lane = threadIdx.x & 63;
-            for (int c0 = threadIdx.x - lane; c0 < cs.ncand; c0 += blockDim.x) {
-                const Cand c = cand_at(cs, c0 + lane, p, dt, ind, sino);
-                LaneRay L;
-                lane_ray<INTERP>(g, dt, fv, p, c, zonly, br, bc, L);
-                int w0, w1;
-                wave_range(L, w0, w1);
-                if (w0 > w1) continue;   // wavefront-uniform
-                lane_march<INTERP>(L, c.datum * L.R.trav, org, w0, w1,
-                                   [&](int a, float v) { lds_add(&acc[a], v); });
-            }
-            continue;   // next view
-        }
         const int nk = khi - klo + 1;
         const int mk = (nk + KSTRIDE - 1) / KSTRIDE;
         const int row = KSTRIDE * mk;
