@@ -296,15 +296,15 @@ struct LaneRay {
 };
 
 template <int INTERP>
-__device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const float* __restrict__ fv,
-                                         int p, const Cand& c, bool zonly, const Brick& br,
+__device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const ViewFrame& F,
+                                         const Cand& c, bool zonly, const Brick& br,
                                          const BrickCtx& bc, LaneRay& L) {
     L.R = RayFast{};
     L.la0 = L.lsa = L.lsb = L.lsc = L.aoff = 0;
     L.t0 = L.tcnt = L.bl = L.cntb = L.cl = L.cntc = 0;
     if (!c.live) return;
     double Sd[3], Bd[3];
-    view_ray_f64(fv, p, c.j, c.k, dt, Sd, Bd);
+    view_ray_frame(F, c.j, c.k, dt, Sd, Bd);
     setup_fast(g, Sd, Bd, zonly, INTERP == CT2_LINEAR, L.R);
     const RayFast& R = L.R;
     L.la0 = sel3i(R.a, br.lo[0], br.lo[1], br.lo[2]);
@@ -410,10 +410,11 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
                 const int p = view0 + (pv0 + b) * vstride;
                 if (s_ranges[b][1] < s_ranges[b][0] || s_ranges[b][3] < s_ranges[b][2]) continue;
                 const CandSet cs = cand_set(s_ranges[b]);
+                const ViewFrame F = load_frame(fv, p);
                 for (int c0 = threadIdx.x - lane; c0 < cs.ncand; c0 += blockDim.x) {
                     const Cand c = cand_at(cs, c0 + lane, p, dt, ind, sino);
                     LaneRay L;
-                    lane_ray<INTERP>(g, dt, fv, p, c, zonly, br, bc, L);
+                    lane_ray<INTERP>(g, dt, F, c, zonly, br, bc, L);
                     int w0, w1;
                     wave_range(L, w0, w1);
                     if (w0 > w1) continue;   // wavefront-uniform

@@ -247,6 +247,41 @@ __device__ __forceinline__ void view_ray_f64(const float* __restrict__ fv, int p
     }
 }
 
+// The frame entries view_ray_f64 reads (source, detector centre, e_u, e_v),
+// loaded once per view into wavefront-uniform registers; view_ray_frame
+// evaluates the same expressions on them, so the rays are identical.
+struct ViewFrame {
+    float s[3], d[3], eu[3], ev[3];
+};
+
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ ViewFrame load_frame(const float* __restrict__ fv, int p) {
+    const float* f = fv + (size_t)p * 18;
+    ViewFrame F;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        F.s[q] = uniform_f(f[q]);
+        F.d[q] = uniform_f(f[6 + q]);
+        F.eu[q] = uniform_f(f[9 + q]);
+        F.ev[q] = uniform_f(f[12 + q]);
+    }
+    return F;
+}
+
+__device__ __forceinline__ void view_ray_frame(const ViewFrame& F, int j, int k, const Det& dt,
+                                               double S[3], double B[3]) {
+    const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
+    const double v = fma((double)k + 0.5, (double)dt.dv, (double)dt.v0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        S[q] = F.s[q];
+        B[q] = fma((double)F.ev[q], v, fma((double)F.eu[q], u, (double)F.d[q]));
+    }
+}
+
 // Restrict the inclusive plane range [l0,l1] to planes where the cell
 // coordinate u0 + l*du may lie in [lo, hi); the interval is widened by `pad`
 // cells and one plane so FP32 rounding of the exact per-sample test never
