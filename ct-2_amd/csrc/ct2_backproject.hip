@@ -296,16 +296,10 @@ struct LaneRay {
 };
 
 template <int INTERP>
-__device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const ViewFrame& F,
-                                         const Cand& c, bool zonly, const Brick& br,
-                                         const BrickCtx& bc, LaneRay& L) {
-    L.R = RayFast{};
-    L.la0 = L.lsa = L.lsb = L.lsc = L.aoff = 0;
-    L.t0 = L.tcnt = L.bl = L.cntb = L.cl = L.cntc = 0;
-    if (!c.live) return;
-    double Sd[3], Bd[3];
-    view_ray_frame(F, c.j, c.k, dt, Sd, Bd);
-    setup_fast(g, Sd, Bd, zonly, INTERP == CT2_LINEAR, L.R);
+__device__ __forceinline__ void lane_ray_ax(const Grid& g, const double Sd[3], double dd0,
+                                            double dd1, double dd2, int a, const Brick& br,
+                                            const BrickCtx& bc, LaneRay& L) {
+    setup_fast_ax(g, Sd, dd0, dd1, dd2, a, INTERP == CT2_LINEAR, L.R);
     const RayFast& R = L.R;
     L.la0 = sel3i(R.a, br.lo[0], br.lo[1], br.lo[2]);
     const int la1 = L.la0 + sel3i(R.a, br.ext[0], br.ext[1], br.ext[2]);
@@ -334,6 +328,22 @@ __device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const Vie
     L.cntc = max(0, min(hc - 1, loc + locext - 1) - L.cl + 1);
 }
 
+// One lane's candidate ray: the fast set-up (bit-identical to the forward's
+// setup_fast), clipped to the brick.
+template <int INTERP>
+__device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const ViewFrame& F,
+                                         const Cand& c, bool zonly, const Brick& br,
+                                         const BrickCtx& bc, LaneRay& L) {
+    L.R = RayFast{};
+    L.la0 = L.lsa = L.lsb = L.lsc = L.aoff = 0;
+    L.t0 = L.tcnt = L.bl = L.cntb = L.cl = L.cntc = 0;
+    if (!c.live) return;
+    double Sd[3], Bd[3];
+    view_ray_frame(F, c.j, c.k, dt, Sd, Bd);
+    const double dd0 = Bd[0] - Sd[0], dd1 = Bd[1] - Sd[1], dd2 = Bd[2] - Sd[2];
+    lane_ray_ax<INTERP>(g, Sd, dd0, dd1, dd2, fast_axis(g, dd0, dd1, dd2, zonly), br, bc, L);
+}
+
 // Union of the wavefront's plane ranges (every lane takes part).
 __device__ __forceinline__ void wave_range(const LaneRay& L, int& w0, int& w1) {
     w0 = L.tcnt > 0 ? L.t0 : 0x7fffffff;
@@ -343,19 +353,25 @@ __device__ __forceinline__ void wave_range(const LaneRay& L, int& w0, int& w1) {
         w0 = min(w0, __shfl_xor(w0, off, 64));
         w1 = max(w1, __shfl_xor(w1, off, 64));
     }
+    // equal in every lane: scalar registers, so the plane loop is uniform
+    w0 = __builtin_amdgcn_readfirstlane(w0);
+    w1 = __builtin_amdgcn_readfirstlane(w1);
 }
 
 // March the lanes' rays over brick planes [w0, w1] together (same LDS plane
 // per instruction; planes outside a ray's own range are masked) and hand
 // every corner contribution dtr*weight to add(slot, value).  lf is the plane
 // index as a float (exact), as the forward evaluates it.
-template <int INTERP, typename Add>
-__device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, int org, int w0, int w1,
-                                           Add add) {
+template <int INTERP>
+__device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, double* acc, int org,
+                                           int w0, int w1) {
     const RayFast& R = L.R;
     float lf = (float)(L.la0 + w0);
-    int base = org + __mul24(w0, L.lsa) + L.aoff;
-    for (int t = w0; t <= w1; ++t, lf += 1.0f, base += L.lsa) {
+    // byte offsets into acc; every sample's offset is non-negative
+    const int sa8 = L.lsa * 8, sb8 = L.lsb * 8, sc8 = L.lsc * 8, sbc8 = sb8 + sc8;
+    int base = (org + __mul24(w0, L.lsa) + L.aoff) * 8;
+    char* const accb = reinterpret_cast<char*>(acc);
+    for (int t = w0; t <= w1; ++t, lf += 1.0f, base += sa8) {
         const float ub = __builtin_fmaf(lf, R.dub, R.ub0);
         const float uc = __builtin_fmaf(lf, R.duc, R.uc0);
         const float fb = floorf(ub), fc = floorf(uc);
@@ -364,18 +380,20 @@ __device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, int org,
               ((unsigned)(ib - L.bl) < (unsigned)L.cntb) &
               ((unsigned)(ic - L.cl) < (unsigned)L.cntc)))
             continue;
-        const int a00 = base + __mul24(ib, L.lsb) + __mul24(ic, L.lsc);
+        const int a00 = base + __mul24(ib, sb8) + __mul24(ic, sc8);
+        __builtin_assume(a00 >= 0);
+        double* const p = reinterpret_cast<double*>(accb + a00);
         if (INTERP == CT2_LINEAR) {
             const float wb = ub - fb, wc = uc - fc;
             const float tb0 = dtr * (1.0f - wb), tb1 = dtr * wb;
             const float w1c = 1.0f - wc;
             // cells in [lob-1, lob+ext): all four corners have a slot
-            add(a00, tb0 * w1c);
-            add(a00 + L.lsb, tb1 * w1c);
-            add(a00 + L.lsc, tb0 * wc);
-            add(a00 + L.lsb + L.lsc, tb1 * wc);
+            lds_add(p, tb0 * w1c);
+            lds_add(reinterpret_cast<double*>(accb + (a00 + sb8)), tb1 * w1c);
+            lds_add(reinterpret_cast<double*>(accb + (a00 + sc8)), tb0 * wc);
+            lds_add(reinterpret_cast<double*>(accb + (a00 + sbc8)), tb1 * wc);
         } else {
-            add(a00, dtr);
+            lds_add(p, dtr);
         }
     }
 }
@@ -418,8 +436,7 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
                     int w0, w1;
                     wave_range(L, w0, w1);
                     if (w0 > w1) continue;   // wavefront-uniform
-                    lane_march<INTERP>(L, c.datum * L.R.trav, org, w0, w1,
-                                       [&](int a, float v) { lds_add(&acc[a], v); });
+                    lane_march<INTERP>(L, c.datum * L.R.trav, acc, org, w0, w1);
                 }
             }
         }

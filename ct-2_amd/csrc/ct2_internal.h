@@ -181,22 +181,24 @@ __device__ __forceinline__ double rcp_f64(double x) {
     return r * fma(-x, r, 2.0);
 }
 
+// Driving axis of direction dd in the fast set-up: the largest |dd_q| / d_q,
+// compared cross-multiplied (z when the policy forces it).
+__device__ __forceinline__ int fast_axis(const Grid& g, double dd0, double dd1, double dd2,
+                                         bool zonly) {
+    if (zonly) return 2;
+    const float ax = fabsf((float)dd0), ay = fabsf((float)dd1), az = fabsf((float)dd2);
+    const float gx = fabsf(g.d[0]), gy = fabsf(g.d[1]), gz = fabsf(g.d[2]);
+    const bool xy = ax * gy > ay * gx, xz = ax * gz > az * gx, yz = ay * gz > az * gy;
+    return (xy && xz) ? 0 : (yz ? 1 : 2);
+}
+
+// Fast set-up of the ray from S along dd = B - S with driving axis a.
 // FP64 where cancellation needs it (the entry coordinates ub0, uc0: source
 // distances of hundreds of voxels against a sub-voxel result), FP32 elsewhere;
 // no FP64 division or square root.
-__device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], const double B[3],
-                                           bool zonly, bool linear, RayFast& R) {
-    const double dd0 = B[0] - S[0], dd1 = B[1] - S[1], dd2 = B[2] - S[2];
-    int a;
-    if (zonly) {
-        a = 2;
-    } else {
-        // driving axis: largest |dd_q| / d_q, compared cross-multiplied
-        const float ax = fabsf((float)dd0), ay = fabsf((float)dd1), az = fabsf((float)dd2);
-        const float gx = fabsf(g.d[0]), gy = fabsf(g.d[1]), gz = fabsf(g.d[2]);
-        const bool xy = ax * gy > ay * gx, xz = ax * gz > az * gx, yz = ay * gz > az * gy;
-        a = (xy && xz) ? 0 : (yz ? 1 : 2);
-    }
+__device__ __forceinline__ void setup_fast_ax(const Grid& g, const double S[3], double dd0,
+                                              double dd1, double dd2, int a, bool linear,
+                                              RayFast& R) {
     const int b = (a == 0) ? 1 : 0;
     const int c = (a == 2) ? 1 : 2;
     const double dda = (a == 0) ? dd0 : (a == 1 ? dd1 : dd2);
@@ -232,6 +234,12 @@ __device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], con
     R.sa = sel3i(a, g.s[0], g.s[1], g.s[2]);
     R.sb = sel3i(b, g.s[0], g.s[1], g.s[2]);
     R.sc = sel3i(c, g.s[0], g.s[1], g.s[2]);
+}
+
+__device__ __forceinline__ void setup_fast(const Grid& g, const double S[3], const double B[3],
+                                           bool zonly, bool linear, RayFast& R) {
+    const double dd0 = B[0] - S[0], dd1 = B[1] - S[1], dd2 = B[2] - S[2];
+    setup_fast_ax(g, S, dd0, dd1, dd2, fast_axis(g, dd0, dd1, dd2, zonly), linear, R);
 }
 
 // FP64 source and bin of view ray (p,j,k) for the fast setup.
