@@ -180,18 +180,25 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
             l0 = min(l0, __shfl_xor(l0, off, 64));
             l1 = max(l1, __shfl_xor(l1, off, 64));
         }
+        // equal in every lane: scalar registers
+        l0 = __builtin_amdgcn_readfirstlane(l0);
+        l1 = __builtin_amdgcn_readfirstlane(l1);
         // the contiguous-c variant when every used lane's c axis has stride 1
         // (driving axis x or y in the default layout)
         const bool zc = __all(!use || R.sc == 1);
+        // a lane whose driving axis is shorter than the union's end stops at
+        // its own last plane; when no lane needs that the loop stays uniform
+        const bool fits = __all(!use || R.na - 1 >= l1);
         if (use) {
-            l1 = min(l1, R.na - 1);
             const bool small = grid_small<VT>(g);
-            if (zc)
-                out = small ? march_fast<INTERP, VT, true, true>(R, vol, false, l0, l1)
-                            : march_fast<INTERP, VT, false, true>(R, vol, false, l0, l1);
-            else
-                out = small ? march_fast<INTERP, VT, true, false>(R, vol, false, l0, l1)
-                            : march_fast<INTERP, VT, false, false>(R, vol, false, l0, l1);
+            auto march = [&](int m1) {
+                if (zc)
+                    return small ? march_fast<INTERP, VT, true, true>(R, vol, false, l0, m1)
+                                 : march_fast<INTERP, VT, false, true>(R, vol, false, l0, m1);
+                return small ? march_fast<INTERP, VT, true, false>(R, vol, false, l0, m1)
+                             : march_fast<INTERP, VT, false, false>(R, vol, false, l0, m1);
+            };
+            out = fits ? march(l1) : march(min(l1, R.na - 1));
         }
     }
     return out;
