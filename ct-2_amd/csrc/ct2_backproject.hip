@@ -374,8 +374,7 @@ __device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, double* 
     for (int t = w0; t <= w1; ++t, lf += 1.0f, base += sa8) {
         const float ub = __builtin_fmaf(lf, R.dub, R.ub0);
         const float uc = __builtin_fmaf(lf, R.duc, R.uc0);
-        const float fb = floorf(ub), fc = floorf(uc);
-        const int ib = (int)fb, ic = (int)fc;
+        const int ib = floor_i(ub), ic = floor_i(uc);
         if (!(((unsigned)(t - L.t0) < (unsigned)L.tcnt) &
               ((unsigned)(ib - L.bl) < (unsigned)L.cntb) &
               ((unsigned)(ic - L.cl) < (unsigned)L.cntc)))
@@ -384,7 +383,7 @@ __device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, double* 
         __builtin_assume(a00 >= 0);
         double* const p = reinterpret_cast<double*>(accb + a00);
         if (INTERP == CT2_LINEAR) {
-            const float wb = ub - fb, wc = uc - fc;
+            const float wb = frac_pos(ub), wc = frac_pos(uc);
             const float tb0 = dtr * (1.0f - wb), tb1 = dtr * wb;
             const float w1c = 1.0f - wc;
             // cells in [lob-1, lob+ext): all four corners have a slot

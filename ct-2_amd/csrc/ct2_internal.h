@@ -173,6 +173,18 @@ struct RayFast {
     float trav;
 };
 
+// floor(x) converted to int in one instruction (v_cvt_flr_i32_f32), for
+// |x| < 2^31; the compiler emits a floor and a conversion for (int)floorf(x).
+__device__ __forceinline__ int floor_i(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// x - floor(x) in one instruction (v_fract_f32); equal to x - floorf(x) for
+// x >= 0, the only samples that reach a weight.
+__device__ __forceinline__ float frac_pos(float x) { return __builtin_amdgcn_fractf(x); }
+
 // FP64 reciprocal: the FP32 hardware reciprocal (relative error ~1e-7) refined
 // by one Newton-Raphson step r <- r(2 - x r), which squares the relative error
 // (~1e-14), for |x| within the FP32 normal range.

@@ -80,8 +80,7 @@ __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restri
     for (int l = l0; l <= l1; ++l, lf += 1.0f, base += R.sa) {
         float ub = __builtin_fmaf(lf, R.dub, R.ub0);
         float uc = __builtin_fmaf(lf, R.duc, R.uc0);
-        float fb = floorf(ub), fc = floorf(uc);
-        int ib = (int)fb, ic = (int)fc;
+        const int ib = floor_i(ub), ic = floor_i(uc);
         if ((unsigned)ib >= hb || (unsigned)ic >= hc) continue;
         // ZC: the c axis is the contiguous one (stride 1), so the c+1
         // neighbours sit one element further, an immediate load offset
@@ -89,7 +88,7 @@ __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restri
                              : cell_offset<SMALL>(ib, R.sb, ic, R.sc));
         float val;
         if (INTERP == CT2_LINEAR) {
-            float wb = ub - fb, wc = uc - fc;
+            const float wb = frac_pos(ub), wc = frac_pos(uc);
             float vi1, vi2;
             if constexpr (SMALL && std::is_same<VT, __half>::value) {
                 const unsigned o = (unsigned)i00 * 2u;
