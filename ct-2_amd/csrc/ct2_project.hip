@@ -68,19 +68,20 @@ __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restr
 #define CT2_FWD_BATCH 2
 #endif
 
-// march_fast for f16 storage, bilinear, volumes with 32-bit byte offsets:
-// the planes are taken UB at a time and all their loads are issued before
-// the first is used, so a wavefront keeps 4*UB loads in flight instead of 4.
-// A sample outside the volume loads element 0 and leaves the sum alone, so
-// the samples, their order and the compensated sum are exactly those of the
-// one-plane loop.
-template <bool ZC, int UB = CT2_FWD_BATCH>
-__device__ __forceinline__ float march_fast_h2(const RayFast& R, const __half* __restrict__ vol,
-                                               int l0, int l1) {
+// march_fast for bilinear sampling of volumes with 32-bit byte offsets: the
+// planes are taken UB at a time and all their loads are issued before the
+// first is used, so a wavefront keeps 4*UB loads in flight instead of 4.  A
+// sample outside the volume loads element 0 and leaves the sum alone, so the
+// samples, their arithmetic (per storage type, as in march_fast), their order
+// and the compensated sum are exactly those of the one-plane loop.
+template <typename VT, bool ZC, int UB = CT2_FWD_BATCH>
+__device__ __forceinline__ float march_fast_batched(const RayFast& R, const VT* __restrict__ vol,
+                                                    int l0, int l1) {
+    constexpr unsigned E = (unsigned)sizeof(VT);
     float sum = 0.0f, comp = 0.0f;
     const unsigned hb = (unsigned)(R.nb - 1), hc = (unsigned)(R.nc - 1);
-    const unsigned db = (unsigned)R.sb * 2u;
-    const unsigned dc = ZC ? 2u : (unsigned)R.sc * 2u;
+    const unsigned db = (unsigned)R.sb * E;
+    const unsigned dc = ZC ? E : (unsigned)R.sc * E;
     float lf = (float)l0;
     int base = l0 * R.sa;
     for (int l = l0; l <= l1; l += UB, lf += (float)UB, base += UB * R.sa) {
@@ -96,24 +97,33 @@ __device__ __forceinline__ float march_fast_h2(const RayFast& R, const __half* _
             ok[u] = (l + u <= l1) & ((unsigned)ib < hb) & ((unsigned)ic < hc);
             const int i00 = base + u * R.sa +
                             (ZC ? __mul24(ib, R.sb) + ic : cell_offset<true>(ib, R.sb, ic, R.sc));
-            o[u] = ok[u] ? (unsigned)i00 * 2u : 0u;
+            o[u] = ok[u] ? (unsigned)i00 * E : 0u;
         }
-        __half h00[UB], h10[UB], h01[UB], h11[UB];
+        VT x00[UB], x10[UB], x01[UB], x11[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            h00[u] = *at_byte(vol, o[u]);
-            h10[u] = *at_byte(vol, o[u] + db);
-            h01[u] = *at_byte(vol, o[u] + dc);
-            h11[u] = *at_byte(vol, o[u] + db + dc);
+            x00[u] = *at_byte(vol, o[u]);
+            x10[u] = *at_byte(vol, o[u] + db);
+            x01[u] = *at_byte(vol, o[u] + dc);
+            x11[u] = *at_byte(vol, o[u] + db + dc);
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
             const float wb = frac_pos(ub[u]), wc = frac_pos(uc[u]);
-            const float wb1 = 1.0f - wb;
-            const float vi1 = __builtin_fmaf(wb, __half2float(h10[u]),
-                                             __builtin_fmaf(wb1, __half2float(h00[u]), 0.0f));
-            const float vi2 = __builtin_fmaf(wb, __half2float(h11[u]),
-                                             __builtin_fmaf(wb1, __half2float(h01[u]), 0.0f));
+            float vi1, vi2;
+            if constexpr (std::is_same<VT, __half>::value) {
+                // (1-wb) v00 + wb v10 with the half operands in mixed FMAs
+                const float wb1 = 1.0f - wb;
+                vi1 = __builtin_fmaf(wb, __half2float(x10[u]),
+                                     __builtin_fmaf(wb1, __half2float(x00[u]), 0.0f));
+                vi2 = __builtin_fmaf(wb, __half2float(x11[u]),
+                                     __builtin_fmaf(wb1, __half2float(x01[u]), 0.0f));
+            } else {
+                const float v00 = static_cast<float>(x00[u]), v10 = static_cast<float>(x10[u]);
+                const float v01 = static_cast<float>(x01[u]), v11 = static_cast<float>(x11[u]);
+                vi1 = __builtin_fmaf(wb, v10 - v00, v00);
+                vi2 = __builtin_fmaf(wb, v11 - v01, v01);
+            }
             const float val = __builtin_fmaf(wc, vi2 - vi1, vi1);
             const float y = val - comp;
             const float t = sum + y;
@@ -252,10 +262,10 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
         if (use) {
             const bool small = grid_small<VT>(g);
             auto march = [&](int m1) {
-                if constexpr (INTERP == CT2_LINEAR && std::is_same<VT, __half>::value) {
+                if constexpr (INTERP == CT2_LINEAR) {
                     if (small)
-                        return zc ? march_fast_h2<true>(R, vol, l0, m1)
-                                  : march_fast_h2<false>(R, vol, l0, m1);
+                        return zc ? march_fast_batched<VT, true>(R, vol, l0, m1)
+                                  : march_fast_batched<VT, false>(R, vol, l0, m1);
                 }
                 if (zc)
                     return small ? march_fast<INTERP, VT, true, true>(R, vol, false, l0, m1)
