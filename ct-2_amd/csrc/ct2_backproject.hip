@@ -348,14 +348,9 @@ __device__ __forceinline__ void lane_ray(const Grid& g, const Det& dt, const Vie
 __device__ __forceinline__ void wave_range(const LaneRay& L, int& w0, int& w1) {
     w0 = L.tcnt > 0 ? L.t0 : 0x7fffffff;
     w1 = L.tcnt > 0 ? L.t0 + L.tcnt - 1 : -1;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        w0 = min(w0, __shfl_xor(w0, off, 64));
-        w1 = max(w1, __shfl_xor(w1, off, 64));
-    }
-    // equal in every lane: scalar registers, so the plane loop is uniform
-    w0 = __builtin_amdgcn_readfirstlane(w0);
-    w1 = __builtin_amdgcn_readfirstlane(w1);
+    // scalar registers, so the plane loop is uniform
+    w0 = wave_min(w0);
+    w1 = wave_max(w1);
 }
 
 // March the lanes' rays over brick planes [w0, w1] together (same LDS plane

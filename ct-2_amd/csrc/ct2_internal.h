@@ -185,6 +185,26 @@ __device__ __forceinline__ int floor_i(float x) {
 // x >= 0, the only samples that reach a weight.
 __device__ __forceinline__ float frac_pos(float x) { return __builtin_amdgcn_fractf(x); }
 
+// Wavefront-wide min / max of an int, returned in a scalar register, with DPP
+// lane moves (no LDS traffic, so no wait on the wavefront's outstanding LDS
+// operations, unlike a ds_bpermute shuffle): row_shr 1, 2, 4, 8 reduce each
+// 16-lane row into its lane 15, row_bcast 15 and 31 carry the rows' results
+// up to lane 63.  Every lane of the wavefront must be active.
+template <bool MAX>
+__device__ __forceinline__ int wave_minmax(int v) {
+    constexpr int ident = MAX ? (int)0x80000000 : 0x7fffffff;
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(ident, v, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min(int v) { return wave_minmax<false>(v); }
+__device__ __forceinline__ int wave_max(int v) { return wave_minmax<true>(v); }
+
 // FP64 reciprocal: the FP32 hardware reciprocal (relative error ~1e-7) refined
 // by one Newton-Raphson step r <- r(2 - x r), which squares the relative error
 // (~1e-14), for |x| within the FP32 normal range.

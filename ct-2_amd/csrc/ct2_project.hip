@@ -244,15 +244,9 @@ __device__ __forceinline__ float project_lane(const Grid& g, const Det& dt, int 
             clip_planes_f(R.uc0, R.duc, 0.0f, hc, 0.05f, a0, a1);
             if (a0 <= a1) { l0 = a0; l1 = a1; }
         }
-        // union over the wavefront (every lane takes part in the shuffles)
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            l0 = min(l0, __shfl_xor(l0, off, 64));
-            l1 = max(l1, __shfl_xor(l1, off, 64));
-        }
-        // equal in every lane: scalar registers
-        l0 = __builtin_amdgcn_readfirstlane(l0);
-        l1 = __builtin_amdgcn_readfirstlane(l1);
+        // union over the wavefront (every lane takes part), in scalar registers
+        l0 = wave_min(l0);
+        l1 = wave_max(l1);
         // the contiguous-c variant when every used lane's c axis has stride 1
         // (driving axis x or y in the default layout)
         const bool zc = __all(!use || R.sc == 1);
