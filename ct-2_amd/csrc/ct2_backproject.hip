@@ -28,6 +28,10 @@ struct Brick {
     int ext[3];
 };
 
+// Threads per workgroup of the brick kernels (their launches use it); a
+// compile-time stride keeps the candidate walk in scalar registers.
+constexpr int BRICK_THREADS = 256;
+
 // LDS accumulation in FP64: on gfx950 ds_add_f64 runs at ~9 cycles per
 // wavefront instruction per CU while ds_add_f32 takes ~190 (measured), and
 // the wider sum is also more accurate.
@@ -223,12 +227,12 @@ struct CandSet {
     bool small;   // candidate numbers below 2^22: division by reciprocal
 };
 
-__device__ __forceinline__ CandSet cand_set(const int* s_range) {
+__device__ __forceinline__ CandSet cand_set(int r0, int r1, int r2, int r3) {
     CandSet cs;
-    cs.jlo = s_range[0];
-    cs.klo = s_range[2];
-    cs.khi = s_range[3];
-    const int nj = s_range[1] - s_range[0] + 1, nk = s_range[3] - s_range[2] + 1;
+    cs.jlo = r0;
+    cs.klo = r2;
+    cs.khi = r3;
+    const int nj = r1 - r0 + 1, nk = r3 - r2 + 1;
     cs.mk = (nk + KSTRIDE - 1) / KSTRIDE;
     cs.row = KSTRIDE * cs.mk;
     cs.ncand = (nj > 0 && nk > 0) ? nj * cs.row : 0;
@@ -405,7 +409,7 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
     const BrickCtx bc = brick_ctx(g, br);
     const int* ls = bc.ls;
     const int org = bc.org;
-    for (int q = threadIdx.x; q < bc.nhalo; q += blockDim.x) acc[q] = 0.0;
+    for (int q = threadIdx.x; q < bc.nhalo; q += BRICK_THREADS) acc[q] = 0.0;
     const double cell_lo = (INTERP == CT2_LINEAR) ? -1.0 : 0.0;
 
     if constexpr (ARITH == CT2_ARITH_FAST) {
@@ -418,12 +422,24 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
             const int nb = min(FP_BATCH, nviews - pv0);
             __syncthreads();   // the previous batch's readers of s_ranges are done
             view_footprints(dt, fv, view0, pv0, nb, vstride, bc, s_ranges);
+            // lane b holds view b's footprint; each view's is taken from it
+            // with v_readlane, into scalar registers
+            int fr0 = 0, fr1 = -1, fr2 = 0, fr3 = -1;
+            if (lane < nb) {
+                fr0 = s_ranges[lane][0]; fr1 = s_ranges[lane][1];
+                fr2 = s_ranges[lane][2]; fr3 = s_ranges[lane][3];
+            }
             for (int b = 0; b < nb; ++b) {
                 const int p = view0 + (pv0 + b) * vstride;
-                if (s_ranges[b][1] < s_ranges[b][0] || s_ranges[b][3] < s_ranges[b][2]) continue;
-                const CandSet cs = cand_set(s_ranges[b]);
+                const int r0 = __builtin_amdgcn_readlane(fr0, b);
+                const int r1 = __builtin_amdgcn_readlane(fr1, b);
+                const int r2 = __builtin_amdgcn_readlane(fr2, b);
+                const int r3 = __builtin_amdgcn_readlane(fr3, b);
+                if (r1 < r0 || r3 < r2) continue;
+                const CandSet cs = cand_set(r0, r1, r2, r3);
                 const ViewFrame F = load_frame(fv, p);
-                for (int c0 = threadIdx.x - lane; c0 < cs.ncand; c0 += blockDim.x) {
+                const int first = __builtin_amdgcn_readfirstlane((int)threadIdx.x - lane);
+                for (int c0 = first; c0 < cs.ncand; c0 += BRICK_THREADS) {
                     const Cand c = cand_at(cs, c0 + lane, p, dt, ind, sino);
                     LaneRay L;
                     lane_ray<INTERP>(g, dt, F, c, zonly, br, bc, L);
@@ -447,7 +463,7 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
         const int mk = (nk + KSTRIDE - 1) / KSTRIDE;
         const int row = KSTRIDE * mk;
         const int ncand = (jhi - jlo + 1) * row;
-        for (int cidx = threadIdx.x; cidx < ncand; cidx += blockDim.x) {
+        for (int cidx = threadIdx.x; cidx < ncand; cidx += BRICK_THREADS) {
             const int j = jlo + cidx / row;
             const int q = cidx % row;
             const int k = klo + (q % mk) * KSTRIDE + q / mk;
@@ -528,7 +544,7 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
 size_t brick_lds_bytes(int3 b) { return (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double); }
 
 template <int INTERP, int ARITH, typename OT>
-__global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int view0,
+__global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det dt, int view0,
                                                            int nviews, int vstride, int zonly,
                                                            int3 bdim, int3 nbr,
                                                            OT* __restrict__ vol,
@@ -541,7 +557,7 @@ __global__ __launch_bounds__(256) void k_backproject_brick(Grid g, Det dt, int v
                                     ind, fv);
     // write the brick: z-rows are contiguous in both LDS and HBM
     const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
-    for (int q = threadIdx.x; q < nloc; q += blockDim.x) {
+    for (int q = threadIdx.x; q < nloc; q += BRICK_THREADS) {
         int lz = q % br.ext[2];
         int ly = (q / br.ext[2]) % br.ext[1];
         int lx = q / (br.ext[2] * br.ext[1]);
@@ -572,7 +588,7 @@ __device__ __forceinline__ void batch_store(const BatchProb& P, size_t gi, int s
 }
 
 template <int INTERP, int ARITH, int MODE>
-__global__ __launch_bounds__(256) void k_batch_backproject(const BatchProb* __restrict__ probs,
+__global__ __launch_bounds__(BRICK_THREADS) void k_batch_backproject(const BatchProb* __restrict__ probs,
                                                            const long long* __restrict__ off,
                                                            int nprob, int sub, int nsub,
                                                            int3 bdim, float lambda,
@@ -599,7 +615,7 @@ __global__ __launch_bounds__(256) void k_batch_backproject(const BatchProb* __re
         brick_accumulate<INTERP, ARITH>(P.g, P.dt, v0, nv, vstr, P.zonly != 0, br, acc, src,
                                         P.ind, P.fv);
         const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
-        for (int q = threadIdx.x; q < nloc; q += blockDim.x) {
+        for (int q = threadIdx.x; q < nloc; q += BRICK_THREADS) {
             int lz = q % br.ext[2];
             int ly = (q / br.ext[2]) % br.ext[1];
             int lx = q / (br.ext[2] * br.ext[1]);
@@ -649,7 +665,7 @@ int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* p
     }
     void* args[] = {(void*)&probs, (void*)&off, (void*)&nprob, (void*)&sub, (void*)&nsub,
                     (void*)&bdim, (void*)&lambda, (void*)&lambda_dev};
-    CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), args, lds, st));
+    CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(BRICK_THREADS), args, lds, st));
     return CT2_OK;
 }
 
@@ -717,7 +733,7 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
     int3 nb = make_int3((g.n[0] + b.x - 1) / b.x, (g.n[1] + b.y - 1) / b.y,
                         (g.n[2] + b.z - 1) / b.z);
     dim3 grid(nb.x * nb.y * nb.z);
-    hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(256), lds, st, g, dt,
+    hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(BRICK_THREADS), lds, st, g, dt,
                        vs.view0, vs.nviews, vs.vstride, zonly, b, nb, (OT*)vol, sino, ind, fv);
     return CT2_OK;
 }
