@@ -68,6 +68,26 @@ __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restr
 #define CT2_FWD_BATCH 2
 #endif
 
+// Bilinear value of a cell from its corners (x_bc) in the fast arithmetic:
+// for f16 storage (1-wb) v00 + wb v10 with the half operands fed to
+// mixed-precision FMAs (no separate conversions), for f32 and bf16
+// v00 + wb (v10 - v00); then the same along c.
+template <typename VT>
+__device__ __forceinline__ float lerp_cell(VT x00, VT x10, VT x01, VT x11, float wb, float wc) {
+    float vi1, vi2;
+    if constexpr (std::is_same<VT, __half>::value) {
+        const float wb1 = 1.0f - wb;
+        vi1 = __builtin_fmaf(wb, __half2float(x10), __builtin_fmaf(wb1, __half2float(x00), 0.0f));
+        vi2 = __builtin_fmaf(wb, __half2float(x11), __builtin_fmaf(wb1, __half2float(x01), 0.0f));
+    } else {
+        const float v00 = static_cast<float>(x00), v10 = static_cast<float>(x10);
+        const float v01 = static_cast<float>(x01), v11 = static_cast<float>(x11);
+        vi1 = __builtin_fmaf(wb, v10 - v00, v00);
+        vi2 = __builtin_fmaf(wb, v11 - v01, v01);
+    }
+    return __builtin_fmaf(wc, vi2 - vi1, vi1);
+}
+
 // march_fast for bilinear sampling of volumes with 32-bit byte offsets: the
 // planes are taken UB at a time and all their loads are issued before the
 // first is used, so a wavefront keeps 4*UB loads in flight instead of 4.  A
@@ -109,22 +129,8 @@ __device__ __forceinline__ float march_fast_batched(const RayFast& R, const VT* 
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const float wb = frac_pos(ub[u]), wc = frac_pos(uc[u]);
-            float vi1, vi2;
-            if constexpr (std::is_same<VT, __half>::value) {
-                // (1-wb) v00 + wb v10 with the half operands in mixed FMAs
-                const float wb1 = 1.0f - wb;
-                vi1 = __builtin_fmaf(wb, __half2float(x10[u]),
-                                     __builtin_fmaf(wb1, __half2float(x00[u]), 0.0f));
-                vi2 = __builtin_fmaf(wb, __half2float(x11[u]),
-                                     __builtin_fmaf(wb1, __half2float(x01[u]), 0.0f));
-            } else {
-                const float v00 = static_cast<float>(x00[u]), v10 = static_cast<float>(x10[u]);
-                const float v01 = static_cast<float>(x01[u]), v11 = static_cast<float>(x11[u]);
-                vi1 = __builtin_fmaf(wb, v10 - v00, v00);
-                vi2 = __builtin_fmaf(wb, v11 - v01, v01);
-            }
-            const float val = __builtin_fmaf(wc, vi2 - vi1, vi1);
+            const float val = lerp_cell(x00[u], x10[u], x01[u], x11[u], frac_pos(ub[u]),
+                                        frac_pos(uc[u]));
             const float y = val - comp;
             const float t = sum + y;
             const float cn = (t - sum) - y;
@@ -159,45 +165,25 @@ __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restri
                              : cell_offset<SMALL>(ib, R.sb, ic, R.sc));
         float val;
         if (INTERP == CT2_LINEAR) {
-            const float wb = frac_pos(ub), wc = frac_pos(uc);
-            float vi1, vi2;
-            if constexpr (SMALL && std::is_same<VT, __half>::value) {
-                const unsigned o = (unsigned)i00 * 2u;
-                const unsigned db = (unsigned)R.sb * 2u;
-                // four 16-bit loads: written as p[0], p[1] the compiler merges
-                // the c pair into one (often unaligned) dword load, which
-                // measured slower (17.1 ms against 15.7 ms at 256^3 x 360)
-                const unsigned dc = ZC ? 2u : (unsigned)R.sc * 2u;
-                const __half h00 = *at_byte(vol, o), h10 = *at_byte(vol, o + db);
-                const __half h01 = *at_byte(vol, o + dc), h11 = *at_byte(vol, o + db + dc);
-                // (1-wb) v00 + wb v10: the half operands feed mixed-precision
-                // FMAs directly, with no separate conversions
-                const float wb1 = 1.0f - wb;
-                vi1 = __builtin_fmaf(wb, __half2float(h10),
-                                     __builtin_fmaf(wb1, __half2float(h00), 0.0f));
-                vi2 = __builtin_fmaf(wb, __half2float(h11),
-                                     __builtin_fmaf(wb1, __half2float(h01), 0.0f));
+            // (linear sampling reaches this loop only for volumes beyond
+            // 32-bit byte offsets; march_fast_batched covers the rest)
+            const int sc = ZC ? 1 : R.sc;
+            VT x00, x10, x01, x11;
+            if (SMALL) {
+                const unsigned o = (unsigned)i00 * (unsigned)sizeof(VT);
+                const unsigned db = (unsigned)R.sb * (unsigned)sizeof(VT);
+                const unsigned dc = (unsigned)sc * (unsigned)sizeof(VT);
+                x00 = *at_byte(vol, o);
+                x10 = *at_byte(vol, o + db);
+                x01 = *at_byte(vol, o + dc);
+                x11 = *at_byte(vol, o + db + dc);
             } else {
-                float v00, v10, v01, v11;
-                if (SMALL) {
-                    const unsigned o = (unsigned)i00 * (unsigned)sizeof(VT);
-                    const unsigned db = (unsigned)R.sb * (unsigned)sizeof(VT);
-                    const unsigned dc = ZC ? (unsigned)sizeof(VT) : (unsigned)R.sc * (unsigned)sizeof(VT);
-                    v00 = Store<VT>::ldb(vol, o);
-                    v10 = Store<VT>::ldb(vol, o + db);
-                    v01 = Store<VT>::ldb(vol, o + dc);
-                    v11 = Store<VT>::ldb(vol, o + db + dc);
-                } else {
-                    const int sc = ZC ? 1 : R.sc;
-                    v00 = Store<VT>::ld(vol, i00);
-                    v10 = Store<VT>::ld(vol, i00 + R.sb);
-                    v01 = Store<VT>::ld(vol, i00 + sc);
-                    v11 = Store<VT>::ld(vol, i00 + R.sb + sc);
-                }
-                vi1 = __builtin_fmaf(wb, v10 - v00, v00);
-                vi2 = __builtin_fmaf(wb, v11 - v01, v01);
+                x00 = vol[i00];
+                x10 = vol[(size_t)i00 + R.sb];
+                x01 = vol[(size_t)i00 + sc];
+                x11 = vol[(size_t)i00 + R.sb + sc];
             }
-            val = __builtin_fmaf(wc, vi2 - vi1, vi1);
+            val = lerp_cell(x00, x10, x01, x11, frac_pos(ub), frac_pos(uc));
         } else {
             val = Store<VT>::ld(vol, i00);
         }
