@@ -196,6 +196,38 @@ def cpu_leg(fv, geo, grid, vol_np, views=8, gpu_out=None, sub=None):
     return base, prec
 
 
+def cpu_multicore(fv, geo, grid, vol_np, max_threads=16):
+    """The FP32 C restatement of the reference path on all host cores this
+    process may use (at most max_threads): one view per thread, each thread
+    running sinoproj_linear + sinobackproject_linear on its own view (the
+    ctypes calls release the GIL, and the restatement keeps no shared state).
+    The reference Fortran itself is single-threaded and is not run this way."""
+    import concurrent.futures as cf
+    import numpy as np
+    import oracle
+    mods = oracle.restated("f32")
+    ns, nu, nv = geo[:3]
+    threads = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    views = list(range(0, ns, max(1, ns // threads)))[:threads]
+    geo1 = (1,) + tuple(geo[1:])
+
+    def one(p):
+        fvp = np.ascontiguousarray(fv[p:p + 1])
+        s = mods.view_proj3D.sinoproj_linear(None, fvp, *geo1, vol_np, *grid)
+        mods.view_proj3D.sinobackproject_linear(s, None, fvp, *geo1, *grid)
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, views))
+    t1 = time.perf_counter()
+    nupd = 2 * len(views) * nu * nv * N
+    return {"value": round(nupd / (t1 - t0) / 1e9, 6), "unit": "Gvox-ray updates/s",
+            "cores": threads, "kind": "port",
+            "sample": (f"{len(views)} of {ns} views, one per thread ({t1 - t0:.1f} s wall), "
+                       f"{N}^3 volume, {nu}x{nv} detector, FP32 C restatement of the "
+                       f"reference, {os.cpu_count()} CPUs visible")}
+
+
 def read_traffic():
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 counter summary, if present."""
@@ -573,6 +605,7 @@ def main():
         log("CPU leg: reference path on a bounded sample; FP64 oracle check of the GPU outputs")
         base, prec = cpu_leg(fv, geo, grid, vol_np, gpu_out=gpu_out, sub=sub)
         line["cpu_baseline"] = base
+        line["cpu_baseline_multicore"] = cpu_multicore(fv, geo, grid, vol_np)
         line["rel_l2_vs_fp64"] = prec
     print(json.dumps(line), flush=True)
 
