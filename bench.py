@@ -506,8 +506,8 @@ def main():
                          "asd: ASD-POCS iterations (SART + TV) in a HIP graph at 256^3")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding is design-only: views for A, a z-slab reduce-scatter "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the "
+              "multi-GPU sharding is a design only: views for A, a z-slab reduce-scatter "
               "for A^T)", file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -10,7 +10,7 @@
 // come in through scalar loads) and adds the components in table order,
 // which is the order of the reference's Python loop, so the sums are
 // bit-identical.  Each component's value follows the Fortran's FP32
-// expressions operation by operation (no FMA contraction in this build);
+// expressions operation by operation (compiled with -ffp-contract=off);
 // cos/sin of the ellipsoid angles are taken on the host with the C
 // library's cosf/sinf, which the compiled Fortran also calls.
 //
