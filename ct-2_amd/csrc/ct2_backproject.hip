@@ -614,14 +614,14 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_batch_backproject(const Batch
                                                            const long long* __restrict__ off,
                                                            int nprob, int sub, int nsub,
                                                            int3 bdim, float lambda,
-                                                           const float* __restrict__ lambda_dev) {
+                                                           const float* __restrict__ lambda_dev,
+                                                           unsigned long long* queue) {
     extern __shared__ double acc[];
+    __shared__ long long s_it;
     if (MODE == BBP_SART && lambda_dev != nullptr) lambda = *lambda_dev;
     const long long total = off[nprob];
-    const int G = gridDim.x;
-    const int slot = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
-    for (long long base = 0;; base += G) {
-        const long long it = base + slot;
+    for (;;) {
+        const long long it = queue_next(queue, &s_it);
         if (it >= total) break;
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
@@ -647,6 +647,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_batch_backproject(const Batch
         }
         __syncthreads();   // write-back done before the next brick clears acc
     }
+    queue_leave(queue);
 }
 
 template <int INTERP, int ARITH>
@@ -677,7 +678,8 @@ int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int*
 
 int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* probs,
                              const long long* off, int nprob, int sub, int nsub, int3 bdim,
-                             float lambda, const float* lambda_dev, int grid, hipStream_t st) {
+                             float lambda, const float* lambda_dev, unsigned long long* queue,
+                             int grid, hipStream_t st) {
     const void* fn = batch_backproject_fn(interp, arith, mode);
     size_t lds = brick_lds_bytes(bdim);
     if (!fn || grid <= 0 || (grid & 7) || lds > 63 * 1024) {
@@ -686,7 +688,7 @@ int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* p
         return CT2_EINVAL;
     }
     void* args[] = {(void*)&probs, (void*)&off, (void*)&nprob, (void*)&sub, (void*)&nsub,
-                    (void*)&bdim, (void*)&lambda, (void*)&lambda_dev};
+                    (void*)&bdim, (void*)&lambda, (void*)&lambda_dev, (void*)&queue};
     CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(BRICK_THREADS), args, lds, st));
     return CT2_OK;
 }

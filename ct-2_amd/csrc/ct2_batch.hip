@@ -32,6 +32,7 @@ struct ct2_batch {
     float* rowsum = nullptr;
     float* resid = nullptr;
     float* colsum = nullptr;
+    unsigned long long* d_queue = nullptr;   // work queues: [0,1] forward, [2,3] adjoint
     bool has_proj = false, has_adj = false;
     size_t bytes = 0;
 };
@@ -46,6 +47,7 @@ void release(ct2_batch* b) {
     hipFree(b->rowsum);
     hipFree(b->resid);
     hipFree(b->colsum);
+    hipFree(b->d_queue);
     delete b;
 }
 
@@ -132,7 +134,7 @@ int fwd(ct2_batch* b, int mode, int sub, const float* ones, hipStream_t st) {
     const bool all = mode != BFWD_RESID;
     return launch_batch_project(b->interp, b->arith, b->tu, mode, b->d_probs,
                                 frow(b, all ? b->nsub : sub), b->nprob, all ? 0 : sub,
-                                all ? 1 : b->nsub, ones, b->fgrid, st);
+                                all ? 1 : b->nsub, ones, b->d_queue, b->fgrid, st);
 }
 
 int bp(ct2_batch* b, int mode, int sub, float lambda, hipStream_t st,
@@ -140,7 +142,8 @@ int bp(ct2_batch* b, int mode, int sub, float lambda, hipStream_t st,
     const bool all = mode == BBP_PLAIN;
     return launch_batch_backproject(b->interp, b->arith, mode, b->d_probs,
                                     brow(b, all ? b->nsub : sub), b->nprob, all ? 0 : sub,
-                                    all ? 1 : b->nsub, b->bdim, lambda, lambda_dev, b->bgrid, st);
+                                    all ? 1 : b->nsub, b->bdim, lambda, lambda_dev,
+                                    b->d_queue + 2, b->bgrid, st);
 }
 
 // time one call of fn on the stream, in ms
@@ -212,6 +215,11 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
     CT2_HIP(hipMalloc(&b->rowsum, sizeof(float) * nsino_tot));
     CT2_HIP(hipMalloc(&b->resid, sizeof(float) * nsino_tot));
     CT2_HIP(hipMalloc(&b->colsum, sizeof(float) * nvox_tot * b->nsub));
+    CT2_HIP(hipMalloc(&b->d_queue, 4 * sizeof(unsigned long long)));
+    {
+        const unsigned long long zero[4] = {0, 0, 0, 0};
+        if ((rc = upload(b->d_queue, zero, sizeof zero, st))) return rc;
+    }
     b->bytes = sizeof(float) * (2 * nsino_tot + nvox_tot * b->nsub);
     size_t so = 0, vo = 0;
     for (int p = 0; p < b->nprob; ++p) {

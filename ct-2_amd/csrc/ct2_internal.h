@@ -205,6 +205,28 @@ __device__ __forceinline__ int wave_minmax(int v) {
 __device__ __forceinline__ int wave_min(int v) { return wave_minmax<false>(v); }
 __device__ __forceinline__ int wave_max(int v) { return wave_minmax<true>(v); }
 
+// Dynamic work queue of the batched persistent kernels: q[0] is the next
+// item, q[1] counts the workgroups that have left.  Thread 0 of a workgroup
+// takes an item and shares it through LDS; the last workgroup to leave puts
+// the queue back to zero, so every launch (graph replays included) starts
+// from item 0 without a separate reset.  A queue serves one launch at a time.
+__device__ __forceinline__ long long queue_next(unsigned long long* q, long long* s_it) {
+    __syncthreads();   // every thread has read the previous item
+    if (threadIdx.x == 0) *s_it = (long long)atomicAdd(q, 1ull);
+    __syncthreads();
+    return *s_it;
+}
+
+__device__ __forceinline__ void queue_leave(unsigned long long* q) {
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(q + 1, 1ull) == (unsigned long long)gridDim.x - 1) {
+            atomicExch(q, 0ull);
+            atomicExch(q + 1, 0ull);
+        }
+    }
+}
+
 // FP64 reciprocal: the FP32 hardware reciprocal (relative error ~1e-7) refined
 // by one Newton-Raphson step r <- r(2 - x r), which squares the relative error
 // (~1e-14), for |x| within the FP32 normal range.
@@ -475,12 +497,13 @@ inline __host__ __device__ int subset_views(int ns, int sub, int nsub) {
 }
 int launch_batch_project(int interp, int arith, int tu, int mode, const BatchProb* probs,
                          const long long* off, int nprob, int sub, int nsub, const float* ones,
-                         int grid, hipStream_t st);
+                         unsigned long long* queue, int grid, hipStream_t st);
 int batch_project_occupancy(int interp, int arith, int tu, int mode, int* blocks_per_cu);
 // lambda_dev (BBP_SART only, may be null): relaxation read at run time
 int launch_batch_backproject(int interp, int arith, int mode, const BatchProb* probs,
                              const long long* off, int nprob, int sub, int nsub, int3 bdim,
-                             float lambda, const float* lambda_dev, int grid, hipStream_t st);
+                             float lambda, const float* lambda_dev, unsigned long long* queue,
+                             int grid, hipStream_t st);
 int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int* blocks_per_cu);
 size_t brick_lds_bytes(int3 bdim);
 // adjoint brick shapes the autotuners time (single problems and batched plans)

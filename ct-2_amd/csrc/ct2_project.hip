@@ -299,13 +299,13 @@ template <int INTERP, int ARITH, int TU, int MODE>
 __global__ __launch_bounds__(64 * TU) void k_batch_project(const BatchProb* __restrict__ probs,
                                                            const long long* __restrict__ off,
                                                            int nprob, int sub, int nsub,
-                                                           const float* __restrict__ ones) {
+                                                           const float* __restrict__ ones,
+                                                           unsigned long long* queue) {
+    __shared__ long long s_it;
     const long long total = off[nprob];
-    const int G = gridDim.x;
-    const int slot = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (long long base = 0;; base += G) {
-        const long long it = base + slot;
+    for (;;) {
+        const long long it = queue_next(queue, &s_it);
         if (it >= total) break;
         const int pi = batch_find(off, nprob, it);
         const BatchProb& P = probs[pi];
@@ -329,6 +329,7 @@ __global__ __launch_bounds__(64 * TU) void k_batch_project(const BatchProb* __re
             else P.resid[r] = sart_resid(P.sino[r], out, P.rowsum[r], used);
         }
     }
+    queue_leave(queue);
 }
 
 template <int INTERP, int ARITH, int TU, int MODE>
@@ -373,14 +374,14 @@ int batch_project_occupancy(int interp, int arith, int tu, int mode, int* blocks
 
 int launch_batch_project(int interp, int arith, int tu, int mode, const BatchProb* probs,
                          const long long* off, int nprob, int sub, int nsub, const float* ones,
-                         int grid, hipStream_t st) {
+                         unsigned long long* queue, int grid, hipStream_t st) {
     const void* fn = batch_project_fn(interp, arith, tu, mode);
     if (!fn || grid <= 0 || (grid & 7)) {
         set_error("batched projection: bad launch (tile %d, mode %d, grid %d)", tu, mode, grid);
         return CT2_EINVAL;
     }
     void* args[] = {(void*)&probs, (void*)&off, (void*)&nprob, (void*)&sub, (void*)&nsub,
-                    (void*)&ones};
+                    (void*)&ones, (void*)&queue};
     CT2_HIP(hipLaunchKernel(fn, dim3(grid), dim3(64 * tu), args, 0, st));
     return CT2_OK;
 }
