@@ -592,8 +592,8 @@ __global__ void k_zero_strided(Grid g, float* __restrict__ vol) {
 // ---- batched, persistent adjoint ----
 // Work items are the bricks of every problem (off[p] = first brick of
 // problem p, off[nprob] = total; problems with no view in the subset have
-// no items), dealt to resident workgroups in XCD-contiguous chunks as in the
-// batched forward.  Modes: BBP_PLAIN adj = A^T sino over all views;
+// no items), taken by the resident workgroups from the plan's work queue as
+// in the batched forward.  Modes: BBP_PLAIN adj = A^T sino over all views;
 // BBP_COLSUM colsum_sub = A_sub^T resid (resid holds ones); BBP_SART
 // vol = sart_update(vol, lambda, A_sub^T resid, colsum_sub).
 // Final value c of voxel gi of problem P in batched mode MODE.

@@ -281,10 +281,8 @@ __global__ __launch_bounds__(64 * TU) void k_project_views(Grid g, Det dt, int v
 // Work items are detector tiles (TU columns x 64 rows) of every problem's
 // views in subset `sub` (views sub + pv*nsub), numbered problem by problem:
 // off[p] is the first item of problem p and off[nprob] the total.  The grid
-// is a whole number of waves of resident workgroups (a multiple of 8), and
-// each round of items is cut into 8 contiguous chunks, one per XCD
-// (workgroups are dealt to the XCDs round-robin), so neighbouring tiles of
-// one view share an L2.
+// is the number of resident workgroups; each takes its next item from the
+// plan's work queue (queue_next), so the items' uneven costs balance out.
 __device__ __forceinline__ int batch_find(const long long* __restrict__ off, int nprob,
                                           long long it) {
     int lo = 0, hi = nprob - 1;
