@@ -205,6 +205,24 @@ __device__ __forceinline__ int wave_minmax(int v) {
 __device__ __forceinline__ int wave_min(int v) { return wave_minmax<false>(v); }
 __device__ __forceinline__ int wave_max(int v) { return wave_minmax<true>(v); }
 
+// Wavefront-wide FP32 sum by the same DPP pattern (a fixed reduction tree, so
+// the result is deterministic), returned in a scalar register.  Every lane
+// of the wavefront must be active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float x) {
+    const int y = __builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, false);
+    return x + __int_as_float(y);
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+    v = dpp_add<0x111, 0xf>(v);   // row_shr:1
+    v = dpp_add<0x112, 0xf>(v);   // row_shr:2
+    v = dpp_add<0x114, 0xf>(v);   // row_shr:4
+    v = dpp_add<0x118, 0xf>(v);   // row_shr:8
+    v = dpp_add<0x142, 0xa>(v);   // row_bcast:15
+    v = dpp_add<0x143, 0xc>(v);   // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // Dynamic work queue of the batched persistent kernels: q[0] is the next
 // item, q[1] counts the workgroups that have left.  Thread 0 of a workgroup
 // takes an item and shares it through LDS; the last workgroup to leave puts

@@ -26,11 +26,7 @@ namespace ct2 {
 
 constexpr int ART_WAVES = 16;
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ float wave_sum(float v) { return wave_sum_f(v); }
 
 struct ArtGrid2 {
     int present;
