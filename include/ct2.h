@@ -108,7 +108,10 @@ int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const i
  * volumes, and with tune != 0 (unless CT2_AUTOTUNE=0) times forward tile
  * heights {2,4,8} and adjoint bricks to pick the fastest; it synchronises
  * the stream.  The other entry points only launch kernels on the stream
- * (graph-capturable).  Calls on one plan must not overlap. */
+ * (graph-capturable).  Calls on one plan must not overlap: the plan's
+ * kernels hand out work items from a counter in the plan, which each launch
+ * leaves at zero for the next one, so a plan is used on one stream at a
+ * time (and a launch that fails part-way leaves the plan unusable). */
 typedef struct ct2_batch ct2_batch;
 int ct2_batch_create(ct2_batch** plan, int nprob, const ct2_desc* descs,
                      const float* const* frame_vectors, const int8_t* const* indicators,
