@@ -205,8 +205,10 @@ __device__ void view_footprints(const Det& dt, const float* __restrict__ fv, int
             if (dt.dv < 0) { double t = kl; kl = kh; kh = t; }
             jl = fmax(jl, -2.0); jh = fmin(jh, (double)dt.nu + 1.0);
             kl = fmax(kl, -2.0); kh = fmin(kh, (double)dt.nv + 1.0);
-            jlo = max(0, (int)floor(jl) - 1); jhi = min(dt.nu - 1, (int)ceil(jh) + 1);
-            klo = max(0, (int)floor(kl) - 1); khi = min(dt.nv - 1, (int)ceil(kh) + 1);
+            // bins whose centre projects into the box's silhouette bounds,
+            // with 0.05 bin to spare for rounding
+            jlo = max(0, (int)ceil(jl - 0.05)); jhi = min(dt.nu - 1, (int)floor(jh + 0.05));
+            klo = max(0, (int)ceil(kl - 0.05)); khi = min(dt.nv - 1, (int)floor(kh + 0.05));
         }
         s_range[b][0] = jlo; s_range[b][1] = jhi; s_range[b][2] = klo; s_range[b][3] = khi;
     }
