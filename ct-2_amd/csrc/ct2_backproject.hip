@@ -155,15 +155,17 @@ __device__ void view_footprint(const Det& dt, const float* __restrict__ f, const
         }
         int jlo = 0, jhi = dt.nu - 1, klo = 0, khi = dt.nv - 1;
         if (allok) {
-            // bin j has centre u0 + (j+0.5) du; one bin of margin each side
+            // bin j has centre u0 + (j+0.5) du: the bins whose centre projects
+            // into the box's silhouette bounds, with 0.25 bin to spare for
+            // the rounding of the reference arithmetic's sample positions
             double jl = (umin - dt.u0) / dt.du - 0.5, jh = (umax - dt.u0) / dt.du - 0.5;
             double kl = (vmin - dt.v0) / dt.dv - 0.5, kh = (vmax - dt.v0) / dt.dv - 0.5;
             if (dt.du < 0) { double t = jl; jl = jh; jh = t; }
             if (dt.dv < 0) { double t = kl; kl = kh; kh = t; }
             jl = fmax(jl, -2.0); jh = fmin(jh, (double)dt.nu + 1.0);
             kl = fmax(kl, -2.0); kh = fmin(kh, (double)dt.nv + 1.0);
-            jlo = max(0, (int)floor(jl) - 1); jhi = min(dt.nu - 1, (int)ceil(jh) + 1);
-            klo = max(0, (int)floor(kl) - 1); khi = min(dt.nv - 1, (int)ceil(kh) + 1);
+            jlo = max(0, (int)ceil(jl - 0.25)); jhi = min(dt.nu - 1, (int)floor(jh + 0.25));
+            klo = max(0, (int)ceil(kl - 0.25)); khi = min(dt.nv - 1, (int)floor(kh + 0.25));
         }
         s_range[0] = jlo; s_range[1] = jhi; s_range[2] = klo; s_range[3] = khi;
     }
