@@ -225,9 +225,11 @@ __device__ void view_footprints(const Det& dt, const float* __restrict__ fv, int
 // 3 53.7 ms, 4 56.3 ms -- wider strides scatter the sinogram reads of a
 // wavefront over more cache lines.
 constexpr int KSTRIDE = 2;
+// The fast path's candidate numbering for KSTRIDE 2, without dead slots: a
+// row of nk bins is the mk = ceil(nk/2) even offsets, then the odd ones.
 struct CandSet {
     int jlo, klo, khi, row, mk, ncand;
-    float rrow, rmk;
+    float rrow;
     bool small;   // candidate numbers below 2^22: division by reciprocal
 };
 
@@ -237,11 +239,11 @@ __device__ __forceinline__ CandSet cand_set(int r0, int r1, int r2, int r3) {
     cs.klo = r2;
     cs.khi = r3;
     const int nj = r1 - r0 + 1, nk = r3 - r2 + 1;
-    cs.mk = (nk + KSTRIDE - 1) / KSTRIDE;
-    cs.row = KSTRIDE * cs.mk;
+    static_assert(KSTRIDE == 2, "two phases per row");
+    cs.mk = (nk + 1) / 2;
+    cs.row = nk;
     cs.ncand = (nj > 0 && nk > 0) ? nj * cs.row : 0;
     cs.rrow = __builtin_amdgcn_rcpf((float)max(cs.row, 1));
-    cs.rmk = __builtin_amdgcn_rcpf((float)max(cs.mk, 1));
     cs.small = cs.ncand < (1 << 22);
     return cs;
 }
@@ -262,19 +264,16 @@ __device__ __forceinline__ Cand cand_at(const CandSet& cs, int cidx, int p, cons
     c.j = 0; c.k = 0; c.r = 0; c.datum = 0.0f;
     c.live = cidx < cs.ncand;
     if (c.live) {
-        int jj, q, kk;
+        int jj, q;
         if (cs.small) {   // uniform
             jj = div_small(cidx, cs.row, cs.rrow);
             q = cidx - __mul24(jj, cs.row);
-            kk = div_small(q, cs.mk, cs.rmk);
         } else {
             jj = cidx / cs.row;
             q = cidx - jj * cs.row;
-            kk = q / cs.mk;
         }
         c.j = cs.jlo + jj;
-        c.k = cs.klo + (q - kk * cs.mk) * KSTRIDE + kk;
-        c.live = c.k <= cs.khi;
+        c.k = cs.klo + (q < cs.mk ? 2 * q : 2 * (q - cs.mk) + 1);
     }
     if (c.live) {
         // ray index: a 24-bit multiply while both factors fit 24 bits and the
