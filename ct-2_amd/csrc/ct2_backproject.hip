@@ -1,15 +1,17 @@
 // ct2_backproject.hip -- exact adjoint of the Joseph projector, gathered per
-// voxel brick with no global atomics.
+// voxel brick.
 //
 // The reference scatters every ray's (weight, voxel) list into the volume
 // (emil_code/view_proj3D.f:1228-1251, :1355-1360).  Here a workgroup owns a
 // brick of voxels held in LDS.  For every view it projects the brick, grown
-// by one voxel, onto the detector; only rays through that footprint can
-// touch the brick.  Each candidate ray is set up and sampled with the same
-// device functions as the forward kernel, so the matrix entries are the
-// forward's own, and every sample's corner weights that fall inside the
-// brick are added to an FP64 accumulator in LDS.  The brick is written to
-// HBM once.
+// to the cells whose corners touch it, onto the detector; only rays through
+// that footprint can touch the brick.  Each candidate ray is set up and
+// sampled with the same device functions as the forward kernel, so the
+// matrix entries are the forward's own, and every sample's corner weights
+// that fall inside the brick are added to an FP64 accumulator in LDS.  In
+// exact arithmetic the brick is written to HBM once; in fast arithmetic with
+// an f32 output two workgroups share a brick's views and each adds its
+// half-sum into the cleared output with one global atomic per voxel.
 //
 // Linear weights per corner are (trav*wb)*wc and the contribution is
 // datum*weight, the reference's order; only the summation order differs.
@@ -177,6 +179,7 @@ __device__ void view_footprint(const Det& dt, const float* __restrict__ f, const
 // s_range[b] receives view b's detector range as view_footprint computes it.
 // Whole workgroup (256 threads); call after a barrier; ends with a barrier.
 constexpr int FP_BATCH = 32;
+static_assert(FP_BATCH <= 64, "one view per lane of a wavefront");
 __device__ void view_footprints(const Det& dt, const float* __restrict__ fv, int view0, int pv0,
                                 int nb, int vstride, const BrickCtx& c, int (*s_range)[4]) {
     const int b = threadIdx.x >> 3, k = threadIdx.x & 7;
