@@ -424,6 +424,18 @@ template <> struct Store<__half> {
     }
     __device__ __forceinline__ static void st(__half* p, size_t i, float v) { p[i] = __float2half_rn(v); }
 };
+// Element of CT2_F16Z2 storage: the f16 value of a voxel and of its
+// successor along the contiguous axis.  Reads take the voxel's own value;
+// the projector's bilinear loop also uses the successor.
+struct __align__(4) h16x2 {
+    __half lo, hi;
+};
+template <> struct Store<h16x2> {
+    __device__ __forceinline__ static float ld(const h16x2* p, int i) { return __half2float(p[i].lo); }
+    __device__ __forceinline__ static float ldb(const h16x2* p, unsigned off) {
+        return __half2float(at_byte(p, off)->lo);
+    }
+};
 template <> struct Store<hip_bfloat16> {
     __device__ __forceinline__ static float ld(const hip_bfloat16* p, int i) {
         return static_cast<float>(p[i]);

@@ -73,6 +73,16 @@ __device__ __forceinline__ float march_exact(const AxisSel& A, const VT* __restr
 // mixed-precision FMAs (no separate conversions), for f32 and bf16
 // v00 + wb (v10 - v00); then the same along c.
 template <typename VT>
+__device__ __forceinline__ float lerp_cell(VT x00, VT x10, VT x01, VT x11, float wb, float wc);
+
+// f16 pairs: each corner's own value (the corners were loaded one by one)
+template <>
+__device__ __forceinline__ float lerp_cell(h16x2 x00, h16x2 x10, h16x2 x01, h16x2 x11, float wb,
+                                           float wc) {
+    return lerp_cell<__half>(x00.lo, x10.lo, x01.lo, x11.lo, wb, wc);
+}
+
+template <typename VT>
 __device__ __forceinline__ float lerp_cell(VT x00, VT x10, VT x01, VT x11, float wb, float wc) {
     float vi1, vi2;
     if constexpr (std::is_same<VT, __half>::value) {
@@ -119,13 +129,29 @@ __device__ __forceinline__ float march_fast_batched(const RayFast& R, const VT* 
                             (ZC ? __mul24(ib, R.sb) + ic : cell_offset<true>(ib, R.sb, ic, R.sc));
             o[u] = ok[u] ? (unsigned)i00 * E : 0u;
         }
-        VT x00[UB], x10[UB], x01[UB], x11[UB];
+        // f16 pairs: along a contiguous c axis one aligned 4-byte load gives
+        // a corner and its c+1 neighbour
+        using XT = typename std::conditional<std::is_same<VT, h16x2>::value, __half, VT>::type;
+        XT x00[UB], x10[UB], x01[UB], x11[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            x00[u] = *at_byte(vol, o[u]);
-            x10[u] = *at_byte(vol, o[u] + db);
-            x01[u] = *at_byte(vol, o[u] + dc);
-            x11[u] = *at_byte(vol, o[u] + db + dc);
+            if constexpr (std::is_same<VT, h16x2>::value) {
+                const h16x2 p0 = *at_byte(vol, o[u]), p1 = *at_byte(vol, o[u] + db);
+                x00[u] = p0.lo;
+                x10[u] = p1.lo;
+                if constexpr (ZC) {
+                    x01[u] = p0.hi;
+                    x11[u] = p1.hi;
+                } else {
+                    x01[u] = at_byte(vol, o[u] + dc)->lo;
+                    x11[u] = at_byte(vol, o[u] + db + dc)->lo;
+                }
+            } else {
+                x00[u] = *at_byte(vol, o[u]);
+                x10[u] = *at_byte(vol, o[u] + db);
+                x01[u] = *at_byte(vol, o[u] + dc);
+                x11[u] = *at_byte(vol, o[u] + db + dc);
+            }
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
@@ -438,6 +464,7 @@ static int dispatch_dtype(const ct2_desc* d, const Grid& g, const Det& dt, const
         case CT2_F32: launch_views<INTERP, ARITH, float>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
+        case CT2_F16Z2: launch_views<INTERP, ARITH, h16x2>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         default: set_error("ct2_project: unknown vol_dtype %d", d->vol_dtype); return CT2_EINVAL;
     }
     return CT2_OK;

@@ -41,36 +41,58 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
     return x;
 }
 
+// The 16-bit pattern that ct2_convert_volume stores for element i of value
+// x: round to nearest, or stochastically between the two neighbours with a
+// counter-based hash of (i, seed).
+template <typename OT>
+__device__ __forceinline__ uint16_t round16(float x, size_t i, int stochastic, uint32_t seed) {
+    if (!stochastic) {
+        OT v;
+        Store<OT>::st(&v, 0, x);
+        return *reinterpret_cast<const uint16_t*>(&v);
+    }
+    uint16_t lo_bits;
+    float lo;
+    if constexpr (std::is_same<OT, __half>::value) {
+        __half h = __float2half_rz(x);
+        lo_bits = __half_as_ushort(h);
+        lo = __half2float(h);
+    } else {
+        uint32_t b = __float_as_uint(x);
+        lo_bits = (uint16_t)(b >> 16);           // truncation = toward zero
+        lo = __uint_as_float((uint32_t)lo_bits << 16);
+    }
+    uint16_t out = lo_bits;
+    if (lo != x && isfinite(x)) {
+        const uint16_t hi_bits = (uint16_t)(lo_bits + 1);   // next magnitude up
+        float hi;
+        if constexpr (std::is_same<OT, __half>::value) hi = __half2float(__ushort_as_half(hi_bits));
+        else hi = __uint_as_float((uint32_t)hi_bits << 16);
+        const float p = (fabsf(x) - fabsf(lo)) / (fabsf(hi) - fabsf(lo));
+        const float u = (float)(hash32((uint32_t)i ^ hash32(seed + (uint32_t)(i >> 32))) >> 8) *
+                        (1.0f / 16777216.0f);
+        if (u < p && isfinite(hi)) out = hi_bits;
+    }
+    return out;
+}
+
 template <typename OT>
 __global__ void k_convert(const float* __restrict__ src, OT* __restrict__ dst, size_t n,
                           int stochastic, uint32_t seed) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        reinterpret_cast<uint16_t*>(dst)[i] = round16<OT>(src[i], i, stochastic, seed);
+}
+
+// f16 pairs: element i holds the f16 patterns of values i and i + 1 (zero
+// past the end), each rounded exactly as k_convert<__half> rounds it.
+__global__ void k_convert_pairs(const float* __restrict__ src, uint32_t* __restrict__ dst,
+                                size_t n, int stochastic, uint32_t seed) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        const float x = src[i];
-        if (!stochastic) { Store<OT>::st(dst, i, x); continue; }
-        uint16_t lo_bits;
-        float lo;
-        if constexpr (std::is_same<OT, __half>::value) {
-            __half h = __float2half_rz(x);
-            lo_bits = __half_as_ushort(h);
-            lo = __half2float(h);
-        } else {
-            uint32_t b = __float_as_uint(x);
-            lo_bits = (uint16_t)(b >> 16);           // truncation = toward zero
-            lo = __uint_as_float((uint32_t)lo_bits << 16);
-        }
-        uint16_t out = lo_bits;
-        if (lo != x && isfinite(x)) {
-            const uint16_t hi_bits = (uint16_t)(lo_bits + 1);   // next magnitude up
-            float hi;
-            if constexpr (std::is_same<OT, __half>::value) hi = __half2float(__ushort_as_half(hi_bits));
-            else hi = __uint_as_float((uint32_t)hi_bits << 16);
-            const float p = (fabsf(x) - fabsf(lo)) / (fabsf(hi) - fabsf(lo));
-            const float u = (float)(hash32((uint32_t)i ^ hash32(seed + (uint32_t)(i >> 32))) >> 8) *
-                            (1.0f / 16777216.0f);
-            if (u < p && isfinite(hi)) out = hi_bits;
-        }
-        reinterpret_cast<uint16_t*>(dst)[i] = out;
+        const uint32_t lo = round16<__half>(src[i], i, stochastic, seed);
+        const uint32_t hi = i + 1 < n ? round16<__half>(src[i + 1], i + 1, stochastic, seed) : 0u;
+        dst[i] = lo | (hi << 16);
     }
 }
 
@@ -167,8 +189,12 @@ int launch_convert(const float* src, void* dst, int dtype, size_t n, int roundin
             hipLaunchKernelGGL(k_convert<hip_bfloat16>, dim3((unsigned)blocks), dim3(256), 0, st,
                                src, (hip_bfloat16*)dst, n, s, (uint32_t)seed);
             break;
+        case CT2_F16Z2:
+            hipLaunchKernelGGL(k_convert_pairs, dim3((unsigned)blocks), dim3(256), 0, st, src,
+                               (uint32_t*)dst, n, s, (uint32_t)seed);
+            break;
         default:
-            set_error("ct2_convert_volume: destination must be CT2_F16 or CT2_BF16");
+            set_error("ct2_convert_volume: destination must be CT2_F16, CT2_BF16 or CT2_F16Z2");
             return CT2_EINVAL;
     }
     CT2_HIP(hipGetLastError());

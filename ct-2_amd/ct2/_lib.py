@@ -16,7 +16,7 @@ LIBPATH = os.path.join(_HERE, "libct2.so")
 
 CT2_LINEAR, CT2_NEAREST = 0, 1
 CT2_AXIS_FREE, CT2_AXIS_Z = 0, 1
-CT2_F32, CT2_F16, CT2_BF16 = 0, 1, 2
+CT2_F32, CT2_F16, CT2_BF16, CT2_F16Z2 = 0, 1, 2, 3
 CT2_ARITH_EXACT, CT2_ARITH_FAST = 0, 1
 CT2_ROUND_NEAREST, CT2_ROUND_STOCHASTIC = 0, 1
 CT2_TV_WORK_DOUBLES = 1040

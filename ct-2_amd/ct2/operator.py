@@ -19,22 +19,31 @@ import torch
 
 from . import _dev
 from ._lib import (CT2_ARITH_EXACT, CT2_ARITH_FAST, CT2_AXIS_FREE, CT2_AXIS_Z, CT2_BF16,
-                   CT2_F16, CT2_F32, CT2_LINEAR, CT2_NEAREST, CT2_ROUND_NEAREST,
+                   CT2_F16, CT2_F16Z2, CT2_F32, CT2_LINEAR, CT2_NEAREST, CT2_ROUND_NEAREST,
                    CT2_ROUND_STOCHASTIC, call, ct2_desc)
 
 _DT = {torch.float32: CT2_F32, torch.float16: CT2_F16, torch.bfloat16: CT2_BF16}
 
 
-def to_storage(vol32, dtype=torch.float16, rounding="stochastic", seed=20260107, out=None):
+def to_storage(vol32, dtype=torch.float16, rounding="stochastic", seed=20260107, out=None,
+               pairs=False):
     """float32 volume -> 16-bit storage (float16 or bfloat16) by the library's
     conversion kernel; ``rounding`` is "stochastic" (unbiased, deterministic
-    in ``seed``) or "nearest"."""
+    in ``seed``) or "nearest".  ``pairs`` (float16 only) gives the f16 pair
+    storage read by the fast forward projector: a (*shape, 2) tensor whose
+    [..., 0] is the volume as float16 storage rounds it and [..., 1] the next
+    element along the last axis (zero after the last element)."""
     if vol32.dtype != torch.float32 or not vol32.is_contiguous():
         raise ValueError("source must be a contiguous float32 tensor")
     if dtype not in (torch.float16, torch.bfloat16):
         raise ValueError("storage dtype must be float16 or bfloat16")
-    out = torch.empty(vol32.shape, dtype=dtype, device=vol32.device) if out is None else out
-    call("ct2_convert_volume", _dev.ptr(vol32), _dev.ptr(out), _DT[dtype],
+    if pairs and dtype != torch.float16:
+        raise ValueError("pair storage is float16")
+    shape = tuple(vol32.shape) + ((2,) if pairs else ())
+    out = torch.empty(shape, dtype=dtype, device=vol32.device) if out is None else out
+    if tuple(out.shape) != shape or out.dtype != dtype or not out.is_contiguous():
+        raise ValueError(f"output must be a contiguous {dtype} tensor of shape {shape}")
+    call("ct2_convert_volume", _dev.ptr(vol32), _dev.ptr(out), CT2_F16Z2 if pairs else _DT[dtype],
          C.c_size_t(vol32.numel()),
          {"stochastic": CT2_ROUND_STOCHASTIC, "nearest": CT2_ROUND_NEAREST}[rounding],
          int(seed), _dev.stream_handle())
@@ -71,21 +80,27 @@ class ConeBeamOperator:
 
     def _desc(self, dtype=torch.float32, view0=0, nviews=0, vstride=1):
         d = ct2_desc.from_buffer_copy(self.d)
-        d.vol_dtype = _DT[dtype]
+        d.vol_dtype = dtype if isinstance(dtype, int) else _DT[dtype]
         d.view0 = view0
         d.nviews = nviews
         d.vstride = vstride
         return d
 
     def forward(self, vol, out=None, view0=0, nviews=0, vstride=1):
-        """A vol -> sinogram (float32).  vol: (nx,ny,nz) float32/float16/bfloat16.
-        With a view subset (views view0 + j*vstride, j < nviews; nviews 0 =
-        all such views) only those views of ``out`` are written."""
-        if vol.shape != self.shape or not vol.is_contiguous() or vol.dtype not in _DT:
-            raise ValueError(f"volume must be a contiguous {self.shape} float32/16/bf16 tensor")
+        """A vol -> sinogram (float32).  vol: (nx,ny,nz) float32/float16/bfloat16,
+        or (nx,ny,nz,2) float16 pair storage (``to_storage(..., pairs=True)``),
+        which projects exactly as its float16 values do.  With a view subset
+        (views view0 + j*vstride, j < nviews; nviews 0 = all such views) only
+        those views of ``out`` are written."""
+        pairs = tuple(vol.shape) == self.shape + (2,) and vol.dtype == torch.float16
+        if not pairs and (vol.shape != self.shape or vol.dtype not in _DT):
+            raise ValueError(f"volume must be a {self.shape} float32/16/bf16 tensor or its "
+                             f"{self.shape + (2,)} float16 pair storage")
+        if not vol.is_contiguous():
+            raise ValueError("volume must be contiguous")
         if out is None:
             out = torch.empty(self.sino_shape, dtype=torch.float32, device=vol.device)
-        d = self._desc(vol.dtype, view0, nviews, vstride)
+        d = self._desc(CT2_F16Z2 if pairs else vol.dtype, view0, nviews, vstride)
         call("ct2_project", C.byref(d), _dev.ptr(out), _dev.ptr(self.ind), _dev.ptr(self.fv),
              _dev.ptr(vol), _dev.stream_handle())
         return out

@@ -36,7 +36,7 @@ typedef void* ct2_stream_t;
 enum { CT2_OK = 0, CT2_EINVAL = -1, CT2_EHIP = -2, CT2_ENOMEM = -3 };
 enum { CT2_LINEAR = 0, CT2_NEAREST = 1 };
 enum { CT2_AXIS_FREE = 0, CT2_AXIS_Z = 1 };
-enum { CT2_F32 = 0, CT2_F16 = 1, CT2_BF16 = 2 };
+enum { CT2_F32 = 0, CT2_F16 = 1, CT2_BF16 = 2, CT2_F16Z2 = 3 };
 /* CT2_ARITH_EXACT: the reference's FP32 operation order, correctly rounded
  * divisions, no FMA contraction -- bit-identical to the reference Fortran
  * for the forward projector.  CT2_ARITH_FAST: per-ray setup in FP64, FMA
@@ -81,7 +81,13 @@ int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const in
  * mixed-precision forward projection.  CT2_ROUND_STOCHASTIC picks the upper
  * or lower 16-bit neighbour with probability proportional to proximity,
  * driven by a counter-based hash of (index, seed), so the stored volume is
- * unbiased and deterministic.  No counterpart in the reference (all REAL*4). */
+ * unbiased and deterministic.  CT2_F16Z2 stores f16 pairs: element i (4
+ * bytes) holds the f16 values of elements i and i+1 (zero past the end),
+ * rounded as CT2_F16 rounds them, so that the fast forward projector reads a
+ * voxel and its neighbour along the contiguous axis with one aligned load;
+ * ct2_project takes it with desc->vol_dtype = CT2_F16Z2 and returns the
+ * same projections as from the CT2_F16 volume.  No counterpart in the
+ * reference (all REAL*4). */
 int ct2_convert_volume(const float* src, void* dst, int dtype, size_t n, int rounding,
                        unsigned seed, ct2_stream_t stream);
 /* SART update over the view subset of desc (views view0 + j*vstride; no

@@ -201,3 +201,38 @@ def test_fast_adjoint_vs_fp64(o64, brick):
     ref = o64.view_proj3D.sinobackproject_linear(y, ind, fv, *geo, *g2)
     err = rel_l2(out.cpu().numpy(), ref)
     assert err <= 2e-5, err
+
+
+@pytest.mark.parametrize("rounding", ["stochastic", "nearest"])
+@pytest.mark.parametrize("interp,case", [("linear", "cone"), ("nearest", "cone"),
+                                         ("linear", "tomo")])
+def test_f16_pair_storage_projects_as_f16(rounding, interp, case):
+    """The f16 pair storage (each value with its successor along z) holds the
+    float16 conversion's values, and the fast forward projects it bit for bit
+    as it projects the float16 volume: along z (cone beam, x- or y-driven
+    rays) with the paired loads, across z (z-driven tomosynthesis) with the
+    values alone."""
+    import torch
+    from ct2.operator import ConeBeamOperator, to_storage
+    if case == "cone":
+        ind, fv, geo, vol, g2 = cone_case(n=40, ns=8, nu=60, nv=52, du=1.0, dv=1.0)
+        axis = "free"
+    else:
+        ind, fv, geo, vol, g2 = tomo_case()
+        axis = "z"
+    ns, nu, nv, du, dv, u0, v0 = geo
+    dx, dy, dz, x0, y0, z0, nx, ny, nz = g2
+    op = ConeBeamOperator(fv, ns, nu, nv, du, dv, u0, v0, (nx, ny, nz), (dx, dy, dz),
+                          (x0, y0, z0), interp=interp, axis=axis, arith="fast", indicator=ind)
+    v32 = torch.from_numpy(vol).cuda()
+    h = to_storage(v32, torch.float16, rounding, seed=7)
+    hp = to_storage(v32, torch.float16, rounding, seed=7, pairs=True)
+    assert hp.shape == tuple(vol.shape) + (2,)
+    flat, pf = h.reshape(-1), hp.reshape(-1, 2)
+    assert torch.equal(pf[:, 0].view(torch.int16), flat.view(torch.int16))
+    assert torch.equal(pf[:-1, 1].view(torch.int16), flat[1:].view(torch.int16))
+    assert int(pf[-1, 1].view(torch.int16)) == 0
+    a = op.forward(h).cpu().numpy()
+    b = op.forward(hp).cpu().numpy()
+    assert np.abs(a).max() > 0.5
+    assert np.array_equal(a, b), np.abs(a - b).max()
