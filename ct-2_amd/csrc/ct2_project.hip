@@ -532,28 +532,34 @@ static int tuned_project(const ct2_desc* d, const Grid& g, const Det& dt, const 
     if (cs != hipStreamCaptureStatusNone || (env && env[0] == '0'))
         return launch_tu(d, g, dt, vs, 4, s, ind, fv, vol, st);
     // one untimed launch first, so that no candidate is timed with a cold
-    // volume in the caches
+    // volume in the caches; then two rounds over the candidates, each scored
+    // by its faster launch, so that one noisy launch does not decide the pick
     const int cand[4] = {4, 2, 8, 1};
-    hipEvent_t ev[8];
-    for (int c = 0; c < 8; ++c) CT2_HIP(hipEventCreate(&ev[c]));
+    constexpr int NROUND = 2, NEV = 2 * 4 * NROUND;
+    hipEvent_t ev[NEV];
+    for (int c = 0; c < NEV; ++c) CT2_HIP(hipEventCreate(&ev[c]));
     int rc = launch_tu(d, g, dt, vs, cand[0], s, ind, fv, vol, st);
-    for (int c = 0; c < 4 && rc == CT2_OK; ++c) {
-        hipEventRecord(ev[2 * c], st);
-        rc = launch_tu(d, g, dt, vs, cand[c], s, ind, fv, vol, st);
-        hipEventRecord(ev[2 * c + 1], st);
+    for (int e = 0; e < 4 * NROUND && rc == CT2_OK; ++e) {
+        hipEventRecord(ev[2 * e], st);
+        rc = launch_tu(d, g, dt, vs, cand[e % 4], s, ind, fv, vol, st);
+        hipEventRecord(ev[2 * e + 1], st);
     }
     int best = 4;
     if (rc == CT2_OK) {
-        hipError_t he = hipEventSynchronize(ev[7]);
+        hipError_t he = hipEventSynchronize(ev[NEV - 1]);
         if (he != hipSuccess) rc = hip_fail(he, "tile autotune");
         float tbest = 1e30f;
         for (int c = 0; c < 4 && rc == CT2_OK; ++c) {
-            float ms = 0;
-            hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]);
-            if (ms < tbest) { tbest = ms; best = cand[c]; }
+            float tc = 1e30f;
+            for (int r = 0; r < NROUND; ++r) {
+                float ms = 0;
+                hipEventElapsedTime(&ms, ev[2 * (r * 4 + c)], ev[2 * (r * 4 + c) + 1]);
+                tc = fminf(tc, ms);
+            }
+            if (tc < tbest) { tbest = tc; best = cand[c]; }
         }
     }
-    for (int c = 0; c < 8; ++c) hipEventDestroy(ev[c]);
+    for (int c = 0; c < NEV; ++c) hipEventDestroy(ev[c]);
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tile_mu);
