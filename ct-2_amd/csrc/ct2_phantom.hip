@@ -131,6 +131,22 @@ __device__ __forceinline__ void to_frame(const float* u, float x, float y, float
     o[2] = u[6] * xt + u[7] * yt + u[8] * zt;
 }
 
+// A point in a surface's frame for the voxel value.  Unit-vector frames as
+// to_frame; angle frames (CT2_SURF_ANGLES) in the operation order of framer
+// (tomo3DfortranAux.f:281-308), u = cos/sin of alpha, beta, gamma, then the
+// centre in u[9..11].
+__device__ __forceinline__ void surf_frame(const QSurf& sf, float x, float y, float z, float* o) {
+    if (!(sf.kind & CT2_SURF_ANGLES)) { to_frame(sf.u, x, y, z, o); return; }
+    const float* u = sf.u;
+    const float xp = (x - u[9]) * u[0] - (u[10] - y) * u[1];
+    const float yp = (y - u[10]) * u[0] + (u[9] - x) * u[1];
+    const float zp = z - u[11];
+    const float xpp = xp * u[2] - zp * u[3], ypp = yp, zpp = xp * u[3] + zp * u[2];
+    o[0] = xpp * u[4] + ypp * u[5];
+    o[1] = -(xpp * u[5]) + ypp * u[4];
+    o[2] = zpp;
+}
+
 // tlen of gen_shape_proj (view_proj3D.f:306-455).  MAX/MIN as the Fortran
 // compiler lowers them: compare, then select the second operand when the
 // compare fails.
@@ -226,24 +242,13 @@ __global__ __launch_bounds__(kThreads) void k_phantom_lines(
     out[o] = acc;
 }
 
-// Voxel embedding of [nx][ny][nz] at voxel centres (tomo3DfortranAux.f:
-// 226-266 for the ellipsoid, :508-580 for general objects).
-__global__ __launch_bounds__(kThreads) void k_phantom_embed(
-    const QShape* __restrict__ shp, int nshape, const QSurf* __restrict__ srf,
-    float* __restrict__ out, int nx, int ny, int nz, float dx, float dy, float dz, float x0i,
-    float y0i, float z0i, int accumulate) {
-    const long long o = (long long)blockIdx.x * kThreads + threadIdx.x;
-    const long long n = (long long)nx * ny * nz;
-    if (o >= n) return;
-    const long long r = o / nz;
-    const int k = (int)(o - r * nz);
-    const int i = (int)(r / ny);
-    const int j = (int)(r - (long long)i * ny);
-    const float x = x0i + ((float)i + 0.5f) * dx;
-    const float y = y0i + ((float)j + 0.5f) * dy;
-    const float z = z0i + ((float)k + 0.5f) * dz;
-    float acc = accumulate ? out[o] : 0.f;
-    bool first = !accumulate;
+// The phantom's value at (x, y, z): the components' voxel values
+// (ellipsoid_embed, gen_obj_embed, ellipse_voxval, gen_obj_voxval of
+// tomo3DfortranAux.f:179-582) added in table order onto acc, or, when first
+// is set, starting from the first component's value.
+__device__ float point_value(const QShape* __restrict__ shp, int nshape,
+                             const QSurf* __restrict__ srf, float x, float y, float z, float acc,
+                             bool first) {
     for (int s = 0; s < nshape; ++s) {
         const QShape sh = shp[s];
         float c;
@@ -259,9 +264,9 @@ __global__ __launch_bounds__(kThreads) void k_phantom_embed(
             for (int q = 0; q < sh.nsurf; ++q) {
                 const QSurf sf = srf[sh.first + q];
                 float f[3];
-                to_frame(sf.u, x, y, z, f);
+                surf_frame(sf, x, y, z, f);
                 float e, a, b;
-                switch (sf.kind) {
+                switch (sf.kind & 3) {
                 case CT2_SURF_PLANE: si = si * (f[2] >= 0.f ? 1.f : 0.f); break;
                 case CT2_SURF_ELLIPSOID:
                     a = f[0] / sf.ax; b = f[1] / sf.ay;
@@ -285,7 +290,37 @@ __global__ __launch_bounds__(kThreads) void k_phantom_embed(
         acc = first ? c : acc + c;
         first = false;
     }
-    out[o] = acc;
+    return acc;
+}
+
+// Voxel embedding of [nx][ny][nz] at voxel centres (tomo3DfortranAux.f:
+// 226-266 for the ellipsoid, :508-580 for general objects).
+__global__ __launch_bounds__(kThreads) void k_phantom_embed(
+    const QShape* __restrict__ shp, int nshape, const QSurf* __restrict__ srf,
+    float* __restrict__ out, int nx, int ny, int nz, float dx, float dy, float dz, float x0i,
+    float y0i, float z0i, int accumulate) {
+    const long long o = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const long long n = (long long)nx * ny * nz;
+    if (o >= n) return;
+    const long long r = o / nz;
+    const int k = (int)(o - r * nz);
+    const int i = (int)(r / ny);
+    const int j = (int)(r - (long long)i * ny);
+    const float x = x0i + ((float)i + 0.5f) * dx;
+    const float y = y0i + ((float)j + 0.5f) * dy;
+    const float z = z0i + ((float)k + 0.5f) * dz;
+    out[o] = point_value(shp, nshape, srf, x, y, z, accumulate ? out[o] : 0.f, !accumulate);
+}
+
+// Voxel values at arbitrary points xyz[n][3] (the scalar point tests of
+// tomo3DfortranAux.f:179-210 and :311-416, one lane per point).
+__global__ __launch_bounds__(kThreads) void k_phantom_points(
+    const QShape* __restrict__ shp, int nshape, const QSurf* __restrict__ srf,
+    const float* __restrict__ xyz, float* __restrict__ out, long long n, int accumulate) {
+    const long long o = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (o >= n) return;
+    const float x = xyz[3 * o], y = xyz[3 * o + 1], z = xyz[3 * o + 2];
+    out[o] = point_value(shp, nshape, srf, x, y, z, accumulate ? out[o] : 0.f, !accumulate);
 }
 
 // Validates the host tables and copies them, packed, into one stream-ordered
@@ -300,7 +335,7 @@ struct Staged {
 };
 
 int stage(const char* who, const ct2_shape* shapes, int nshape, const ct2_surface* surfaces,
-          int nsurface, hipStream_t st, Staged& out) {
+          int nsurface, hipStream_t st, Staged& out, bool allow_angles = false) {
     if (nshape < 0 || nsurface < 0) { set_error("%s: negative table size", who); return CT2_EINVAL; }
     if (nshape > 0 && !shapes) { set_error("%s: null shape table", who); return CT2_EINVAL; }
     std::vector<QShape> qs((size_t)nshape);
@@ -328,19 +363,32 @@ int stage(const char* who, const ct2_shape* shapes, int nshape, const ct2_surfac
             for (int kind = CT2_SURF_PLANE; kind <= CT2_SURF_CONE; ++kind)
                 for (int m = 0; m < in.nsurf; ++m) {
                     const ct2_surface& f = surfaces[in.first + m];
-                    if (f.kind < CT2_SURF_PLANE || f.kind > CT2_SURF_CONE) {
-                        set_error("%s: surface %d: unknown kind %d", who, in.first + m, f.kind);
+                    const bool angles = (f.kind & CT2_SURF_ANGLES) != 0;
+                    const int base = f.kind & ~CT2_SURF_ANGLES;
+                    if (base < CT2_SURF_PLANE || base > CT2_SURF_CONE || (angles && !allow_angles)) {
+                        set_error("%s: surface %d: unknown kind %d%s", who, in.first + m, f.kind,
+                                  angles ? " (angle frames are for voxel values only)" : "");
                         return CT2_EINVAL;
                     }
-                    if (f.kind != kind) continue;
+                    if (base != kind) continue;
                     QSurf r{};
                     r.kind = f.kind;
                     r.ax = f.ax; r.ay = f.ay; r.az = f.az;
-                    for (int c = 0; c < 3; ++c) {
-                        r.u[c] = f.xu[c];
-                        r.u[3 + c] = f.yu[c];
-                        r.u[6 + c] = f.zu[c];
-                        r.u[9 + c] = f.c[c];
+                    if (angles) {
+                        // the angles in xu; their cosines and sines by the C
+                        // library, as the compiled Fortran's framer takes them
+                        for (int c = 0; c < 3; ++c) {
+                            r.u[2 * c] = cosf(f.xu[c]);
+                            r.u[2 * c + 1] = sinf(f.xu[c]);
+                            r.u[9 + c] = f.c[c];
+                        }
+                    } else {
+                        for (int c = 0; c < 3; ++c) {
+                            r.u[c] = f.xu[c];
+                            r.u[3 + c] = f.yu[c];
+                            r.u[6 + c] = f.zu[c];
+                            r.u[9 + c] = f.c[c];
+                        }
                     }
                     qf.push_back(r);
                 }
@@ -437,10 +485,29 @@ int ct2_phantom_embed(const ct2_shape* shapes, int nshape, const ct2_surface* su
     if (blocks_for(n) > 0x7fffffffu) { set_error("%s: volume too large", who); return CT2_EINVAL; }
     hipStream_t st = (hipStream_t)stream;
     Staged t;
-    int rc = stage(who, shapes, nshape, surfaces, nsurface, st, t);
+    int rc = stage(who, shapes, nshape, surfaces, nsurface, st, t, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_phantom_embed, dim3(blocks_for(n)), dim3(kThreads), 0, st, t.shapes(),
                        nshape, t.surfs(), smat, nx, ny, nz, dx, dy, dz, x0, y0, z0, accumulate);
+    hipError_t e = hipGetLastError();
+    rc = release(t, st);
+    CT2_HIP(e);
+    return rc;
+}
+
+int ct2_phantom_points(const ct2_shape* shapes, int nshape, const ct2_surface* surfaces,
+                       int nsurface, const float* xyz, float* out, size_t n, int accumulate,
+                       ct2_stream_t stream) {
+    const char* who = "ct2_phantom_points";
+    if (n == 0) return CT2_OK;
+    if (!xyz || !out) { set_error("%s: null pointer", who); return CT2_EINVAL; }
+    if (n > (size_t)0x7fffffff * kThreads) { set_error("%s: too many points", who); return CT2_EINVAL; }
+    hipStream_t st = (hipStream_t)stream;
+    Staged t;
+    int rc = stage(who, shapes, nshape, surfaces, nsurface, st, t, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_phantom_points, dim3(blocks_for((long long)n)), dim3(kThreads), 0, st,
+                       t.shapes(), nshape, t.surfs(), xyz, out, (long long)n, accumulate);
     hipError_t e = hipGetLastError();
     rc = release(t, st);
     CT2_HIP(e);

@@ -76,6 +76,7 @@ SIGNATURES = {
     "ct2_phantom_project": "PiPiPPPiiiffffiS",
     "ct2_phantom_project_rays": "PiPiPPPiiS",
     "ct2_phantom_embed": "PiPiPiiiffffffiS",
+    "ct2_phantom_points": "PiPiPPziS",
     "ct2_sinoscale": "PPiiiffffS",
     "ct2_sinoave": "PiiiP" + "i" * 9 + "S",
     "ct2_sinodist": "PiiiP" + "i" * 9 + "S",

@@ -29,6 +29,7 @@ from ._lib import call
 
 CT2_SHAPE_ELLIPSOID, CT2_SHAPE_GENERAL = 0, 1
 CT2_SURF_PLANE, CT2_SURF_ELLIPSOID, CT2_SURF_CYLINDER, CT2_SURF_CONE = 0, 1, 2, 3
+CT2_SURF_ANGLES = 16   # voxel values only: frame from angles, as framer
 _KIND_OF_LABEL = {"plane": CT2_SURF_PLANE, "ellipsoid": CT2_SURF_ELLIPSOID,
                   "cylinder": CT2_SURF_CYLINDER, "cone": CT2_SURF_CONE}
 
@@ -108,6 +109,31 @@ def tables_from_f2py(att, nsurf, xu1, xu2, xu3, yu1, yu2, yu3, zu1, zu2, zu3, x0
     return t
 
 
+def tables_from_f2py_angles(att, nsurf, alpha, beta, gam, x0, y0, z0, nplane, nellipsoid,
+                            ax_ell, ay_ell, az_ell, ncyl, ax_cyl, ay_cyl, ncone, ax_cone, ay_cone):
+    """One general object from gen_obj_voxval's arguments
+    (tomo3DfortranAux.f:311-346): surface frames given by the angles alpha,
+    beta, gam and the centres, taken by framer (tomo3DfortranAux.f:281-308)."""
+    cols = [np.asarray(a, np.float64).reshape(-1) for a in (alpha, beta, gam, x0, y0, z0)]
+    nsurf = int(nsurf)
+    counts = [int(nplane) - 1, int(nellipsoid) - 1, int(ncyl) - 1, int(ncone) - 1]
+    if min(counts) < 0 or sum(counts) > nsurf or any(c.size < nsurf for c in cols):
+        raise ValueError("inconsistent surface counts for a general object")
+    axes = {CT2_SURF_ELLIPSOID: (ax_ell, ay_ell, az_ell), CT2_SURF_CYLINDER: (ax_cyl, ay_cyl),
+            CT2_SURF_CONE: (ax_cone, ay_cone)}
+    rows, s = [], 0
+    for kind, cnt in enumerate(counts):
+        for q in range(1, cnt + 1):
+            a = [float(np.asarray(v).reshape(-1)[q]) for v in axes.get(kind, ())]
+            a += [0.0] * (3 - len(a))
+            u = [c[s] for c in cols]
+            rows.append((kind | CT2_SURF_ANGLES, a, u[0:3], (0., 0., 0.), (0., 0., 0.), u[3:6]))
+            s += 1
+    t = Tables()
+    t.add_general(att, rows)
+    return t
+
+
 # ------------------------------------------------------------ device calls ----
 def project_views(tables, sino, ind, fv, ns, nu, nv, du, dv, u0, v0, accumulate):
     """sino: contiguous float32 device tensor [ns, nu, nv] (written in place)."""
@@ -128,6 +154,20 @@ def embed(tables, vol, nx, ny, nz, dx, dy, dz, x0, y0, z0, accumulate):
     call("ct2_phantom_embed", C.cast(sh, C.c_void_p), nsh, C.cast(sf, C.c_void_p), nsf,
          _dev.ptr(vol), nx, ny, nz, dx, dy, dz, x0, y0, z0, int(accumulate),
          _dev.stream_handle())
+
+
+def voxval_at(tables, x, y, z):
+    """The phantom's voxel value at points (x, y, z), broadcast together, in
+    one ct2_phantom_points launch: a float for scalar inputs, else a float32
+    array of the broadcast shape."""
+    b = np.broadcast_arrays(*(np.asarray(v, np.float32) for v in (x, y, z)))
+    xyz = torch.from_numpy(np.ascontiguousarray(np.stack(b, -1).reshape(-1, 3))).cuda()
+    out = torch.empty(xyz.shape[0], dtype=torch.float32, device="cuda")
+    sh, nsh, sf, nsf = tables.arrays()
+    call("ct2_phantom_points", C.cast(sh, C.c_void_p), nsh, C.cast(sf, C.c_void_p), nsf,
+         _dev.ptr(xyz), _dev.ptr(out), xyz.shape[0], 0, _dev.stream_handle())
+    res = out.cpu().numpy().reshape(b[0].shape)
+    return float(res) if res.ndim == 0 else res
 
 
 def _f32_dev(x, shape):
@@ -210,6 +250,16 @@ class shape3D:
     def embed_in(self, image):
         """Adds the voxelised shape to image.mat (tomo3D.py:1967, :2732)."""
         _embed_image(self.tables(), image)
+
+    def _voxval_tables(self):
+        """The tables of the shape's point test voxvalf."""
+        return self.tables()
+
+    def voxvalf(self, x, y, z):
+        """The shape's value at (x, y, z) by the reference's point test
+        (ellipsoid: tomo3D.py:2725-2730; general object: tomo3D.py:2122-2220);
+        arrays of points are evaluated in one launch."""
+        return voxval_at(self._voxval_tables(), x, y, z)
 
 
 class ellipsoid(shape3D):
@@ -334,6 +384,22 @@ class gen_object(shape3D):
                              (s.zu1, s.zu2, s.zu3),
                              (s.xc + self.x0, s.yc + self.y0, s.zc + self.z0)))
         tables.add_general(self.att, rows)
+
+    def _voxval_tables(self):
+        """gen_object.voxvalf's arguments (tomo3D.py:2122-2206): each surface's
+        own angles, not the frame that rotate() turned, and its centre offset
+        by the object's, planes first, then ellipsoids, cylinders, cones."""
+        rows = []
+        for kind in (CT2_SURF_PLANE, CT2_SURF_ELLIPSOID, CT2_SURF_CYLINDER, CT2_SURF_CONE):
+            for s in self.surfaces:
+                if _KIND_OF_LABEL.get(s.label) != kind:
+                    continue
+                rows.append((kind | CT2_SURF_ANGLES, s.axes(), (s.alpha, s.beta, s.gamma),
+                             (0., 0., 0.), (0., 0., 0.),
+                             (s.xc + self.x0, s.yc + self.y0, s.zc + self.z0)))
+        t = Tables()
+        t.add_general(self.att, rows)
+        return t
 
 
 class phantom3D:

@@ -286,6 +286,12 @@ class image3D:
     def get_dz(self):
         return self.zlen / self.nz
 
+    def add_shape(self, shape):
+        """tomo3D.py:1073-1085: adds shape.voxvalf at every voxel centre to
+        mat; one launch over the volume instead of a Python loop over voxels."""
+        from . import quadric
+        quadric._embed_image(shape._voxval_tables(), self)
+
     def _grid(self):
         return (self.dx, self.dy, self.dz, self.x0, self.y0, self.z0, self.nx, self.ny, self.nz)
 

@@ -33,6 +33,22 @@ def gen_obj_embed(nx, ny, nz, dx, dy, dz, x0i, y0i, z0i, att, nsurf, *surf):
     return _embed(t, nx, ny, nz, dx, dy, dz, x0i, y0i, z0i)
 
 
+def ellipse_voxval(x, y, z, att, alpha, beta, x0, y0, z0, ax, ay, az):
+    """tomo3DfortranAux.f:179-210: att when (x, y, z) is strictly inside the
+    ellipsoid, else 0; called from emil_code/tomo3D.py:2728.  Arrays of points
+    (broadcast together) give an array, in one launch."""
+    t = quadric.Tables()
+    t.add_ellipsoid(att, ax, ay, az, x0, y0, z0, alpha, beta)
+    return quadric.voxval_at(t, x, y, z)
+
+
+def gen_obj_voxval(x, y, z, att, nsurf, *surf):
+    """tomo3DfortranAux.f:311-416: att times the product of the surfaces'
+    steps at (x, y, z), surface frames by framer; called from
+    emil_code/tomo3D.py:2207.  Arrays of points give an array."""
+    return quadric.voxval_at(quadric.tables_from_f2py_angles(att, nsurf, *surf), x, y, z)
+
+
 # ---- ray bunching (tomo3DfortranAux.f:1-176) ----
 def sinoscale(sinomat, frame_vectors, ns, nu, nv, du, dv, u0, v0):
     """tomo3DfortranAux.f:1-67 (intent(in,out) sinomat): each bin divided by

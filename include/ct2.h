@@ -344,6 +344,13 @@ int ct2_fdk_backproj_linear3d(int ns, int nu, int nv, float du, float dv, float 
  * arrays; the library stages them in a stream-ordered device allocation. */
 enum { CT2_SHAPE_ELLIPSOID = 0, CT2_SHAPE_GENERAL = 1 };
 enum { CT2_SURF_PLANE = 0, CT2_SURF_ELLIPSOID = 1, CT2_SURF_CYLINDER = 2, CT2_SURF_CONE = 3 };
+/* Kind flag for the voxel-value calls (ct2_phantom_embed, ct2_phantom_points)
+ * only: the surface frame is given by angles, alpha, beta, gamma in xu[0..2]
+ * (yu, zu unused), and a point is taken into it in the operation order of
+ * framer (tomo3DfortranAux.f:281-308), which the reference's point tests
+ * gen_obj_voxval (tomo3DfortranAux.f:311-416) and image3D.add_shape
+ * (tomo3D.py:1073-1085, through gen_object.voxvalf, tomo3D.py:2122-2220) use. */
+enum { CT2_SURF_ANGLES = 16 };
 
 /* One bounding surface of a general object (tomo3D.py:1704-1832): the
  * surface frame's x, y and z unit vectors in space-fixed coordinates (the
@@ -398,6 +405,16 @@ int ct2_phantom_embed(const ct2_shape* shapes, int nshape, const ct2_surface* su
                       int nsurface, float* smat, int nx, int ny, int nz, float dx, float dy,
                       float dz, float x0, float y0, float z0, int accumulate,
                       ct2_stream_t stream);
+
+/* out[q] = the phantom's voxel value at the point xyz[q][0..2] (device
+ * arrays, n points; accumulate = 1 adds it to out[q]): the scalar point tests
+ * ellipse_voxval (tomo3DfortranAux.f:179-210) and gen_obj_voxval
+ * (tomo3DfortranAux.f:311-416), which tomo3D.py:2122-2220 and :2725-2730
+ * call once per point, evaluated for n points in one launch.  Boundaries as
+ * the reference: an ellipsoid component is strict, a surface's step closed. */
+int ct2_phantom_points(const ct2_shape* shapes, int nshape, const ct2_surface* surfaces,
+                       int nsurface, const float* xyz, float* out, size_t n, int accumulate,
+                       ct2_stream_t stream);
 
 /* -------------------------------------------------------- ray bunching -- */
 /* tomo3DfortranAux.f:1-67 sinoscale (sinogram3D.homoscale, tomo3D.py:395):

@@ -281,12 +281,13 @@ class _Restated:
 
     # ---- analytic phantoms (view_proj3D.f:11-603, ray_proj3D.f:175-512,
     #      tomo3DfortranAux.f:212-582) ----
-    def _gen_tables(self, nsurf, rest):
+    def _gen_tables(self, nsurf, rest, ncols=12):
         """rest = (xu1, xu2, xu3, yu1, yu2, yu3, zu1, zu2, zu3, x0, y0, z0,
         nplane, nellipsoid, ax_el, ay_el, az_el, ncyl, ax_cyl, ay_cyl,
-        ncone, ax_cone, ay_cone) as the f2py wrappers take them."""
-        cols, (npl, nel, axe, aye, aze, ncy, axc, ayc, nco, axk, ayk) = rest[:12], rest[12:]
-        sf = np.zeros((max(int(nsurf), 1), 12), self.dt)
+        ncone, ax_cone, ay_cone) as the f2py wrappers take them (ncols = 6:
+        alpha, beta, gam, x0, y0, z0 in place of the twelve frame columns)."""
+        cols, (npl, nel, axe, aye, aze, ncy, axc, ayc, nco, axk, ayk) = rest[:ncols], rest[ncols:]
+        sf = np.zeros((max(int(nsurf), 1), ncols), self.dt)
         for c, v in enumerate(cols):
             sf[:int(nsurf), c] = np.asarray(v, self.dt).reshape(-1)[:int(nsurf)]
         arr = [self._a(np.asarray(a, self.dt).reshape(-1)) for a in (axe, aye, aze, axc, ayc, axk, ayk)]
@@ -341,6 +342,25 @@ class _Restated:
                                         *self._geo(dx, dy, dz, x0i, y0i, z0i, att, alpha, beta,
                                                    x0, y0, z0, ax, ay, az))
         return out
+
+    def _points(self, x, y, z):
+        b = np.broadcast_arrays(*(np.asarray(v, np.float64) for v in (x, y, z)))
+        return b[0].shape, np.ascontiguousarray(np.stack(b, -1).reshape(-1, 3), self.dt)
+
+    def ellipse_voxval(self, x, y, z, att, alpha, beta, x0, y0, z0, ax, ay, az):
+        shape, xyz = self._points(x, y, z)
+        out = np.zeros(len(xyz), self.dt)
+        self._fn("ora_ellipse_voxval")(_ptr(out), _ptr(xyz), C.c_long(len(xyz)),
+                                       *self._geo(att, alpha, beta, x0, y0, z0, ax, ay, az))
+        return out.reshape(shape)
+
+    def gen_obj_voxval(self, x, y, z, att, nsurf, *rest):
+        shape, xyz = self._points(x, y, z)
+        out = np.zeros(len(xyz), self.dt)
+        keep, targs = self._gen_tables(nsurf, rest, ncols=6)
+        self._fn("ora_gen_obj_voxval")(_ptr(out), _ptr(xyz), C.c_long(len(xyz)),
+                                       self.cr(att), *targs)
+        return out.reshape(shape)
 
     def gen_obj_embed(self, nx, ny, nz, dx, dy, dz, x0i, y0i, z0i, att, nsurf, *rest):
         out = np.zeros((nx, ny, nz), self.dt)
@@ -451,6 +471,7 @@ class _Restated:
             ellipsoid_proj=o.ray_ellipsoid_proj,
         )
         aux = SimpleNamespace(ellipsoid_embed=o.ellipsoid_embed, gen_obj_embed=o.gen_obj_embed,
+                              ellipse_voxval=o.ellipse_voxval, gen_obj_voxval=o.gen_obj_voxval,
                               sinoscale=o.sinoscale, sinoave=o.sinoave, sinodist=o.sinodist)
         tv = SimpleNamespace(
             total_var3d=lambda lam, s, nx, ny, nz: o.total_var3d(lam, s, nx, ny, nz),
@@ -714,6 +735,32 @@ class _Ref:
                    *[r(x) for x in p15])
         return np.ascontiguousarray(out)
 
+    def _voxfun(self, name, x, y, z, args):
+        """A REAL*4 function of tomo3DfortranAux.f called point by point."""
+        fn = getattr(self.libs["tomo3DfortranAux"], name + "_")
+        fn.restype = C.c_float
+        b = np.broadcast_arrays(*(np.asarray(v, np.float32) for v in (x, y, z)))
+        out = np.array([fn(self._r(p), self._r(q), self._r(t), *args)
+                        for p, q, t in zip(*(v.reshape(-1) for v in b))], np.float32)
+        return out.reshape(b[0].shape)
+
+    def _ell_voxval(self, x, y, z, *p9):
+        return self._voxfun("ellipse_voxval", x, y, z, [self._r(v) for v in p9])
+
+    def _gen_voxval(self, x, y, z, att, nsurf, *rest):
+        cols, tail = rest[:6], rest[6:]
+        n = int(nsurf)
+        arrs = [np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1)[:n]) if n else
+                np.zeros(1, np.float32) for v in cols]
+        npl, nel, axe, aye, aze, ncy, axc, ayc, nco, axk, ayk = tail
+        ax = [np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1)) for a in
+              (axe, aye, aze, axc, ayc, axk, ayk)]
+        i = self._i
+        args = [self._r(att), i(n), *[_ptr(a) for a in arrs], i(npl), i(nel), _ptr(ax[0]),
+                _ptr(ax[1]), _ptr(ax[2]), i(ncy), _ptr(ax[3]), _ptr(ax[4]), i(nco), _ptr(ax[5]),
+                _ptr(ax[6])]
+        return self._voxfun("gen_obj_voxval", x, y, z, args)
+
     def _gen_embed(self, nx, ny, nz, dx, dy, dz, x0i, y0i, z0i, att, nsurf, *rest):
         out = self._F(np.zeros((nx, ny, nz)))
         keep, targs = self._gen_refargs(nsurf, rest)
@@ -785,6 +832,7 @@ class _Ref:
             total_p_var_grad3d=lambda *a: o._pgrad("total_p_var_grad3d_", *a),
             total_p_dvar_grad3d=lambda *a: o._pgrad("total_p_dvar_grad3d_", *a))
         aux = SimpleNamespace(ellipsoid_embed=o._ell_embed, gen_obj_embed=o._gen_embed,
+                              ellipse_voxval=o._ell_voxval, gen_obj_voxval=o._gen_voxval,
                               sinoscale=o._sinoscale, sinoave=o._sinoave, sinodist=o._sinodist)
         return SimpleNamespace(view_proj3D=vp, view_proj3D_tomo=vpt, ray_proj3D=rp, totalvar=tv,
                                tomo3Dfort=aux)

@@ -1136,6 +1136,74 @@ EXPORT void FN(ora_gen_obj_embed)(REAL *s, int nx, int ny, int nz, REAL dx, REAL
                 s[((long)i * ny + j) * nz + k] = att * si;
             }
 }
+/* ellipse_voxval (tomo3DfortranAux.f:179-210) at n points xyz[n][3]: att
+ * strictly inside the ellipsoid, zero elsewhere */
+EXPORT void FN(ora_ellipse_voxval)(REAL *out, const REAL *xyz, long n, REAL att, REAL alpha,
+                                   REAL beta, REAL x0, REAL y0, REAL z0, REAL ax, REAL ay,
+                                   REAL az)
+{
+    REAL ca = COS(alpha), sa = SIN(alpha), cb = COS(beta), sb = SIN(beta);
+    for (long q = 0; q < n; ++q) {
+        REAL rx = xyz[3 * q] - x0, ry = xyz[3 * q + 1] - y0, rz = xyz[3 * q + 2] - z0;
+        REAL r1x = cb * rx + sb * ry, r1y = -(sb * rx) + cb * ry, r1z = rz;
+        REAL r2x = ca * r1x + sa * r1z, r2y = r1y, r2z = -(sa * r1x) + ca * r1z;
+        REAL lhs = (r2x / ax) * (r2x / ax) + (r2y / ay) * (r2y / ay) + (r2z / az) * (r2z / az);
+        out[q] = (lhs < R(1.)) ? att : R(0);
+    }
+}
+
+/* framer (tomo3DfortranAux.f:281-308): (x, y, z) in the frame turned by
+ * alpha about z, then beta, then gam, about the centre a[3..5]; a = (alpha,
+ * beta, gam, x0, y0, z0) */
+static void FN(framer)(REAL x, REAL y, REAL z, const REAL *a, REAL *o)
+{
+    REAL ca = COS(a[0]), sa = SIN(a[0]), cb = COS(a[1]), sb = SIN(a[1]);
+    REAL cg = COS(a[2]), sg = SIN(a[2]);
+    REAL xp = (x - a[3]) * ca - (a[4] - y) * sa;
+    REAL yp = (y - a[4]) * ca + (a[3] - x) * sa;
+    REAL zp = z - a[5];
+    REAL xpp = xp * cb - zp * sb, ypp = yp, zpp = xp * sb + zp * cb;
+    o[0] = xpp * cg + ypp * sg;
+    o[1] = -(xpp * sg) + ypp * cg;
+    o[2] = zpp;
+}
+
+/* gen_obj_voxval (tomo3DfortranAux.f:311-416) at n points: att times the
+ * product of the surfaces' steps, the surfaces' frames by framer from the
+ * rows sf[nsurf][6] = (alpha, beta, gam, x0, y0, z0); counts and semi-axis
+ * arrays carry the reference's leading dummy entry */
+EXPORT void FN(ora_gen_obj_voxval)(REAL *out, const REAL *xyz, long n, REAL att, const REAL *sf,
+                                   int nplane, int nell, const REAL *axe, const REAL *aye,
+                                   const REAL *aze, int ncyl, const REAL *axc, const REAL *ayc,
+                                   int ncone, const REAL *axk, const REAL *ayk)
+{
+    int counts[4] = {nplane, nell, ncyl, ncone};
+    for (long p = 0; p < n; ++p) {
+        REAL si = R(1.0);
+        int is = 0;
+        for (int kind = 0; kind < 4; ++kind)
+            for (int q = 1; q < counts[kind]; ++q, ++is) {
+                REAL f[3], e;
+                FN(framer)(xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2], sf + 6 * is, f);
+                if (kind == 0) {
+                    si = si * (f[2] >= R(0) ? R(1) : R(0));
+                } else if (kind == 1) {
+                    e = (f[0] / axe[q]) * (f[0] / axe[q]) + (f[1] / aye[q]) * (f[1] / aye[q])
+                      + (f[2] / aze[q]) * (f[2] / aze[q]);
+                    si = si * ((R(1.0) - e) >= R(0) ? R(1) : R(0));
+                } else if (kind == 2) {
+                    e = (f[0] / axc[q]) * (f[0] / axc[q]) + (f[1] / ayc[q]) * (f[1] / ayc[q]);
+                    si = si * ((R(1.0) - e) >= R(0) ? R(1) : R(0));
+                } else {
+                    e = (f[0] / axk[q]) * (f[0] / axk[q]) + (f[1] / ayk[q]) * (f[1] / ayk[q]);
+                    si = si * ((f[2] * f[2] - e) >= R(0) ? R(1) : R(0))
+                            * (f[2] >= R(0) ? R(1) : R(0));
+                }
+            }
+        out[p] = att * si;
+    }
+}
+
 #undef ALARGE
 
 /* ---- ray bunching (tomo3DfortranAux.f:1-176; tomo3D.py:395-399, :484-534) ---- */
