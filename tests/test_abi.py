@@ -115,3 +115,20 @@ def test_argument_errors_without_gpu():
     with pytest.raises(CT2Error, match="null pointer"):
         call("ct2_stencil_art", C.byref(d), C.byref(st), None, 1.0, None, None, None, None, None,
              None)
+
+
+def test_angle_frames_only_for_voxel_values():
+    """A surface with the angle-frame flag is rejected by the projection
+    entries on the host, with a message, before any HIP call."""
+    import ctypes as C
+    from ct2 import _lib, quadric
+    t = quadric.Tables()
+    t.add_general(1.0, [(quadric.CT2_SURF_PLANE | quadric.CT2_SURF_ANGLES, (0., 0., 0.),
+                         (0.1, 0.2, 0.3), (0., 0., 0.), (0., 0., 0.), (0., 0., 0.))])
+    sh, nsh, sf, nsf = t.arrays()
+    buf = (C.c_float * 64)()
+    L = _lib.lib()
+    rc = L.ct2_phantom_project(C.cast(sh, C.c_void_p), nsh, C.cast(sf, C.c_void_p), nsf,
+                               C.cast(buf, C.c_void_p), None, C.cast(buf, C.c_void_p), 1, 2, 2,
+                               1.0, 1.0, 0.0, 0.0, 0, None)
+    assert rc == -1 and "angle frames" in L.ct2_last_error().decode()
