@@ -245,27 +245,6 @@ def read_traffic():
     return None
 
 
-def gpu_ellipsoids(shape, voxel, count, rng):
-    """Additive random axis-aligned ellipsoids rasterised on the device
-    (centres U[-0.7,0.7], semi-axes U[0.03,0.4], densities U[0.05,1.0] in
-    units of the half-width of the volume's largest side)."""
-    import torch
-    nx, ny, nz = shape
-    hw = max(shape) * voxel / 2
-    ax = [(torch.arange(m, device="cuda", dtype=torch.float32) + 0.5 - m / 2) * voxel
-          for m in shape]
-    X, Y, Z = ax[0][:, None, None], ax[1][None, :, None], ax[2][None, None, :]
-    vol = torch.zeros(shape, dtype=torch.float32, device="cuda")
-    c = rng.uniform(-0.7, 0.7, (count, 3)) * hw
-    a = rng.uniform(0.03, 0.4, (count, 3)) * hw
-    rho = rng.uniform(0.05, 1.0, count)
-    for i in range(count):
-        q = ((X - c[i, 0]) / a[i, 0]) ** 2 + ((Y - c[i, 1]) / a[i, 1]) ** 2 + \
-            ((Z - c[i, 2]) / a[i, 2]) ** 2
-        vol += float(rho[i]) * (q <= 1.0).float()
-    return vol
-
-
 def batch_workload(args):
     """64 independent cone-beam problems (seed 20260104): cubic volumes of
     n ~ U{64..192} voxels of 1 mm, views ~ U{30..180} over 360 deg, detector
@@ -276,7 +255,7 @@ def batch_workload(args):
     through ct2_sart_step is timed for comparison."""
     import numpy as np
     import torch
-    from ct2 import configs3D
+    from ct2 import configs3D, phantoms
     from ct2.batch import BatchPlan
     from ct2.operator import ConeBeamOperator
     from ct2.solvers import capture
@@ -293,7 +272,7 @@ def batch_workload(args):
                                      ns, 2 * np.pi, 0.0, 0)
         op = ConeBeamOperator(fv, ns, nd, nd, 1.6, 1.6, -nd * 0.8, -nd * 0.8, (n, n, n),
                               (1.0, 1.0, 1.0), (-n / 2, -n / 2, -n / 2), arith="fast")
-        v = gpu_ellipsoids((n, n, n), 1.0, 20, rng)
+        v = phantoms.random_ellipsoids_device((n, n, n), 1.0, 20, rng)
         ops.append(op)
         truth.append(v)
         sinos.append(op.forward(v))

@@ -112,3 +112,24 @@ def breast_like(shape, voxel, seed, n_background=200, n_calc=50):
         ells.append((*c, r, r, r, 2.0, 0.0))
     add_ellipsoids(vol, grid, ells)
     return vol
+
+
+def random_ellipsoids_device(shape, voxel, count, rng, device="cuda"):
+    """Additive random axis-aligned ellipsoids rasterised into a float32
+    device tensor: centres U[-0.7,0.7], semi-axes U[0.03,0.4], densities
+    U[0.05,1.0], lengths in units of the half-width of the volume's largest
+    side; ``rng`` (a numpy Generator) draws the table, so a seed fixes it."""
+    import torch
+    hw = max(shape) * voxel / 2
+    ax = [(torch.arange(m, device=device, dtype=torch.float32) + 0.5 - m / 2) * voxel
+          for m in shape]
+    X, Y, Z = ax[0][:, None, None], ax[1][None, :, None], ax[2][None, None, :]
+    vol = torch.zeros(shape, dtype=torch.float32, device=device)
+    c = rng.uniform(-0.7, 0.7, (count, 3)) * hw
+    a = rng.uniform(0.03, 0.4, (count, 3)) * hw
+    rho = rng.uniform(0.05, 1.0, count)
+    for i in range(count):
+        q = ((X - c[i, 0]) / a[i, 0]) ** 2 + ((Y - c[i, 1]) / a[i, 1]) ** 2 + \
+            ((Z - c[i, 2]) / a[i, 2]) ** 2
+        vol += float(rho[i]) * (q <= 1.0).float()
+    return vol
