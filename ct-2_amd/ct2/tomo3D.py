@@ -57,6 +57,12 @@ ray_art_algs_dict = {"art_nearest": ray_proj3D.artstep_nearest,
 ray_em_algs_dict = {"em_nearest": ray_proj3D.emstep_nearest}
 
 
+def _round_half_away(x):
+    """Python 2's round(): halves go away from zero (Python 3 rounds them to
+    even); returns a float."""
+    return float(np.copysign(np.floor(abs(x) + 0.5), x))
+
+
 def _pick(table, alg, what):
     if alg not in table:
         raise ValueError(f"bad algorithm choice for {what}: {alg!r} (choose from {sorted(table)})")
@@ -169,6 +175,125 @@ class sinogram3D:
     def copy_to(self, dest):
         dest.mat = self.mat.clone() if _is_tensor(self.mat) else self.mat.copy()
         dest.indicator = self.indicator.clone() if _is_tensor(self.indicator) else self.indicator.copy()
+
+    def flip_detector(self):
+        """Exchange the detector's u and v axes (tomo3D.py:213-238): e_u and e_v
+        of every frame swap, mat and indicator are transposed to [ns][nv][nu],
+        and u0/v0, ulen/vlen, du/dv, nu/nv swap.  The rays are the same, so
+        projections come out transposed and backprojections unchanged."""
+        fv = self.frame_vectors
+        if _is_tensor(fv):
+            fv = fv.clone()
+            fv[:, 9:12], fv[:, 12:15] = self.frame_vectors[:, 12:15], self.frame_vectors[:, 9:12]
+        else:
+            fv = np.array(fv)
+            fv[:, 9:12], fv[:, 12:15] = self.frame_vectors[:, 12:15], self.frame_vectors[:, 9:12]
+        self.frame_vectors = fv
+        if _is_tensor(self.mat):
+            self.mat = self.mat.transpose(1, 2).contiguous()
+        else:
+            self.mat = np.ascontiguousarray(self.mat.transpose((0, 2, 1)))
+        if _is_tensor(self.indicator):
+            self.indicator = self.indicator.transpose(1, 2).contiguous()
+        else:
+            self.indicator = np.ascontiguousarray(self.indicator.transpose((0, 2, 1)))
+        self.shape = tuple(self.mat.shape)
+        self.nu, self.nv = self.shape[1], self.shape[2]
+        self.u0, self.v0 = self.v0, self.u0
+        self.ulen, self.vlen = self.vlen, self.ulen
+        self.du, self.dv = self.dv, self.du
+
+    def scramble(self, spacing):
+        """Reorder the views' frame vectors as views p, p+spacing, ... for
+        p = 0..spacing-1 (tomo3D.py:240-252); ns must be a multiple of
+        spacing."""
+        if self.ns % spacing:
+            raise ValueError(f"ns = {self.ns} is not a multiple of spacing {spacing}")
+        idx = np.arange(self.ns).reshape(self.ns // spacing, spacing).T.ravel()
+        if _is_tensor(self.frame_vectors):
+            idx = _torch().from_numpy(idx).to(self.frame_vectors.device)
+        self.frame_vectors = self.frame_vectors[idx]
+
+    def clear_indicator(self):
+        """Every ray off (tomo3D.py:262-263)."""
+        if _is_tensor(self.indicator):
+            self.indicator = self.indicator.new_zeros((self.ns, self.nu, self.nv))
+        else:
+            self.indicator = np.zeros((self.ns, self.nu, self.nv), indicator_type)
+
+    def config_indicator(self, slow, shigh, ulow, uhigh, vlow, vhigh, ray_op='add'):
+        """Turn on the rays in the s-, u- and v-ranges (tomo3D.py:267-299), with
+        the reference's grid snapping and loops: Python 2 rounding (half away
+        from zero), float accumulation of s, u and v, and v's range open at
+        the top while s and u include theirs."""
+        if ray_op == 'replace':
+            self.clear_indicator()
+        r = _round_half_away
+        sl = max(self.s0, self.s0 + self.ds * r((slow - self.s0) / self.ds))
+        sh = min(self.s0 + self.ds * (self.ns - 1), self.s0 + self.ds * r((shigh - self.s0) / self.ds))
+        ul = max(self.u0, self.u0 + self.du * r((ulow - self.u0) / self.du))
+        uh = min(self.u0 + self.du * (self.nu - 1), self.u0 + self.du * r((uhigh - self.u0) / self.du))
+        vl = max(self.v0, self.v0 + self.dv * r((vlow - self.v0) / self.dv))
+        vh = min(self.v0 + self.dv * (self.nv - 1), self.v0 + self.dv * r((vhigh - self.v0) / self.dv))
+        on = np.zeros((self.ns, self.nu, self.nv), bool)
+        s = sl
+        while s <= sh:
+            i = int(r((s - self.s0) / self.ds))
+            u = ul
+            while u <= uh:
+                j = int(r((u - self.u0) / self.du))
+                v = vl
+                while v < vh:
+                    on[i, j, int(r((v - self.v0) / self.dv))] = True
+                    v += self.dv
+                u += self.du
+            s += self.ds
+        if _is_tensor(self.indicator):
+            self.indicator[_torch().from_numpy(on).to(self.indicator.device)] = 1
+        else:
+            self.indicator[on] = 1
+
+    def _host_mat(self):
+        m = self.mat.cpu().numpy() if _is_tensor(self.mat) else self.mat
+        return np.asarray(m)
+
+    def _set_mat(self, m):
+        m = np.asarray(m, default_type)
+        self.mat = _torch().from_numpy(m).to(self.mat.device) if _is_tensor(self.mat) else m
+
+    def _used(self):
+        ind = self.indicator.cpu().numpy() if _is_tensor(self.indicator) else self.indicator
+        return np.asarray(ind) != 0
+
+    def add_noise(self, std=0.1, rng=None):
+        """Multiplicative noise: every bin times max(0, N(1, std))
+        (tomo3D.py:334-342).  Draws come in the reference's (i, j, k) loop
+        order from ``rng`` (default: numpy's global generator, as the
+        reference), so a seeded numpy.random gives the reference's values."""
+        rng = np.random if rng is None else rng
+        n = self.ns * self.nu * self.nv
+        f = np.maximum(0.0, rng.normal(1.0, std, n)).reshape(self.ns, self.nu, self.nv)
+        self._set_mat(self._host_mat() * f)
+
+    def add_abs_noise(self, std=0.1, rng=None):
+        """Additive N(0, std) noise on the used rays (tomo3D.py:344-352), drawn
+        in (i, j, k) order over the used rays only."""
+        rng = np.random if rng is None else rng
+        used = self._used()
+        # float32 value + float64 draw, rounded to float32, as the reference's
+        # scalar update does
+        m = np.array(self._host_mat(), np.float64)
+        m[used] += rng.normal(0.0, std, int(used.sum()))
+        self._set_mat(m)
+
+    def add_rel_noise(self, delta=0.002, rng=None):
+        """Additive N(0, delta(1 + value/2)) noise on the used rays
+        (tomo3D.py:356-366), drawn in (i, j, k) order over the used rays."""
+        rng = np.random if rng is None else rng
+        used = self._used()
+        m = np.array(self._host_mat(), np.float64)
+        m[used] += rng.normal(0.0, delta * (1.0 + m[used] / 2.0))
+        self._set_mat(m)
 
     def add_poisson_noise(self, photon_num_per_bin=1000, seed=None):
         """Poisson counts I ~ P(N e^-p), p_hat = -ln(max(I,1)/N).  The reference
