@@ -7,7 +7,7 @@
 //   Filt = dp*Re(FFT(ifftshift(h))) (Kak-Slaney ramp h, optional Hann window)
 //   and cropped.  With x real and Filt real this is exactly the circular
 //   convolution y[m] = sum_{i<n} x[i] * hr[(m-i) mod npad], hr = Re(IFFT(Filt)).
-//   hr is computed once per call on the host in FP64 (direct DFT with a
+//   hr is computed once per call on the device in FP64 (direct DFTs over a
 //   cosine table), then every line is filtered by an LDS-tiled batched GEMM
 //   against the implicit Toeplitz matrix T[m][i] = hr[(m-i) & (npad-1)], which
 //   also reproduces the reference's wrap-around when npad < 2n-1.
@@ -15,8 +15,6 @@
 //   zoom = (F+D)/(F - x cos s - y sin s), u = (-x sin s + y cos s) zoom,
 //   v = z zoom, bilinear read with offsets, += val*zoom^2, times pi/ns.
 #include <math.h>
-
-#include <vector>
 
 #include "ct2_internal.h"
 
@@ -146,50 +144,118 @@ __global__ __launch_bounds__(256) void k_fdk_bp(int ns, int nu, int nv, float du
     recon[((size_t)i * ny + j) * nz + k] = acc * (float)(M_PI / ns);
 }
 
-// hr = Re(IFFT(Filt)), Filt = dp*Re(FFT(ifftshift(h))) [* Hann], FP64 direct DFT.
-static void ramp_kernel(int npad, double dp, int hann, double L, std::vector<double>& hr) {
-    const int N = npad;
-    std::vector<double> h(N, 0.0), g(N), ct(N), filt(N);
-    // nn = -N/2 .. N/2-1 (N even: npad is a power of two >= 2)
-    h[N / 2] = 1.0 / (4.0 * dp * dp);
-    for (int ii = 0; ii < N; ++ii) {
-        long nn = ii - N / 2;
-        if (labs(nn) % 2 == 1) {
-            double t = M_PI * (double)nn * dp;
-            h[ii] = -1.0 / (t * t);
+// ---- filter taps on the device ----
+// hr = Re(IFFT(Filt)), Filt = dp*Re(FFT(ifftshift(h))) [* Hann window], as
+// two FP64 direct DFTs over a cosine table.  npad is a power of two, so the
+// angle index k*d mod npad is (k*d) & (npad-1) in 32-bit unsigned arithmetic
+// (npad <= 2^16).  Workspace: cos table [npad] and spectrum [npad] doubles,
+// then the float taps [npad].
+size_t fdk_filter_ws_bytes(int npad) { return (size_t)npad * (2 * sizeof(double) + sizeof(float)); }
+
+__global__ void k_fdk_cos_table(int N, double* __restrict__ ct) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < N; m += gridDim.x * blockDim.x)
+        ct[m] = cospi(2.0 * (double)m / (double)N);
+}
+
+// Filt[k]: the Kak-Slaney kernel h(d') (h(0) = 1/(4 dp^2), h(odd) =
+// -1/(pi d' dp)^2, 0 at even d' != 0) at d' = d for d < N/2 and d - N
+// otherwise (ifftshift), transformed; then the window on the FFT-ordered
+// frequency index: kappa of index i = (k + N/2) mod N is (i - N/2)/(dp N),
+// window 0.5 + 0.5 cos(pi kappa / cutoff) for wlo <= i <= whi, 0 elsewhere
+// (FDK.py:33-73).
+__global__ void k_fdk_ramp_spectrum(int N, double dp, int hann, int wlo, int whi, double cutoff,
+                                    const double* __restrict__ ct, double* __restrict__ filt) {
+    const unsigned msk = (unsigned)N - 1u;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+        double acc = 1.0 / (4.0 * dp * dp);   // d = 0: cos(0) = 1
+        for (int d = 1; d < N; d += 2) {
+            const double dd = (double)(d < N / 2 ? d : d - N);
+            const double t = M_PI * dd * dp;
+            acc += (-1.0 / (t * t)) * ct[((unsigned)k * (unsigned)d) & msk];
         }
-    }
-    for (int d = 0; d < N; ++d) g[d] = h[(d + N / 2) % N];   // ifftshift
-    for (int m = 0; m < N; ++m) ct[m] = cos(2.0 * M_PI * (double)m / (double)N);
-    for (int k = 0; k < N; ++k) {
-        double acc = 0.0;
-        for (int d = 0; d < N; ++d) acc += g[d] * ct[(long)k * d % N];
-        filt[k] = dp * acc;
-    }
-    if (hann) {
-        // kappa = (-N/2 .. N/2-1)/(dp N); window 0.5+0.5cos(pi kappa/cutoff),
-        // cutoff = L/(2 dp); for L < 1 only between the kappa closest to
-        // +-cutoff (FDK.py:55-73); then ifftshift onto FFT order
-        std::vector<double> kap(N), win(N, 0.0);
-        double cutoff = 1.0 / (2.0 * dp) * L;
-        for (int i = 0; i < N; ++i) kap[i] = (double)(i - N / 2) / (dp * N);
-        int start = 0, end = N - 1;
-        if (L < 1.0) {
-            double mind = 1e300;
-            for (int i = 0; i < N; ++i) mind = fmin(mind, fabs(fabs(kap[i]) - cutoff));
-            start = -1;
-            for (int i = 0; i < N; ++i)
-                if (fabs(fabs(kap[i]) - cutoff) == mind) { if (start < 0) start = i; end = i; }
+        double f = dp * acc;
+        if (hann) {
+            const int i = (k + N / 2) & (int)msk;
+            const double kap = (double)(i - N / 2) / (dp * (double)N);
+            f *= (i >= wlo && i <= whi) ? 0.5 + 0.5 * cos(M_PI * kap / cutoff) : 0.0;
         }
-        for (int i = start; i <= end; ++i) win[i] = 0.5 + 0.5 * cos(M_PI * kap[i] / cutoff);
-        for (int k = 0; k < N; ++k) filt[k] *= win[(k + N / 2) % N];
+        filt[k] = f;
     }
-    hr.assign(N, 0.0);
-    for (int m = 0; m < N; ++m) {
+}
+
+__global__ void k_fdk_ramp_taps(int N, const double* __restrict__ ct,
+                                const double* __restrict__ filt, float* __restrict__ hr) {
+    const unsigned msk = (unsigned)N - 1u;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < N; m += gridDim.x * blockDim.x) {
         double acc = 0.0;
-        for (int k = 0; k < N; ++k) acc += filt[k] * ct[(long)k * m % N];
-        hr[m] = acc / (double)N;
+        for (int k = 0; k < N; ++k) acc += filt[k] * ct[((unsigned)k * (unsigned)m) & msk];
+        hr[m] = (float)(acc / (double)N);
     }
+}
+
+// npad = 2^round(log2(2n-1)) (FDK.py:36); 2n-1 is odd, so never half-way.
+int fdk_npad(int n) {
+    int e = (int)lround(log2(2.0 * n - 1.0));
+    return 1 << (e < 1 ? 1 : e);
+}
+
+// Window bounds on the centred frequency grid: every index for L >= 1; for
+// L < 1 from the first to the last index whose |kappa| is nearest the
+// cutoff L/(2 dp) (FDK.py:55-73).  Index arithmetic only.
+static void hann_bounds(int N, double dp, double L, int& wlo, int& whi) {
+    wlo = 0;
+    whi = N - 1;
+    if (L >= 1.0) return;
+    const double cutoff = 1.0 / (2.0 * dp) * L;
+    double mind = 1e300;
+    for (int i = 0; i < N; ++i) mind = fmin(mind, fabs(fabs((double)(i - N / 2) / (dp * N)) - cutoff));
+    wlo = -1;
+    for (int i = 0; i < N; ++i)
+        if (fabs(fabs((double)(i - N / 2) / (dp * N)) - cutoff) == mind) {
+            if (wlo < 0) wlo = i;
+            whi = i;
+        }
+}
+
+// Filters every line of sinomat into out with the taps built in workspace
+// ws (fdk_filter_ws_bytes(npad) bytes of device memory).
+int fdk_filter_run(float dp, int dim, int ns, int nu, int nv, int npad, const float* sinomat,
+                   float* out, int hann, float L, void* ws, hipStream_t st) {
+    double* ct = (double*)ws;
+    double* filt = ct + npad;
+    float* hr = (float*)(filt + npad);
+    const unsigned nb = (unsigned)((npad + 255) / 256);
+    int wlo = 0, whi = npad - 1;
+    if (hann) hann_bounds(npad, (double)dp, (double)L, wlo, whi);
+    hipLaunchKernelGGL(k_fdk_cos_table, dim3(nb), dim3(256), 0, st, npad, ct);
+    hipLaunchKernelGGL(k_fdk_ramp_spectrum, dim3(nb), dim3(256), 0, st, npad, (double)dp, hann,
+                       wlo, whi, 1.0 / (2.0 * (double)dp) * (double)L, (const double*)ct, filt);
+    hipLaunchKernelGGL(k_fdk_ramp_taps, dim3(nb), dim3(256), 0, st, npad, (const double*)ct,
+                       (const double*)filt, hr);
+    dim3 grid((nv + 63) / 64, (nu + 63) / 64, ns);
+    if (dim == 1)
+        hipLaunchKernelGGL(k_fdk_filter<1>, grid, dim3(256), 0, st, nu, nv, npad - 1,
+                           (const float*)hr, sinomat, out);
+    else
+        hipLaunchKernelGGL(k_fdk_filter<2>, grid, dim3(256), 0, st, nu, nv, npad - 1,
+                           (const float*)hr, sinomat, out);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+// Argument checks shared by the filter entry points; sets *npad.
+int fdk_filter_check(const char* who, float dp, int dim, int ns, int nu, int nv,
+                     const float* sinomat, const float* out, int* npad) {
+    if (ns <= 0 || nu <= 0 || nv <= 0 || !sinomat || !out || (dim != 1 && dim != 2) ||
+        dp == 0.0f) {
+        set_error("%s: bad arguments", who);
+        return CT2_EINVAL;
+    }
+    if (sinomat == out) { set_error("%s: in-place filtering is not supported", who); return CT2_EINVAL; }
+    const int n = dim == 1 ? nu : nv;
+    if (n > 32768) { set_error("%s: line of %d bins is too long", who, n); return CT2_EINVAL; }
+    *npad = fdk_npad(n);
+    return CT2_OK;
 }
 
 }  // namespace ct2
@@ -218,41 +284,15 @@ int ct2_fdk_sino_weight3d(float radius, float center2detector, int ns, int nu, i
 
 int ct2_fdk_sino_filter3d(float dp, int dim, int ns, int nu, int nv, const float* sinomat,
                           float* out, int hann, float L, ct2_stream_t stream) {
-    if (ns <= 0 || nu <= 0 || nv <= 0 || !sinomat || !out || (dim != 1 && dim != 2) ||
-        dp == 0.0f) {
-        set_error("ct2_fdk_sino_filter3d: bad arguments");
-        return CT2_EINVAL;
-    }
-    if (sinomat == out) { set_error("ct2_fdk_sino_filter3d: in-place filtering is not supported"); return CT2_EINVAL; }
-    const int n = dim == 1 ? nu : nv;
-    // npad = 2^round(log2(2n-1)) (FDK.py:36); never exactly half-way
-    int e = (int)lround(log2(2.0 * n - 1.0));
-    if (e < 1) e = 1;
-    if (e > 16) { set_error("ct2_fdk_sino_filter3d: line of %d bins is too long", n); return CT2_EINVAL; }
-    const int npad = 1 << e;
-    std::vector<double> hr;
-    ramp_kernel(npad, (double)dp, hann, (double)L, hr);
-    std::vector<float> hrf(npad);
-    for (int q = 0; q < npad; ++q) hrf[q] = (float)hr[q];
+    int npad = 0;
+    int rc = fdk_filter_check("ct2_fdk_sino_filter3d", dp, dim, ns, nu, nv, sinomat, out, &npad);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    void* dh = nullptr;
-    CT2_HIP(hipMallocAsync(&dh, sizeof(float) * npad, st));
-    hipError_t err = hipMemcpyAsync(dh, hrf.data(), sizeof(float) * npad, hipMemcpyHostToDevice, st);
-    if (err == hipSuccess) {
-        dim3 grid((nv + 63) / 64, (nu + 63) / 64, ns);
-        if (dim == 1)
-            hipLaunchKernelGGL(k_fdk_filter<1>, grid, dim3(256), 0, st, nu, nv, npad - 1,
-                               (const float*)dh, sinomat, out);
-        else
-            hipLaunchKernelGGL(k_fdk_filter<2>, grid, dim3(256), 0, st, nu, nv, npad - 1,
-                               (const float*)dh, sinomat, out);
-        err = hipGetLastError();
-    }
-    // the host coefficients must outlive the copy
-    hipError_t s = hipStreamSynchronize(st);
-    hipError_t f = hipFreeAsync(dh, st);
-    CT2_HIP(err);
-    CT2_HIP(s);
+    void* ws = nullptr;
+    CT2_HIP(hipMallocAsync(&ws, fdk_filter_ws_bytes(npad), st));
+    rc = fdk_filter_run(dp, dim, ns, nu, nv, npad, sinomat, out, hann, L, ws, st);
+    hipError_t f = hipFreeAsync(ws, st);
+    if (rc) return rc;
     CT2_HIP(f);
     return CT2_OK;
 }

@@ -319,7 +319,10 @@ int ct2_fdk_sino_weight3d(float radius, float center2detector, int ns, int nu, i
 /* FDK.py:25 fdk_sino_filter3D, dim = 1 (filter along u) or 2 (along v):
  * zero-pad to npad = 2^round(log2(2n-1)), multiply the FFT by the
  * Kak-Slaney ramp (optionally Hann-windowed), crop.  Implemented as the
- * equivalent real circulant matrix applied to every detector line. */
+ * equivalent real circulant matrix applied to every detector line; its taps
+ * are computed on the device (FP64 DFTs).  Allocates a workspace of
+ * 20*npad bytes stream-ordered (hipMallocAsync / hipFreeAsync on the
+ * stream); lines up to 32768 bins. */
 int ct2_fdk_sino_filter3d(float dp, int dim, int ns, int nu, int nv, const float* sinomat,
                           float* out, int hann, float L, ct2_stream_t stream);
 /* FDK.py:130 fdk_backproj_linear3D over the voxel grid
