@@ -16,6 +16,8 @@
 //   v = z zoom, bilinear read with offsets, += val*zoom^2, times pi/ns.
 #include <math.h>
 
+#include <mutex>
+
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -258,6 +260,45 @@ int fdk_filter_check(const char* who, float dp, int dim, int ns, int nu, int nv,
     return CT2_OK;
 }
 
+// ---- scratch of the allocation-free filter ----
+// ct2_fdk_sino_filter3d_noalloc builds its taps in this file-scope device
+// array instead of an allocation: every GPU the library runs on holds its own
+// copy until the process ends.  Host code never passes the array's name to a
+// kernel; the address of each device's copy comes from hipGetSymbolAddress,
+// taken once per device with that device current.  Every call on a device
+// overwrites the whole workspace it uses before reading it, so nothing needs
+// clearing; calls that overlap on one device race for it.
+constexpr int FDK_SCRATCH_DOUBLES =
+    (int)((size_t)CT2_FDK_NOALLOC_MAX_NPAD * (2 * sizeof(double) + sizeof(float)) / sizeof(double));
+static __device__ __align__(256) double fdk_scratch[FDK_SCRATCH_DOUBLES];
+static_assert(sizeof(double) * FDK_SCRATCH_DOUBLES >= CT2_FDK_NOALLOC_MAX_NPAD *
+                                                          (2 * sizeof(double) + sizeof(float)),
+              "scratch holds the workspace of the largest pad");
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices];
+
+// Address of the current device's scratch (looked up on its first use).
+static int fdk_scratch_ptr(void** out) {
+    int dev = 0;
+    CT2_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("ct2_fdk_sino_filter3d_noalloc: device %d beyond the %d supported", dev,
+                  kMaxDevices);
+        return CT2_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        CT2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(fdk_scratch)));
+        if (!p) { set_error("ct2_fdk_sino_filter3d_noalloc: no scratch address"); return CT2_EHIP; }
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return CT2_OK;
+}
+
 }  // namespace ct2
 
 using namespace ct2;
@@ -295,6 +336,25 @@ int ct2_fdk_sino_filter3d(float dp, int dim, int ns, int nu, int nv, const float
     if (rc) return rc;
     CT2_HIP(f);
     return CT2_OK;
+}
+
+int ct2_fdk_sino_filter3d_noalloc(float dp, int dim, int ns, int nu, int nv,
+                                  const float* sinomat, float* out, int hann, float L,
+                                  ct2_stream_t stream) {
+    const char* who = "ct2_fdk_sino_filter3d_noalloc";
+    int npad = 0;
+    int rc = fdk_filter_check(who, dp, dim, ns, nu, nv, sinomat, out, &npad);
+    if (rc) return rc;
+    if (npad > CT2_FDK_NOALLOC_MAX_NPAD) {
+        set_error("%s: lines of %d bins pad to %d samples, beyond the %d of the static "
+                  "scratch (use ct2_fdk_sino_filter3d)", who, dim == 1 ? nu : nv, npad,
+                  CT2_FDK_NOALLOC_MAX_NPAD);
+        return CT2_EINVAL;
+    }
+    void* ws = nullptr;
+    if ((rc = fdk_scratch_ptr(&ws))) return rc;
+    return fdk_filter_run(dp, dim, ns, nu, nv, npad, sinomat, out, hann, L, ws,
+                          (hipStream_t)stream);
 }
 
 int ct2_fdk_backproj_linear3d(int ns, int nu, int nv, float du, float dv, float u0, float v0,

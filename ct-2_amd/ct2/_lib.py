@@ -85,6 +85,7 @@ SIGNATURES = {
     "ct2_stencil_art": "PPPfPPPPPS",
     "ct2_fdk_sino_weight3d": "ffiiiffffffPPS",
     "ct2_fdk_sino_filter3d": "fiiiiPPifS",
+    "ct2_fdk_sino_filter3d_noalloc": "fiiiiPPifS",
     "ct2_fdk_backproj_linear3d": "iii" + "f" * 8 + "dd" + "iii" + "f" * 9 + "PPS",
 }
 

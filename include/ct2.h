@@ -325,6 +325,17 @@ int ct2_fdk_sino_weight3d(float radius, float center2detector, int ns, int nu, i
  * stream); lines up to 32768 bins. */
 int ct2_fdk_sino_filter3d(float dp, int dim, int ns, int nu, int nv, const float* sinomat,
                           float* out, int hann, float L, ct2_stream_t stream);
+/* ct2_fdk_sino_filter3d without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).  No
+ * counterpart in the reference.  Same arguments and result; the taps are
+ * built in a static device array that the library holds on every GPU it runs
+ * on, so lines may pad to at most CT2_FDK_NOALLOC_MAX_NPAD samples (npad =
+ * 2^round(log2(2n-1)): n <= 5792 bins); longer lines are rejected with
+ * CT2_EINVAL.  Calls that overlap on one GPU share that array and race. */
+#define CT2_FDK_NOALLOC_MAX_NPAD 8192
+int ct2_fdk_sino_filter3d_noalloc(float dp, int dim, int ns, int nu, int nv,
+                                  const float* sinomat, float* out, int hann, float L,
+                                  ct2_stream_t stream);
 /* FDK.py:130 fdk_backproj_linear3D over the voxel grid
  * x = x00+(i+xoff)dx, y = y00+(j+yoff)dy, z = z00+(k+zoff)dz, angles
  * vs[p] = s0 + p*ds; recon[i][j][k] (C order over x, y, z). */
