@@ -48,39 +48,6 @@ __device__ __forceinline__ int div_small(int x, int d, float rd) {
     return q;
 }
 
-// Detector footprint of an axis-aligned box for view p: rays from S through
-// every corner hit the detector plane D + u e_u + v e_v; returns the bin
-// range [jlo,jhi] x [klo,khi] (inclusive, clamped) or the full detector when
-// the box is not entirely in front of the source.  FP64, once per view per
-// workgroup.
-__device__ void project_corner(const float* __restrict__ f, const double P[3], double& u,
-                               double& v, int& ok) {
-    double S[3] = {f[0], f[1], f[2]};
-    double D[3] = {f[6], f[7], f[8]};
-    double eu[3] = {f[9], f[10], f[11]};
-    double ev[3] = {f[12], f[13], f[14]};
-    double n[3] = {eu[1] * ev[2] - eu[2] * ev[1], eu[2] * ev[0] - eu[0] * ev[2],
-                   eu[0] * ev[1] - eu[1] * ev[0]};
-    double dp[3] = {P[0] - S[0], P[1] - S[1], P[2] - S[2]};
-    double ds[3] = {D[0] - S[0], D[1] - S[1], D[2] - S[2]};
-    double den = dp[0] * n[0] + dp[1] * n[1] + dp[2] * n[2];
-    double num = ds[0] * n[0] + ds[1] * n[1] + ds[2] * n[2];
-    if (fabs(den) < 1e-12 * (fabs(num) + 1e-30)) { ok = 0; return; }
-    double t = num / den;
-    if (!(t > 0.0)) { ok = 0; return; }
-    double X[3] = {S[0] + t * dp[0] - D[0], S[1] + t * dp[1] - D[1], S[2] + t * dp[2] - D[2]};
-    double guu = eu[0] * eu[0] + eu[1] * eu[1] + eu[2] * eu[2];
-    double gvv = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
-    double guv = eu[0] * ev[0] + eu[1] * ev[1] + eu[2] * ev[2];
-    double bu = X[0] * eu[0] + X[1] * eu[1] + X[2] * eu[2];
-    double bv = X[0] * ev[0] + X[1] * ev[1] + X[2] * ev[2];
-    double det = guu * gvv - guv * guv;
-    if (fabs(det) < 1e-30) { ok = 0; return; }
-    u = (bu * gvv - bv * guv) / det;
-    v = (bv * guu - bu * guv) / det;
-    ok = 1;
-}
-
 // Brick number bid of a volume tiled by bdim (nbr bricks per axis, z
 // fastest, so neighbouring workgroups share z).
 __device__ __forceinline__ Brick brick_of(const Grid& g, int bid, int3 bdim, int3 nbr) {
@@ -405,12 +372,15 @@ __device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, double* 
 // Accumulate A^T sino over views view0 + pv*vstride (pv < nviews) for the
 // voxels of brick br into the FP64 LDS accumulator acc (zeroed here).  Called
 // by the whole workgroup; ends with a barrier, after which acc holds the sums.
+// With skip_sep (fast arithmetic only), views that view_separable accepts are
+// left out: the column-separable kernel takes them.  Returns whether any view
+// was taken (the same in every thread).
 template <int INTERP, int ARITH>
-__device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nviews,
+__device__ bool brick_accumulate(const Grid& g, const Det& dt, int view0, int nviews,
                                  int vstride, bool zonly, const Brick& br, double* acc,
                                  const float* __restrict__ sino,
                                  const int8_t* __restrict__ ind,
-                                 const float* __restrict__ fv) {
+                                 const float* __restrict__ fv, bool skip_sep = false) {
     __shared__ int s_range[4];
     const BrickCtx bc = brick_ctx(g, br);
     const int* ls = bc.ls;
@@ -424,8 +394,19 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
         // candidates are done (the LDS accumulation is atomic)
         __shared__ int s_ranges[FP_BATCH][4];
         const int lane = threadIdx.x & 63;
+        bool any = false;
         for (int pv0 = 0; pv0 < nviews; pv0 += FP_BATCH) {
             const int nb = min(FP_BATCH, nviews - pv0);
+            // views the separable kernel takes: lane b classifies view b; every
+            // wavefront finds the same mask, so skipping a batch is uniform
+            unsigned long long sepm = 0ull;
+            if (skip_sep) {
+                const bool sb = lane < nb &&
+                                view_separable(fv + (size_t)(view0 + (pv0 + lane) * vstride) * 18, g, dt);
+                sepm = __ballot(sb);
+                if (__popcll(sepm) == nb) continue;
+            }
+            any = true;
             __syncthreads();   // the previous batch's readers of s_ranges are done
             view_footprints(dt, fv, view0, pv0, nb, vstride, bc, s_ranges);
             // lane b holds view b's footprint; each view's is taken from it
@@ -436,6 +417,7 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
                 fr2 = s_ranges[lane][2]; fr3 = s_ranges[lane][3];
             }
             for (int b = 0; b < nb; ++b) {
+                if ((sepm >> b) & 1ull) continue;
                 const int p = view0 + (pv0 + b) * vstride;
                 const int r0 = __builtin_amdgcn_readlane(fr0, b);
                 const int r1 = __builtin_amdgcn_readlane(fr1, b);
@@ -457,7 +439,7 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
             }
         }
         __syncthreads();
-        return;
+        return any;
     }
     for (int pv = 0; pv < nviews; ++pv) {
         const int p = view0 + pv * vstride;
@@ -544,16 +526,20 @@ __device__ void brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
         }
     }
     __syncthreads();
+    return nviews > 0;
 }
 
 // LDS bytes of a brick accumulator (FP64 slots, one-voxel halo)
 size_t brick_lds_bytes(int3 b) { return (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double); }
 
+// skip_sep: leave out the views view_separable accepts and ADD the brick's
+// sums to vol (after the column-separable kernels wrote it), if any view
+// was taken; requires nsplit = 1 and an f32 output.
 template <int INTERP, int ARITH, typename OT>
 __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det dt, int view0,
                                                            int nviews, int vstride, int zonly,
                                                            int3 bdim, int3 nbr, int nsplit,
-                                                           OT* __restrict__ vol,
+                                                           int skip_sep, OT* __restrict__ vol,
                                                            const float* __restrict__ sino,
                                                            const int8_t* __restrict__ ind,
                                                            const float* __restrict__ fv) {
@@ -562,8 +548,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det
     const int part = blockIdx.x % nsplit;
     const Brick br = brick_of(g, blockIdx.x / nsplit, bdim, nbr);
     const int pv_lo = part * nviews / nsplit, pv_hi = (part + 1) * nviews / nsplit;
-    brick_accumulate<INTERP, ARITH>(g, dt, view0 + pv_lo * vstride, pv_hi - pv_lo, vstride,
-                                    zonly != 0, br, acc, sino, ind, fv);
+    const bool any = brick_accumulate<INTERP, ARITH>(g, dt, view0 + pv_lo * vstride,
+                                                     pv_hi - pv_lo, vstride, zonly != 0, br, acc,
+                                                     sino, ind, fv, skip_sep != 0);
+    if (skip_sep && !any) return;   // uniform: vol already holds these voxels' sums
     // write the brick: z-rows are contiguous in both LDS and HBM
     const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
     for (int q = threadIdx.x; q < nloc; q += BRICK_THREADS) {
@@ -575,6 +563,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det
         if constexpr (std::is_same<OT, float>::value) {
             if (nsplit > 1) {
                 atomicAdd(vol + gi, (float)acc[brick_slot(br, lx, ly, lz)]);
+                continue;
+            }
+            if (skip_sep) {   // one workgroup per brick: a plain add
+                vol[gi] += (float)acc[brick_slot(br, lx, ly, lz)];
                 continue;
             }
         }
@@ -768,12 +760,29 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
     // commute, so the result stays deterministic.  Shorter workgroups shrink
     // the kernel's tail, where bricks drain and the CUs run part-empty
     // (256^3 x 360 views: 55.2 -> 52.9 ms).
+    //
+    // Fast linear arithmetic into f32 with e_v along z and no brick named by
+    // the caller: the column-separable kernels store the views
+    // view_separable accepts, then one workgroup per brick adds any other
+    // view (stream order fixes the order of the adds).
+    if constexpr (ARITH == CT2_ARITH_FAST && INTERP == CT2_LINEAR &&
+                  std::is_same<OT, float>::value) {
+        if (sep_route(d, g)) {
+            int rc = launch_backproject_sep(g, dt, vs, (float*)vol, sino, ind, fv, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), dim3(nb.x * nb.y * nb.z),
+                               dim3(BRICK_THREADS), lds, st, g, dt, vs.view0, vs.nviews,
+                               vs.vstride, zonly, b, nb, 1, 1, (OT*)vol, sino, ind, fv);
+            return CT2_OK;
+        }
+    }
     const int nsplit = (ARITH == CT2_ARITH_FAST && std::is_same<OT, float>::value) ? 2 : 1;
     if (nsplit > 1)
         hipLaunchKernelGGL(k_zero_strided, dim3(2048), dim3(256), 0, st, g, (float*)vol);
     dim3 grid(nb.x * nb.y * nb.z * nsplit);
     hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(BRICK_THREADS), lds, st, g, dt,
-                       vs.view0, vs.nviews, vs.vstride, zonly, b, nb, nsplit, (OT*)vol, sino, ind, fv);
+                       vs.view0, vs.nviews, vs.vstride, zonly, b, nb, nsplit, 0, (OT*)vol, sino,
+                       ind, fv);
     return CT2_OK;
 }
 
@@ -894,7 +903,9 @@ int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int
         set_error("ct2_backproject: unknown arith %d", d->arith);
         return CT2_EINVAL;
     }
-    if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0)
+    // (the separable route leaves the brick kernel only the other views:
+    // no brick is tuned for it)
+    if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 && !sep_route(d, g))
         return tuned_backproject(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
     return launch_with(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
 }

@@ -1,0 +1,450 @@
+// ct2_bp_sep.hip -- the adjoint of the fast Joseph projector, column by
+// column, for views whose detector rows run along z (view_separable).
+//
+// The reference scatters each ray's samples (emil_code/view_proj3D.f:
+// 1228-1251, :1355-1360).  With e_v parallel to z, the rays of one detector
+// column share their driving axis a (x or y) and the fast set-up of the other
+// transverse axis b, so on plane l of a every ray of the column samples b cell
+// ib(l) with weights 1-wb, wb (the forward's own fma(l, dub, ub0)), and only
+// the z cell coordinate differs from row to row, uc(k, l) = fma(l, duc_k,
+// uc0_k).  The adjoint of plane l then factorises: the column's data
+// resampled onto z with the forward's linear-interpolation weights,
+//     G(c) = sum_k td_k max(0, 1 - |uc(k,l) - c|),   td_k = trav_k datum_k,
+// lands on the two voxel rows (l, ib, :) and (l, ib+1, :) with weights 1-wb
+// and wb.
+//
+// The resampling uses prefix sums.  In exact arithmetic uc(k, l) = A(l) +
+// B(l) k is linear in the row (the ray's z slope is linear in the detector
+// v coordinate and the plane's distance is shared), with B(l) of one sign
+// over the planes; rows are stored in ascending z.  The rows of cell c (c <=
+// uc < c+1) are then the stored rows [r_c, r_c+1), r_c the first row whose uc
+// (the forward's FP32 value) reaches c: the linear model puts r_c within a
+// hundredth of a row of (c - A)/B, and one look at the forward's uc of the
+// nearest row settles it, so every row lands in the cell the forward samples
+// it in.  With FP64 prefix sums PS(r) = sum_{k<r} td_k and PK(r) = sum_{k<r}
+// td_k k, cell c receives S = PS(r_c+1) - PS(r_c) and
+//     H = sum td_k (uc_k - c) = (A - c) S + B (PK(r_c+1) - PK(r_c)),
+// its lower voxel S - H and its upper voxel H.  The weights use the exact
+// linear uc where the forward uses its FP32 rounding (about 1e-5 of a cell
+// apart, and of random sign), so the kernel is the transpose of the fast
+// forward up to rounding.
+//
+// One launch takes the x-driven columns of every separable view, a second the
+// y-driven ones.  A workgroup owns SEP_TA consecutive planes of the driving
+// axis a, SEP_TB cells of the other axis b, and every z; wavefront w owns
+// planes a0 + 2w and a0 + 2w + 1.  Lane l owns the CPL z cells [l*CPL,
+// (l+1)*CPL) and voxels; the upper part of its last cell moves to the next
+// lane with one DPP lane shift.  Sums accumulate in FP32 registers per
+// (plane, b cell, z cell), so no LDS or global atomics are needed, and each
+// voxel is written by one workgroup: the x launch stores and the y launch
+// adds, in stream order, so the result is deterministic.
+//
+// Each view's columns are taken SEP_CHUNK at a time, one per wavefront: the
+// wavefront sets the column up (FP64, the forward's expressions) and writes
+// its table to LDS: per row the forward's (uc0, duc) and the prefix sums.
+#include "ct2_internal.h"
+
+namespace ct2 {
+
+constexpr int SEP_PPW = 2;                      // planes per wavefront
+constexpr int SEP_WAVES = 4;
+constexpr int SEP_TA = SEP_PPW * SEP_WAVES;     // tile planes along a
+constexpr int SEP_TB = 8;                       // tile cells along b
+constexpr int SEP_THREADS = 64 * SEP_WAVES;
+constexpr int SEP_CHUNK = SEP_WAVES;            // detector columns per LDS table set
+constexpr int SEP_MAXV = 1024;                  // views per launch (one LDS word each)
+constexpr int SEP_FILLB = 4;                    // table rows per batch of loads
+
+// Per (column, tile plane): the linear model uc = A + B kk over stored rows
+// kk (FP64, and A's FP32 value and B's reciprocal), and the forward's b cell
+// and weight.
+struct SepPlane {
+    double A, B;
+    float Af, rB, wb;
+    int ib;
+};
+
+// Table slot of entry e: one spare slot after every 16, so that lanes whose
+// entries lie a multiple of 8 apart (CPL/B rows) spread over the LDS banks.
+__host__ __device__ __forceinline__ int sep_slot(int e) { return e + (e >> 4); }
+// LDS per column: the prefix pairs (PS, PK) of entries 0 .. nv, then the
+// position pairs (uc0, duc) of rows -1 .. nv as entries 0 .. nv+1 (the two
+// ends are sentinels below and above every row).
+__host__ __device__ inline int sep_pre_slots(int nv) { return sep_slot(nv) + 1; }
+__host__ __device__ inline size_t sep_col_bytes(int nv) {
+    return ((size_t)16 * sep_pre_slots(nv) + (size_t)8 * (sep_slot(nv + 1) + 1) + 15) &
+           ~(size_t)15;
+}
+size_t sep_lds_bytes(int nv) { return SEP_CHUNK * sep_col_bytes(nv); }
+
+// x from the lane CTRL names (DPP), 0 where that lane is out of range or
+// the row is masked off by RM.
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Inclusive prefix sum over the 64 lanes (every lane active).
+__device__ __forceinline__ double wave_scan_f64(double x) {
+    x += dpp_f64<0x111, 0xf>(x);   // row_shr:1
+    x += dpp_f64<0x112, 0xf>(x);   // row_shr:2
+    x += dpp_f64<0x114, 0xf>(x);   // row_shr:4
+    x += dpp_f64<0x118, 0xf>(x);   // row_shr:8
+    x += dpp_f64<0x142, 0xa>(x);   // row_bcast:15 into rows 1 and 3
+    x += dpp_f64<0x143, 0xc>(x);   // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
+// Detector columns whose rays can pass through the box x in [xa, xb],
+// y in [ya, yb] (every z): the box corners projected through the source
+// (u does not depend on z when e_v is parallel to z), one bin to spare,
+// clamped to the detector; jlo > jhi when the box's silhouette misses it.
+__device__ void sep_column_range(const float* __restrict__ f, const Det& dt, double xa,
+                                 double xb, double ya, double yb, int& jlo, int& jhi) {
+    double umin = 1e300, umax = -1e300;
+    bool ok = true;
+    for (int c = 0; c < 4; ++c) {
+        const double P[3] = {(c & 1) ? xb : xa, (c & 2) ? yb : ya, 0.0};
+        double u = 0, v = 0;
+        int okc = 0;
+        project_corner(f, P, u, v, okc);
+        ok &= okc != 0;
+        umin = fmin(umin, u);
+        umax = fmax(umax, u);
+    }
+    jlo = 0;
+    jhi = dt.nu - 1;
+    if (!ok) return;
+    double jl = (umin - dt.u0) / dt.du - 0.5, jh = (umax - dt.u0) / dt.du - 0.5;
+    if (dt.du < 0) { const double t = jl; jl = jh; jh = t; }
+    jl = fmin(fmax(jl, -2.0), (double)dt.nu + 1.0);
+    jh = fmin(fmax(jh, -2.0), (double)dt.nu + 1.0);
+    jlo = max(0, (int)ceil(jl) - 1);
+    jhi = min(dt.nu - 1, (int)floor(jh) + 1);
+}
+
+template <int AX, int CPL>
+__global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view0, int nviews,
+                                                        int vstride, int add,
+                                                        float* __restrict__ vol,
+                                                        const float* __restrict__ sino,
+                                                        const int8_t* __restrict__ ind,
+                                                        const float* __restrict__ fv) {
+    constexpr int A = AX, Bx = 1 - AX;   // driving axis, other transverse axis
+    extern __shared__ double s_tab[];    // SEP_CHUNK column tables
+    __shared__ SepPlane s_pl[SEP_CHUNK][SEP_TA];
+    __shared__ unsigned s_mask[SEP_CHUNK];  // bit q: tile plane q samples the tile
+    __shared__ unsigned s_view[SEP_MAXV];   // per view: jlo | jhi << 16, or ~0u
+    const int nv = dt.nv;
+    const size_t colb = sep_col_bytes(nv);
+    const int nta = (g.n[A] + SEP_TA - 1) / SEP_TA;
+    const int P0 = (blockIdx.x % nta) * SEP_TA, Q0 = (blockIdx.x / nta) * SEP_TB;
+    const int nP = min(SEP_TA, g.n[A] - P0), nQ = min(SEP_TB, g.n[Bx] - Q0);
+    const int i0 = A == 0 ? P0 : Q0, j0 = A == 0 ? Q0 : P0;
+    const int ni = A == 0 ? nP : nQ, nj = A == 0 ? nQ : nP;
+    const int nb = g.n[Bx], nc = g.n[2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the fast set-up's grid constants (setup_fast_ax), FP64
+    const double da = (double)g.d[A], oa = (double)g.o[A];
+    const double ob = (double)g.o[Bx], oc = (double)g.o[2];
+    const double idb = rcp_f64((double)g.d[Bx]), idc = rcp_f64((double)g.d[2]);
+    const float gxa = fabsf(g.d[0]), gya = fabsf(g.d[1]);
+    const float trav_a = fabsf(g.d[A]);
+    // this lane's z cells c0 .. c0+CPL-1 (cell c is the forward's corner
+    // pair (c, c+1), valid for 0 <= c <= nc-2)
+    const int c0 = lane * CPL;
+    // stored rows per lane in the table fill (odd, so that the lanes' LDS
+    // rows fall in different banks)
+    const int R = ((nv + 63) >> 6) | 1;
+
+    // the tile's cell box: sample points whose cells touch the tile
+    const double xa = (double)g.o[0] + ((double)(i0 - 1) + 0.5) * g.d[0];
+    const double xb = (double)g.o[0] + ((double)(i0 + ni) + 0.5) * g.d[0];
+    const double ya = (double)g.o[1] + ((double)(j0 - 1) + 0.5) * g.d[1];
+    const double yb = (double)g.o[1] + ((double)(j0 + nj) + 0.5) * g.d[1];
+    // prologue, before the accumulators live: which views are separable and
+    // which detector columns can reach the tile in each
+    for (int pv = threadIdx.x; pv < nviews; pv += SEP_THREADS) {
+        const float* f = fv + (size_t)(view0 + pv * vstride) * 18;
+        unsigned w = ~0u;
+        if (view_separable(f, g, dt)) {
+            int jl, jh;
+            sep_column_range(f, dt, fmin(xa, xb), fmax(xa, xb), fmin(ya, yb), fmax(ya, yb), jl,
+                             jh);
+            // 0 <= jl, jh < nu < 2^16; an empty range skips the view here
+            if (jl <= jh) w = (unsigned)jl | ((unsigned)jh << 16);
+        }
+        s_view[pv] = w;
+    }
+    __syncthreads();
+
+    float acc[SEP_PPW][SEP_TB][CPL];
+#pragma unroll
+    for (int pl = 0; pl < SEP_PPW; ++pl)
+#pragma unroll
+        for (int q = 0; q < SEP_TB; ++q)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) acc[pl][q][c] = 0.0f;
+
+    for (int pv = 0; pv < nviews; ++pv) {
+        const int p = view0 + pv * vstride;
+        const float* f = fv + (size_t)p * 18;
+        const unsigned vw = (unsigned)__builtin_amdgcn_readfirstlane((int)s_view[pv]);
+        if (vw == ~0u) continue;   // not separable (the brick kernel takes it), or no column reaches the tile
+        const int jlo = (int)(vw & 0xffffu), jhi = (int)(vw >> 16);
+        const double Sa = f[A], Sb = f[Bx], Sc = f[2];
+        const double ta = fma(0.5, da, oa) - Sa;   // plane 0 centre from the source
+        const double ScOc = Sc - oc;
+        for (int cb = jlo; cb <= jhi; cb += SEP_CHUNK) {
+            __syncthreads();   // the previous chunk's samples are done with the tables
+            // ---- wavefront w: column cb + w, its set-up and its table ----
+            {
+                const int j = cb + wave;
+                double2* pre = reinterpret_cast<double2*>(reinterpret_cast<char*>(s_tab) +
+                                                          wave * colb);
+                float2* pos = reinterpret_cast<float2*>(pre + sep_pre_slots(nv));
+                unsigned mask = 0u;
+                if (j <= jhi && j < dt.nu) {
+                    const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
+                    // x and y of the bin do not depend on the row (e_v has no x, y)
+                    const double dd0 = fma((double)f[9], u, (double)f[6]) - (double)f[0];
+                    const double dd1 = fma((double)f[10], u, (double)f[7]) - (double)f[1];
+                    // the fast set-up's axis choice (fast_axis; z never drives here)
+                    const bool xy = fabsf((float)dd0) * gya > fabsf((float)dd1) * gxa;
+                    if ((xy ? 0 : 1) == A) {
+                        const double dda = A == 0 ? dd0 : dd1, ddb = A == 0 ? dd1 : dd0;
+                        const double ia = rcp_f64(dda);
+                        const double r1 = ddb * ia;
+                        const float ub0 = (float)fma(fma(r1, ta, Sb) - ob, idb, -0.5);
+                        const float dub = (float)(r1 * (da * idb));
+                        const float r1f = (float)r1;
+                        const float r1sq = 1.0f + r1f * r1f;
+                        const double X2 = fma((double)f[11], u, (double)f[8]);
+                        const double v0c = fma(0.5, (double)dt.dv, (double)dt.v0);
+                        // z slope of row k: R0 + R1 k (real arithmetic)
+                        const double R0 = (fma((double)f[14], v0c, X2) - Sc) * ia;
+                        const double R1 = (double)f[14] * (double)dt.dv * ia;
+                        // rows in ascending z: B(l) has one sign over the planes
+                        const bool rev = R1 * fma((double)P0, da, ta) * idc < 0.0;
+                        // lane q < nP: tile plane P0 + q
+                        const int lo = max(Q0 - 1, 0), hi = min(Q0 + nQ - 1, nb - 2);
+                        bool ok = false;
+                        if (lane < nP) {
+                            const int l = P0 + lane;
+                            const float ubl = __builtin_fmaf((float)l, dub, ub0);
+                            SepPlane sp;
+                            sp.ib = floor_i(ubl);
+                            sp.wb = frac_pos(ubl);
+                            ok = sp.ib >= lo && sp.ib <= hi;
+                            // uc of row k on plane l: A(l) + B(l) k, in z cells
+                            const double T = fma((double)l, da, ta);
+                            sp.A = fma(fma(R0, T, ScOc), idc, -0.5);
+                            sp.B = R1 * T * idc;
+                            if (rev) {   // stored row kk is row nv-1-kk
+                                sp.A = fma(sp.B, (double)(nv - 1), sp.A);
+                                sp.B = -sp.B;
+                            }
+                            sp.Af = (float)sp.A;
+                            sp.rB = __builtin_amdgcn_rcpf((float)sp.B);
+                            if (ok) s_pl[wave][lane] = sp;
+                        }
+                        mask = (unsigned)__ballot(ok);
+                        if (mask != 0u) {
+                            // the table: lane takes stored rows [lane*R, lane*R + R)
+                            const size_t rbase = ((size_t)p * dt.nu + j) * nv;
+                            const int kb = lane * R;
+                            double s0 = 0.0, s1 = 0.0;
+                            // SEP_FILLB rows at a time: their loads first (rows
+                            // past the end read row nv-1 and are dropped)
+                            for (int i0 = 0; i0 < R; i0 += SEP_FILLB) {
+                                float dat[SEP_FILLB];
+                                int use[SEP_FILLB];
+#pragma unroll
+                                for (int u = 0; u < SEP_FILLB; ++u) {
+                                    const int kk = min(kb + i0 + u, nv - 1);
+                                    const size_t r = rbase + (rev ? nv - 1 - kk : kk);
+                                    dat[u] = sino[r];
+                                    use[u] = ind == nullptr ? 1 : (int)ind[r];
+                                }
+#pragma unroll
+                                for (int u = 0; u < SEP_FILLB; ++u) {
+                                    const int kk = kb + i0 + u;
+                                    if (i0 + u >= R || kk >= nv) continue;
+                                    const int k = rev ? nv - 1 - kk : kk;
+                                    const double v = fma((double)k + 0.5, (double)dt.dv, (double)dt.v0);
+                                    const double r2 = (fma((double)f[14], v, X2) - Sc) * ia;
+                                    // setup_fast_ax's c-axis start and step of this ray
+                                    float2 pc;
+                                    pc.x = (float)fma(fma(r2, ta, Sc) - oc, idc, -0.5);
+                                    pc.y = (float)(r2 * (da * idc));
+                                    float val = 0.0f;
+                                    if (use[u] == 1) {
+                                        const float r2f = (float)r2;
+                                        // the forward's trav (setup_fast_ax), times the datum
+                                        val = trav_a * fast_sqrt(r1sq + r2f * r2f) * dat[u];
+                                    }
+                                    const double td = (double)val;
+                                    s0 += td;
+                                    s1 = fma(td, (double)kk, s1);
+                                    pre[sep_slot(kk + 1)] = make_double2(s0, s1);
+                                    pos[sep_slot(kk + 1)] = pc;
+                                }
+                            }
+                            // this lane's rows follow the sums of the lanes below
+                            const double e0 = wave_scan_f64(s0) - s0;
+                            const double e1 = wave_scan_f64(s1) - s1;
+                            for (int i = 0; i < R; ++i) {
+                                const int kk = kb + i;
+                                if (kk >= nv) break;
+                                double2 t = pre[sep_slot(kk + 1)];
+                                t.x += e0;
+                                t.y += e1;
+                                pre[sep_slot(kk + 1)] = t;
+                            }
+                            if (lane == 0) {
+                                pre[0] = make_double2(0.0, 0.0);
+                                pos[0] = make_float2(-1e30f, 0.0f);
+                                pos[sep_slot(nv + 1)] = make_float2(1e30f, 0.0f);
+                            }
+                        }
+                    }
+                }
+                if (lane == 0) s_mask[wave] = mask;
+            }
+            __syncthreads();
+            // ---- samples on this wavefront's planes, column by column ----
+            for (int c = 0; c < SEP_CHUNK; ++c) {
+                const unsigned mask = (unsigned)__builtin_amdgcn_readfirstlane((int)s_mask[c]);
+                const double2* pre = reinterpret_cast<const double2*>(
+                    reinterpret_cast<const char*>(s_tab) + c * colb);
+                const float2* pos = reinterpret_cast<const float2*>(pre + sep_pre_slots(nv));
+#pragma unroll
+                for (int pl = 0; pl < SEP_PPW; ++pl) {
+                    const int q = wave * SEP_PPW + pl;
+                    if (!((mask >> q) & 1u)) continue;   // uniform
+                    const SepPlane& sp = s_pl[c][q];
+                    const double Ad = sp.A, Bd = sp.B;
+                    const float Af = uniform_f(sp.Af), rB = uniform_f(sp.rB);
+                    const float wb = uniform_f(sp.wb);
+                    const int ib = __builtin_amdgcn_readfirstlane(sp.ib);
+                    const float lf = (float)(P0 + q);
+                    // cell boundaries z = c0 .. c0+CPL (at most nc-1, so the
+                    // cells from nc-1 up, which the forward never samples, are
+                    // empty): the first stored row whose uc reaches z, and the
+                    // prefix sums there
+                    double PS[CPL + 1], PK[CPL + 1];
+#pragma unroll
+                    for (int bq = 0; bq <= CPL; ++bq) {
+                        const float zf = (float)min(c0 + bq, nc - 1);
+                        const float kf = (zf - Af) * rB;
+                        const int m = (int)floorf(fminf(fmaxf(kf + 0.5f, -1.0f), (float)nv));
+                        const float2 pm = pos[sep_slot(m + 1)];
+                        const int r = m + (__builtin_fmaf(lf, pm.y, pm.x) < zf ? 1 : 0);
+                        const double2 P = pre[sep_slot(r)];
+                        PS[bq] = P.x;
+                        PK[bq] = P.y;
+                    }
+                    double lo[CPL], hi[CPL];
+#pragma unroll
+                    for (int cc = 0; cc < CPL; ++cc) {
+                        const double S = PS[cc + 1] - PS[cc];
+                        const double H = fma(Bd, PK[cc + 1] - PK[cc], (Ad - (double)(c0 + cc)) * S);
+                        lo[cc] = S - H;
+                        hi[cc] = H;
+                    }
+                    // voxel c0+cc: lower part of cell c0+cc, upper part of
+                    // the cell below (the previous lane's last cell for cc = 0)
+                    const double carry = dpp_f64<0x138, 0xf>(hi[CPL - 1]);   // wave_shr:1
+                    // b rows of the tile: ib takes 1-wb, ib+1 takes wb, the
+                    // others 0 (uniform weights, so every row is one FMA with
+                    // a scalar operand)
+                    const float w0 = uniform_f(1.0f - wb), w1 = uniform_f(wb);
+                    const int jj = ib - Q0;
+                    float wr[SEP_TB];
+#pragma unroll
+                    for (int bi = 0; bi < SEP_TB; ++bi)
+                        wr[bi] = bi == jj ? w0 : (bi == jj + 1 ? w1 : 0.0f);
+#pragma unroll
+                    for (int cc = 0; cc < CPL; ++cc) {
+                        const float G = (float)(lo[cc] + (cc == 0 ? carry : hi[cc - 1]));
+#pragma unroll
+                        for (int bi = 0; bi < SEP_TB; ++bi)
+                            acc[pl][bi][cc] = __builtin_fmaf(wr[bi], G, acc[pl][bi][cc]);
+                    }
+                }
+            }
+        }
+    }
+    // ---- write-back: this wavefront's planes, every b cell, CPL z per lane ----
+#pragma unroll
+    for (int pl = 0; pl < SEP_PPW; ++pl) {
+        const int q = wave * SEP_PPW + pl;
+        if (q >= nP) continue;
+        const int l = P0 + q;
+#pragma unroll
+        for (int bi = 0; bi < SEP_TB; ++bi) {
+            if (bi >= nQ) continue;
+            const int i = A == 0 ? l : Q0 + bi;
+            const int j = A == 0 ? Q0 + bi : l;
+#pragma unroll
+            for (int cc = 0; cc < CPL; ++cc) {
+                const int k = c0 + cc;
+                if (k >= nc) continue;
+                const size_t gi = (size_t)i * g.s[0] + (size_t)j * g.s[1] + (size_t)k * g.s[2];
+                const float v = acc[pl][bi][cc];
+                vol[gi] = add ? vol[gi] + v : v;
+            }
+        }
+    }
+}
+
+// The column-separable adjoint of the fast forward over the view subset:
+// every voxel of vol is written (x-driven columns) and then added to
+// (y-driven columns).  Views that fail view_separable contribute nothing
+// here; the caller adds them with the brick kernel.  Requires linear
+// sampling, f32 output, the default C-order strides, nz <= 256, nu < 65536
+// and column tables within 60 KiB of LDS (nv <= 638).
+bool sep_supported(const ct2_desc* d, const Grid& g) {
+    return d->arith == CT2_ARITH_FAST && d->interp == CT2_LINEAR && d->vol_dtype == CT2_F32 &&
+           d->axis == CT2_AXIS_FREE && g.n[2] <= 256 && d->nv >= 1 &&
+           sep_lds_bytes(d->nv) <= 60 * 1024 && d->nu < 65536 && g.s[2] == 1 &&
+           g.s[1] == g.n[2] && g.s[0] == g.n[1] * g.n[2];
+}
+
+template <int CPL>
+static int launch_sep_cpl(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
+                          const float* sino, const int8_t* ind, const float* fv, hipStream_t st) {
+    const size_t lds = sep_lds_bytes(dt.nv);
+    const int tx = (g.n[0] + SEP_TA - 1) / SEP_TA, ty = (g.n[1] + SEP_TB - 1) / SEP_TB;
+    const int ux = (g.n[0] + SEP_TB - 1) / SEP_TB, uy = (g.n[1] + SEP_TA - 1) / SEP_TA;
+    // SEP_MAXV views per launch pair; the first x launch stores, every
+    // later launch adds (stream order fixes the order of the additions)
+    for (int v = 0; v < vs.nviews; v += SEP_MAXV) {
+        const int nv = min(SEP_MAXV, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
+        hipLaunchKernelGGL((k_bp_sep<0, CPL>), dim3(tx * ty), dim3(SEP_THREADS), lds, st, g, dt,
+                           v0, nv, vs.vstride, v > 0 ? 1 : 0, vol, sino, ind, fv);
+        hipLaunchKernelGGL((k_bp_sep<1, CPL>), dim3(ux * uy), dim3(SEP_THREADS), lds, st, g, dt,
+                           v0, nv, vs.vstride, 1, vol, sino, ind, fv);
+    }
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
+                           const float* sino, const int8_t* ind, const float* fv,
+                           hipStream_t st) {
+    const int cpl = (g.n[2] + 63) / 64;
+    switch (cpl) {
+        case 1: return launch_sep_cpl<1>(g, dt, vs, vol, sino, ind, fv, st);
+        case 2: return launch_sep_cpl<2>(g, dt, vs, vol, sino, ind, fv, st);
+        case 3: return launch_sep_cpl<3>(g, dt, vs, vol, sino, ind, fv, st);
+        case 4: return launch_sep_cpl<4>(g, dt, vs, vol, sino, ind, fv, st);
+    }
+    set_error("separable adjoint: nz = %d beyond 256", g.n[2]);
+    return CT2_EINVAL;
+}
+
+}  // namespace ct2

@@ -253,6 +253,12 @@ __device__ __forceinline__ double rcp_f64(double x) {
     return r * fma(-x, r, 2.0);
 }
 
+// Square root of the fast set-up's ray weight: the hardware v_sqrt_f32
+// (within 1 ulp), one instruction where sqrtf's correctly rounded sequence
+// takes about twenty; every fast-arithmetic kernel weighs rays with it, so
+// their matrix entries agree.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Driving axis of direction dd in the fast set-up: the largest |dd_q| / d_q,
 // compared cross-multiplied (z when the policy forces it).
 __device__ __forceinline__ int fast_axis(const Grid& g, double dd0, double dd1, double dd2,
@@ -298,7 +304,7 @@ __device__ __forceinline__ void setup_fast_ax(const Grid& g, const double S[3], 
     R.dub = (float)(r1 * ((double)da * idb));
     R.duc = (float)(r2 * ((double)da * idc));
     const float r1f = (float)r1, r2f = (float)r2;
-    R.trav = fabsf(da) * sqrtf(1.0f + r1f * r1f + r2f * r2f);
+    R.trav = fabsf(da) * fast_sqrt(1.0f + r1f * r1f + r2f * r2f);
     R.a = a; R.b = b; R.c = c;
     R.na = sel3i(a, g.n[0], g.n[1], g.n[2]);
     R.nb = sel3i(b, g.n[0], g.n[1], g.n[2]);
@@ -360,6 +366,124 @@ __device__ __forceinline__ void view_ray_frame(const ViewFrame& F, int j, int k,
         S[q] = F.s[q];
         B[q] = fma((double)F.ev[q], v, fma((double)F.eu[q], u, (double)F.d[q]));
     }
+}
+
+// Detector coordinates (u, v) of point P seen from the source of frame f:
+// where the ray from S through P meets the detector plane D + u e_u + v e_v.
+// ok = 0 when P is not in front of the source or the ray runs parallel to
+// the plane.  FP64.
+__device__ inline void project_corner(const float* __restrict__ f, const double P[3], double& u,
+                                      double& v, int& ok) {
+    double S[3] = {f[0], f[1], f[2]};
+    double D[3] = {f[6], f[7], f[8]};
+    double eu[3] = {f[9], f[10], f[11]};
+    double ev[3] = {f[12], f[13], f[14]};
+    double n[3] = {eu[1] * ev[2] - eu[2] * ev[1], eu[2] * ev[0] - eu[0] * ev[2],
+                   eu[0] * ev[1] - eu[1] * ev[0]};
+    double dp[3] = {P[0] - S[0], P[1] - S[1], P[2] - S[2]};
+    double ds[3] = {D[0] - S[0], D[1] - S[1], D[2] - S[2]};
+    double den = dp[0] * n[0] + dp[1] * n[1] + dp[2] * n[2];
+    double num = ds[0] * n[0] + ds[1] * n[1] + ds[2] * n[2];
+    if (fabs(den) < 1e-12 * (fabs(num) + 1e-30)) { ok = 0; return; }
+    double t = num / den;
+    if (!(t > 0.0)) { ok = 0; return; }
+    double X[3] = {S[0] + t * dp[0] - D[0], S[1] + t * dp[1] - D[1], S[2] + t * dp[2] - D[2]};
+    double guu = eu[0] * eu[0] + eu[1] * eu[1] + eu[2] * eu[2];
+    double gvv = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
+    double guv = eu[0] * ev[0] + eu[1] * ev[1] + eu[2] * ev[2];
+    double bu = X[0] * eu[0] + X[1] * eu[1] + X[2] * eu[2];
+    double bv = X[0] * ev[0] + X[1] * ev[1] + X[2] * ev[2];
+    double det = guu * gvv - guv * guv;
+    if (fabs(det) < 1e-30) { ok = 0; return; }
+    u = (bu * gvv - bv * guv) / det;
+    v = (bv * guu - bu * guv) / det;
+    ok = 1;
+}
+
+// ---- column-separable views (the adjoint's fast route) ----
+// With e_v exactly parallel to z, the rays of one detector column differ
+// only in their z direction component: they share the driving axis a (x or
+// y) and the per-ray set-up of the other transverse axis b, so on every
+// plane of a all of them sample the same b cell with the same weight, and
+// their z coordinate is linear in the row index.  view_separable says whether
+// every ray of view F qualifies:
+//   (1) e_v = (0, 0, +-1 or any nonzero);
+//   (2) no ray is z-driven: the largest |dz|/|d_z| over the detector corners
+//       stays below the smallest max(|dx|/|d_x|, |dy|/|d_y|) over the bin
+//       columns by a relative margin far above the rounding of the fast
+//       set-up's axis comparisons;
+//   (3) for x and for y, when that axis can drive some column, the source
+//       lies outside the span of its plane centres widened by one voxel, so
+//       a column's plane parameters (plane - source along a) share one sign;
+//   (4) a z cell holds at most 48 rows of a column on any plane.
+// It depends on the frame, grid and detector only, so every kernel that
+// calls it classifies a view the same way.
+__device__ __forceinline__ double sep_f(double ax, double bx, double ay, double by, double gx,
+                                        double gy, double u) {
+    return fmax(fabs(fma(u, bx, ax)) / gx, fabs(fma(u, by, ay)) / gy);
+}
+__device__ __forceinline__ double sep_h(double ax, double bx, double ay, double by, double gx,
+                                        double gy, double u) {
+    return fabs(fma(u, bx, ax)) / gx - fabs(fma(u, by, ay)) / gy;
+}
+
+__device__ inline bool view_separable(const float* __restrict__ f, const Grid& g, const Det& dt) {
+    if (f[12] != 0.0f || f[13] != 0.0f || f[14] == 0.0f) return false;
+    const double S0 = f[0], S1 = f[1], S2 = f[2];
+    const double ax = (double)f[6] - S0, bx = f[9];     // dx(u) = ax + u bx (e_v has no x)
+    const double ay = (double)f[7] - S1, by = f[10];
+    const double gx = fabs((double)g.d[0]), gy = fabs((double)g.d[1]), gz = fabs((double)g.d[2]);
+    double ua = fma(0.5, (double)dt.du, (double)dt.u0);
+    double ub = fma((double)dt.nu - 0.5, (double)dt.du, (double)dt.u0);
+    if (ua > ub) { const double t = ua; ua = ub; ub = t; }
+    double va = fma(0.5, (double)dt.dv, (double)dt.v0);
+    double vb = fma((double)dt.nv - 0.5, (double)dt.dv, (double)dt.v0);
+    double zmax = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        const double u = (c & 1) ? ub : ua, v = (c & 2) ? vb : va;
+        zmax = fmax(zmax, fabs((double)f[8] - S2 + u * f[11] + v * f[14]) / gz);
+    }
+    // candidate points of the piecewise-linear functions of u: the ends and
+    // the roots of dx, dy, dx/gx - dy/gy and dx/gx + dy/gy
+    double cand[6] = {ua, ub, ua, ua, ua, ua};
+    if (bx != 0.0) cand[2] = -ax / bx;
+    if (by != 0.0) cand[3] = -ay / by;
+    const double d1 = bx / gx - by / gy, d2 = bx / gx + by / gy;
+    if (d1 != 0.0) cand[4] = -(ax / gx - ay / gy) / d1;
+    if (d2 != 0.0) cand[5] = -(ax / gx + ay / gy) / d2;
+    double fmn = 1e300, hmax = -1e300, hmin = 1e300;
+    for (int c = 0; c < 6; ++c) {
+        const double u = fmin(fmax(cand[c], ua), ub);
+        fmn = fmin(fmn, sep_f(ax, bx, ay, by, gx, gy, u));
+        const double h = sep_h(ax, bx, ay, by, gx, gy, u);
+        hmax = fmax(hmax, h);
+        hmin = fmin(hmin, h);
+    }
+    if (!(zmax < fmn * (1.0 - 1e-5))) return false;
+    const double m = 1e-6 * fmn;
+    const bool xmay = hmax > -m, ymay = hmin < m;
+    double dist = 1e300;
+    for (int a = 0; a < 2; ++a) {
+        if (!(a == 0 ? xmay : ymay)) continue;
+        const double d = g.d[a], o = g.o[a];
+        double lo = o + 0.5 * d, hi = o + ((double)g.n[a] - 0.5) * d;
+        if (lo > hi) { const double t = lo; lo = hi; hi = t; }
+        const double Sa = a == 0 ? S0 : S1;
+        if (Sa >= lo - fabs(d) && Sa <= hi + fabs(d)) return false;
+        dist = fmin(dist, fmin(fabs(Sa - lo), fabs(Sa - hi)));
+    }
+    // (4) on every plane, the z spacing of a column's rows is at least 1/48
+    //     of a voxel: |e_v,z dv / d_z| times the plane's fraction of the ray,
+    //     which is at least the source's distance to the nearest plane centre
+    //     over the longest ray
+    double maxlen = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        const double u = (c & 1) ? ub : ua, v = (c & 2) ? vb : va;
+        const double e0 = ax + u * bx, e1 = ay + u * by;
+        const double e2 = (double)f[8] - S2 + u * f[11] + v * f[14];
+        maxlen = fmax(maxlen, sqrt(e0 * e0 + e1 * e1 + e2 * e2));
+    }
+    return fabs((double)f[14] * (double)dt.dv / (double)g.d[2]) * dist >= maxlen / 48.0;
 }
 
 // Restrict the inclusive plane range [l0,l1] to planes where the cell
@@ -571,6 +695,17 @@ int launch_ray_em(const ct2_desc* d, int strict, const float* sino, const int8_t
 size_t sart_work_floats(const ct2_desc* d);
 int launch_sart(const ct2_desc* d, float lambda, const float* sino, const int8_t* ind,
                 const float* fv, float* vol, float* work, hipStream_t st);
+// Column-separable adjoint (ct2_bp_sep.hip) and when it applies.
+bool sep_supported(const ct2_desc* d, const Grid& g);
+// The adjoint takes the column-separable route when sep_supported holds and
+// the caller names no brick (a brick shape selects the brick kernel, whose
+// weights are the fast forward's own FP32 values).
+inline bool sep_route(const ct2_desc* d, const Grid& g) {
+    return d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 && sep_supported(d, g);
+}
+int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
+                           const float* sino, const int8_t* ind, const float* fv,
+                           hipStream_t st);
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
                           hipStream_t st);
 int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,

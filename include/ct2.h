@@ -56,7 +56,10 @@ typedef struct ct2_desc {
     int arith;                         /* CT2_ARITH_EXACT | CT2_ARITH_FAST */
     int view0, nviews;                 /* view subset: views view0 + j*vstride, j < nviews; */
                                        /* nviews 0 = every such view up to ns-1          */
-    int brick[3];                      /* adjoint voxel brick (0 = autotuned/default) */
+    int brick[3];                      /* adjoint voxel brick (0 = autotuned/default); */
+                                       /* naming one also keeps the fast linear f32    */
+                                       /* adjoint on the brick kernel (exact transpose */
+                                       /* of the fast forward's FP32 weights)          */
     int vstride;                       /* view step of the subset (0 or 1 = contiguous) */
 } ct2_desc;
 

@@ -158,13 +158,16 @@ def _desc(geo, g2, **kw):
 
 @pytest.mark.parametrize("interp", [0, 1])
 def test_fast_pair_dot_product(interp):
-    """Fast-arithmetic forward and adjoint share their matrix entries."""
+    """Fast-arithmetic forward and the brick adjoint share their matrix
+    entries (a named brick keeps the adjoint on the brick kernel; the
+    column-separable route has its own tests)."""
     import ctypes as C
     import torch
     from ct2 import _dev
     from ct2._lib import call, CT2_ARITH_FAST
     ind, fv, geo, vol, g2 = cone_case(n=40, ns=12, nu=64, nv=56, du=1.0, dv=1.0)
     d = _desc(geo, g2, arith=CT2_ARITH_FAST, interp=interp)
+    d.brick[0], d.brick[1], d.brick[2] = 10, 10, 24
     rng = np.random.default_rng(31)
     x = torch.from_numpy(rng.standard_normal(vol.shape).astype(np.float32)).cuda()
     y = torch.from_numpy(rng.standard_normal(geo[:3]).astype(np.float32)).cuda()
