@@ -52,7 +52,7 @@ constexpr int SEP_TA = SEP_PPW * SEP_WAVES;     // tile planes along a
 constexpr int SEP_TB = 8;                       // tile cells along b
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
 constexpr int SEP_CHUNK = SEP_WAVES;            // detector columns per LDS table set
-constexpr int SEP_MAXV = 1024;                  // views per launch (one LDS word each)
+constexpr int SEP_MAXV = 256;                   // views per launch (one LDS word each)
 constexpr int SEP_FILLB = 4;                    // table rows per batch of loads
 
 // Per (column, tile plane): the linear model uc = A + B kk over stored rows
@@ -64,16 +64,14 @@ struct SepPlane {
     int ib;
 };
 
-// Table slot of entry e: one spare slot after every 16, so that lanes whose
-// entries lie a multiple of 8 apart (CPL/B rows) spread over the LDS banks.
-__host__ __device__ __forceinline__ int sep_slot(int e) { return e + (e >> 4); }
 // LDS per column: the prefix pairs (PS, PK) of entries 0 .. nv, then the
 // position pairs (uc0, duc) of rows -1 .. nv as entries 0 .. nv+1 (the two
-// ends are sentinels below and above every row).
-__host__ __device__ inline int sep_pre_slots(int nv) { return sep_slot(nv) + 1; }
+// ends are sentinels below and above every row).  (Lanes whose entries lie
+// a multiple of 8 apart share LDS banks; spreading them with spare slots
+// costs more address arithmetic than the conflicts cost: 37.9 against
+// 35.0 ms at 256^3 x 360 views.)
 __host__ __device__ inline size_t sep_col_bytes(int nv) {
-    return ((size_t)16 * sep_pre_slots(nv) + (size_t)8 * (sep_slot(nv + 1) + 1) + 15) &
-           ~(size_t)15;
+    return ((size_t)16 * (nv + 1) + (size_t)8 * (nv + 2) + 15) & ~(size_t)15;
 }
 size_t sep_lds_bytes(int nv) { return SEP_CHUNK * sep_col_bytes(nv); }
 
@@ -127,7 +125,7 @@ __device__ void sep_column_range(const float* __restrict__ f, const Det& dt, dou
 }
 
 template <int AX, int CPL>
-__global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view0, int nviews,
+__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_bp_sep(Grid g, Det dt, int view0, int nviews,
                                                         int vstride, int add,
                                                         float* __restrict__ vol,
                                                         const float* __restrict__ sino,
@@ -156,6 +154,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
     // this lane's z cells c0 .. c0+CPL-1 (cell c is the forward's corner
     // pair (c, c+1), valid for 0 <= c <= nc-2)
     const int c0 = lane * CPL;
+    const float c0f = (float)c0, ztop = (float)(nc - 1), nvh = (float)nv + 0.5f;
     // stored rows per lane in the table fill (odd, so that the lanes' LDS
     // rows fall in different banks)
     const int R = ((nv + 63) >> 6) | 1;
@@ -205,7 +204,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
                 const int j = cb + wave;
                 double2* pre = reinterpret_cast<double2*>(reinterpret_cast<char*>(s_tab) +
                                                           wave * colb);
-                float2* pos = reinterpret_cast<float2*>(pre + sep_pre_slots(nv));
+                float2* pos = reinterpret_cast<float2*>(pre + (nv + 1));
                 unsigned mask = 0u;
                 if (j <= jhi && j < dt.nu) {
                     const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
@@ -289,8 +288,8 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
                                     const double td = (double)val;
                                     s0 += td;
                                     s1 = fma(td, (double)kk, s1);
-                                    pre[sep_slot(kk + 1)] = make_double2(s0, s1);
-                                    pos[sep_slot(kk + 1)] = pc;
+                                    pre[kk + 1] = make_double2(s0, s1);
+                                    pos[kk + 1] = pc;
                                 }
                             }
                             // this lane's rows follow the sums of the lanes below
@@ -299,15 +298,15 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
                             for (int i = 0; i < R; ++i) {
                                 const int kk = kb + i;
                                 if (kk >= nv) break;
-                                double2 t = pre[sep_slot(kk + 1)];
+                                double2 t = pre[kk + 1];
                                 t.x += e0;
                                 t.y += e1;
-                                pre[sep_slot(kk + 1)] = t;
+                                pre[kk + 1] = t;
                             }
                             if (lane == 0) {
                                 pre[0] = make_double2(0.0, 0.0);
                                 pos[0] = make_float2(-1e30f, 0.0f);
-                                pos[sep_slot(nv + 1)] = make_float2(1e30f, 0.0f);
+                                pos[nv + 1] = make_float2(1e30f, 0.0f);
                             }
                         }
                     }
@@ -320,14 +319,16 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
                 const unsigned mask = (unsigned)__builtin_amdgcn_readfirstlane((int)s_mask[c]);
                 const double2* pre = reinterpret_cast<const double2*>(
                     reinterpret_cast<const char*>(s_tab) + c * colb);
-                const float2* pos = reinterpret_cast<const float2*>(pre + sep_pre_slots(nv));
+                const float2* pos = reinterpret_cast<const float2*>(pre + (nv + 1));
 #pragma unroll
                 for (int pl = 0; pl < SEP_PPW; ++pl) {
                     const int q = wave * SEP_PPW + pl;
                     if (!((mask >> q) & 1u)) continue;   // uniform
                     const SepPlane& sp = s_pl[c][q];
                     const double Ad = sp.A, Bd = sp.B;
-                    const float Af = uniform_f(sp.Af), rB = uniform_f(sp.rB);
+                    const float rB = uniform_f(sp.rB);
+                    // nearest row to (z - A) / B, as (z - A) / B + 1/2 truncated
+                    const float nAr = uniform_f(__builtin_fmaf(-sp.Af, rB, 0.5f));
                     const float wb = uniform_f(sp.wb);
                     const int ib = __builtin_amdgcn_readfirstlane(sp.ib);
                     const float lf = (float)(P0 + q);
@@ -338,12 +339,14 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
                     double PS[CPL + 1], PK[CPL + 1];
 #pragma unroll
                     for (int bq = 0; bq <= CPL; ++bq) {
-                        const float zf = (float)min(c0 + bq, nc - 1);
-                        const float kf = (zf - Af) * rB;
-                        const int m = (int)floorf(fminf(fmaxf(kf + 0.5f, -1.0f), (float)nv));
-                        const float2 pm = pos[sep_slot(m + 1)];
+                        const float zf = fminf(c0f + (float)bq, ztop);
+                        // m in [0, nv]: the nearest row, or 0 when that is row
+                        // -1 (whose uc lies below every z anyway)
+                        const int m = (int)__builtin_amdgcn_fmed3f(
+                            __builtin_fmaf(zf, rB, nAr), 0.0f, nvh);
+                        const float2 pm = pos[m + 1];
                         const int r = m + (__builtin_fmaf(lf, pm.y, pm.x) < zf ? 1 : 0);
-                        const double2 P = pre[sep_slot(r)];
+                        const double2 P = pre[r];
                         PS[bq] = P.x;
                         PK[bq] = P.y;
                     }
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(SEP_THREADS) void k_bp_sep(Grid g, Det dt, int view
                     // b rows of the tile: ib takes 1-wb, ib+1 takes wb, the
                     // others 0 (uniform weights, so every row is one FMA with
                     // a scalar operand)
-                    const float w0 = uniform_f(1.0f - wb), w1 = uniform_f(wb);
+                    const float w0 = 1.0f - wb, w1 = wb;
                     const int jj = ib - Q0;
                     float wr[SEP_TB];
 #pragma unroll
