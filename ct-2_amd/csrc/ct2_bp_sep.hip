@@ -52,7 +52,7 @@ constexpr int SEP_TA = SEP_PPW * SEP_WAVES;     // tile planes along a
 constexpr int SEP_TB = 8;                       // tile cells along b
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
 constexpr int SEP_CHUNK = SEP_WAVES;            // detector columns per LDS table set
-constexpr int SEP_MAXV = 256;                   // views per launch (one LDS word each)
+constexpr int SEP_MAXV = 512;                   // views per launch (one LDS word each)
 constexpr int SEP_FILLB = 4;                    // table rows per batch of loads
 
 // Per (column, tile plane): the linear model uc = A + B kk over stored rows
@@ -124,22 +124,26 @@ __device__ void sep_column_range(const float* __restrict__ f, const Det& dt, dou
     jhi = min(dt.nu - 1, (int)floor(jh) + 1);
 }
 
+// Write-back of a tile's sums: store, add, or add atomically (two atomic
+// adds onto a zeroed voxel commute, so that result is deterministic too).
+enum { SEP_STORE = 0, SEP_ADD = 1, SEP_ATOMIC = 2 };
+
+// One tile of driving axis AX: tile number `tile` of that axis's tiling.
+// s_tab: the dynamic LDS tables; s_pl, s_mask, s_view: the kernel's.
 template <int AX, int CPL>
-__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_bp_sep(Grid g, Det dt, int view0, int nviews,
-                                                        int vstride, int add,
-                                                        float* __restrict__ vol,
-                                                        const float* __restrict__ sino,
-                                                        const int8_t* __restrict__ ind,
-                                                        const float* __restrict__ fv) {
+__device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& dt, int view0,
+                                            int nviews, int vstride, int mode,
+                                            float* __restrict__ vol,
+                                            const float* __restrict__ sino,
+                                            const int8_t* __restrict__ ind,
+                                            const float* __restrict__ fv, double* s_tab,
+                                            SepPlane (*s_pl)[SEP_TA], unsigned* s_mask,
+                                            unsigned* s_view) {
     constexpr int A = AX, Bx = 1 - AX;   // driving axis, other transverse axis
-    extern __shared__ double s_tab[];    // SEP_CHUNK column tables
-    __shared__ SepPlane s_pl[SEP_CHUNK][SEP_TA];
-    __shared__ unsigned s_mask[SEP_CHUNK];  // bit q: tile plane q samples the tile
-    __shared__ unsigned s_view[SEP_MAXV];   // per view: jlo | jhi << 16, or ~0u
     const int nv = dt.nv;
     const size_t colb = sep_col_bytes(nv);
     const int nta = (g.n[A] + SEP_TA - 1) / SEP_TA;
-    const int P0 = (blockIdx.x % nta) * SEP_TA, Q0 = (blockIdx.x / nta) * SEP_TB;
+    const int P0 = (tile % nta) * SEP_TA, Q0 = (tile / nta) * SEP_TB;
     const int nP = min(SEP_TA, g.n[A] - P0), nQ = min(SEP_TB, g.n[Bx] - Q0);
     const int i0 = A == 0 ? P0 : Q0, j0 = A == 0 ? Q0 : P0;
     const int ni = A == 0 ? nP : nQ, nj = A == 0 ? nQ : nP;
@@ -398,15 +402,44 @@ __global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))
                 if (k >= nc) continue;
                 const size_t gi = (size_t)i * g.s[0] + (size_t)j * g.s[1] + (size_t)k * g.s[2];
                 const float v = acc[pl][bi][cc];
-                vol[gi] = add ? vol[gi] + v : v;
+                if (mode == SEP_ATOMIC)
+                    atomicAdd(vol + gi, v);
+                else
+                    vol[gi] = mode == SEP_ADD ? vol[gi] + v : v;
             }
         }
     }
 }
 
+// axes bit 0: the first ntx workgroups take x-driven tiles; bit 1: the
+// (other) workgroups take y-driven tiles.
+template <int CPL>
+__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_bp_sep(
+    Grid g, Det dt, int view0, int nviews, int vstride, int axes, int ntx, int mode,
+    float* __restrict__ vol, const float* __restrict__ sino, const int8_t* __restrict__ ind,
+    const float* __restrict__ fv) {
+    extern __shared__ double s_tab[];       // SEP_CHUNK column tables
+    __shared__ SepPlane s_pl[SEP_CHUNK][SEP_TA];
+    __shared__ unsigned s_mask[SEP_CHUNK];  // bit q: tile plane q samples the tile
+    __shared__ unsigned s_view[SEP_MAXV];   // per view: jlo | jhi << 16, or ~0u
+    const int b = (int)blockIdx.x;
+    if ((axes & 1) && b < ntx)
+        bp_sep_tile<0, CPL>(b, g, dt, view0, nviews, vstride, mode, vol, sino, ind, fv, s_tab,
+                            s_pl, s_mask, s_view);
+    else
+        bp_sep_tile<1, CPL>((axes & 1) ? b - ntx : b, g, dt, view0, nviews, vstride, mode, vol,
+                            sino, ind, fv, s_tab, s_pl, s_mask, s_view);
+}
+
+__global__ void k_sep_zero(float* __restrict__ vol, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        vol[i] = 0.0f;
+}
+
 // The column-separable adjoint of the fast forward over the view subset:
-// every voxel of vol is written (x-driven columns) and then added to
-// (y-driven columns).  Views that fail view_separable contribute nothing
+// every voxel of vol receives the sums of its x-driven tile and its y-driven
+// tile.  Views that fail view_separable contribute nothing
 // here; the caller adds them with the brick kernel.  Requires linear
 // sampling, f32 output, the default C-order strides, nz <= 256, nu < 65536
 // and column tables within 60 KiB of LDS (nv <= 638).
@@ -421,16 +454,28 @@ template <int CPL>
 static int launch_sep_cpl(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
                           const float* sino, const int8_t* ind, const float* fv, hipStream_t st) {
     const size_t lds = sep_lds_bytes(dt.nv);
-    const int tx = (g.n[0] + SEP_TA - 1) / SEP_TA, ty = (g.n[1] + SEP_TB - 1) / SEP_TB;
-    const int ux = (g.n[0] + SEP_TB - 1) / SEP_TB, uy = (g.n[1] + SEP_TA - 1) / SEP_TA;
-    // SEP_MAXV views per launch pair; the first x launch stores, every
-    // later launch adds (stream order fixes the order of the additions)
-    for (int v = 0; v < vs.nviews; v += SEP_MAXV) {
-        const int nv = min(SEP_MAXV, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
-        hipLaunchKernelGGL((k_bp_sep<0, CPL>), dim3(tx * ty), dim3(SEP_THREADS), lds, st, g, dt,
-                           v0, nv, vs.vstride, v > 0 ? 1 : 0, vol, sino, ind, fv);
-        hipLaunchKernelGGL((k_bp_sep<1, CPL>), dim3(ux * uy), dim3(SEP_THREADS), lds, st, g, dt,
-                           v0, nv, vs.vstride, 1, vol, sino, ind, fv);
+    const int ntx = ((g.n[0] + SEP_TA - 1) / SEP_TA) * ((g.n[1] + SEP_TB - 1) / SEP_TB);
+    const int nty = ((g.n[0] + SEP_TB - 1) / SEP_TB) * ((g.n[1] + SEP_TA - 1) / SEP_TA);
+    if (vs.nviews <= SEP_MAXV) {
+        // one launch for both axes (a single tiling leaves the last round of
+        // workgroups on a fraction of the CUs): each tile adds its sums
+        // atomically into the zeroed volume, two adds per voxel
+        const size_t n = (size_t)g.n[0] * g.n[1] * g.n[2];
+        hipLaunchKernelGGL(k_sep_zero, dim3(2048), dim3(256), 0, st, vol, n);
+        hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(ntx + nty), dim3(SEP_THREADS), lds, st, g, dt,
+                           vs.view0, vs.nviews, vs.vstride, 3, ntx, (int)SEP_ATOMIC, vol, sino,
+                           ind, fv);
+    } else {
+        // SEP_MAXV views per launch pair; the first x launch stores, every
+        // later launch adds (stream order fixes the order of the additions)
+        for (int v = 0; v < vs.nviews; v += SEP_MAXV) {
+            const int nv = min(SEP_MAXV, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
+            hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(ntx), dim3(SEP_THREADS), lds, st, g, dt, v0,
+                               nv, vs.vstride, 1, ntx, v > 0 ? (int)SEP_ADD : (int)SEP_STORE,
+                               vol, sino, ind, fv);
+            hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(nty), dim3(SEP_THREADS), lds, st, g, dt, v0,
+                               nv, vs.vstride, 2, 0, (int)SEP_ADD, vol, sino, ind, fv);
+        }
     }
     CT2_HIP(hipGetLastError());
     return CT2_OK;

@@ -14,7 +14,8 @@ rounding) and to the brick adjoint (relative L2 <= 2e-5, the same bar; up to
 1e-6 of the absolute bilinear form <A|x|, |y|> for Gaussian x, y (it is
 about 2e-8 of it) and by 1e-6 relative for positive x, y.
 
-Cases: volume edges that cut the 8 x 8 tiles, z sizes that leave lanes
+Cases: volume edges that cut the 8 x 8 tiles, 600 views (more than one
+launch takes), z sizes that leave lanes
 without cells (nz = 70, 130, 256), rows stored in reverse (e_v = -z), a
 detector shifted so columns and rows miss the volume, indicator holes and
 zero rays, view subsets, and scans that mix separable views with views
@@ -107,6 +108,8 @@ CASES = {
     "shifted_detector": dict(u_shift=14.0, v_shift=-20.0),
     "mixed_tilted_views": dict(ns=10, tilt_every=3),
     "nz2": dict(shape=(20, 18, 2), nv=8, dv=0.5),
+    # more views than one launch takes (512): launch pairs that store, then add
+    "views600": dict(shape=(14, 12, 20), ns=600, nu=18, nv=24, tilt_every=97),
 }
 
 
