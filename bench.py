@@ -8,11 +8,16 @@ Workload (one step = fwd + bp over all views, inputs resident in HBM):
   scan    circular cone beam, 360 views over 360 deg, source-isocentre 500 mm,
           source-detector 1000 mm, 512x512 detector at 0.8 mm
   updates N = 2 * 360 * 512 * 512 * 256 = 4.832e10 per step
-Also reported: the relative L2 error of the mixed and FP32 paths against the
-FP64 oracle on a subset of views, the dominant kernel's roofline fraction
-(algorithmic bytes 4 * bytes-per-voxel per update over the MI355X HBM peak),
-and the reference path (the reference Fortran compiled from its sources,
-1 thread) timed on a bounded sample of the same workload.
+Also reported: the FP32-fast path (fast arithmetic, f32 volume storage) and
+the plain FP32 path (reference operation order), the relative L2 error of the
+mixed and FP32 paths against the FP64 oracle on a subset of views, the
+dominant kernel's roofline (its binding roof, VALU issue: the executed
+VALU instructions per launch from the committed rocprofv3 counter summary
+over the live duration, against the issue rates ct2_calib_issue measures on
+the same GPU for the kernel's FP64/other instruction mix; beside it the
+compulsory HBM bytes and the measured HBM traffic against 8 TB/s), and the
+reference path (the reference Fortran compiled from its sources, 1 thread)
+timed on a bounded sample of the same workload.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-check]
 
@@ -230,19 +235,45 @@ def cpu_multicore(fv, geo, grid, vol_np, max_threads=16):
                        f"reference, {os.cpu_count()} CPUs visible")}
 
 
-def read_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary, if present."""
-    p = os.path.join(ROOT, "profiles", "bp_brick_fast_c2_hbm_traffic.txt")
+def issue_rates(blocks=2048, iters=20000):
+    """Measured VALU issue rates of this GPU, wavefront instructions per
+    second, for FP32 FMA, FP64 FMA and 32-bit integer rotates (ct2_calib_issue:
+    8 independent chains per thread, 8 wavefronts per SIMD), best of 3."""
+    import torch
+    from ct2 import _dev
+    from ct2._lib import call
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    rates = {}
+    for kind, name in ((0, "f32_fma"), (1, "f64_fma"), (2, "i32_rotate")):
+        best = 1e30
+        for rep in range(4):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call("ct2_calib_issue", kind, blocks, iters, _dev.ptr(out), _dev.stream_handle())
+            e1.record()
+            torch.cuda.synchronize()
+            if rep:   # the first launch warms up
+                best = min(best, e0.elapsed_time(e1))
+        rates[name] = blocks * 4 * iters * 8 / (best * 1e-3)
+    return rates
+
+
+def read_counters(name="bp_sep_c2_counters.txt"):
+    """Per-launch figures of the dominant kernel (the column-separable
+    adjoint) from the committed rocprofv3 counter summary: lines
+    "key: value" (valu_insts_per_launch, traffic_bytes_per_launch, ...)."""
+    p = os.path.join(ROOT, "profiles", name)
+    out = {}
     if not os.path.exists(p):
-        return None
+        return out
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
+        k, sep, v = line.partition(":")
+        if sep and k.strip().endswith("_per_launch"):
             try:
-                return float(line.split(":", 1)[1])
+                out[k.strip()] = float(v)
             except ValueError:
-                return None
-    return None
+                pass
+    return out
 
 
 def batch_workload(args):
@@ -536,8 +567,12 @@ def main():
                                          args.warmup)
     log(f"  convert {tc_m:.3f} ms  fwd {tf_m:.2f} ms  bp {tb_m:.2f} ms  "
         f"wall/step {1e3 * wall_m / args.steps:.2f} ms")
-    log("timing FP32 path (FP32 storage, reference operation order)")
+    log("timing FP32-fast path (FP32 storage, fast arithmetic)")
     sino32 = torch.empty_like(sino)
+    wall_q, _, tf_q, tb_q = time_path(op_mixed, vol32, None, sino32, data, recon,
+                                      max(1, args.steps // 2), 1)
+    log(f"  fwd {tf_q:.2f} ms  bp {tb_q:.2f} ms")
+    log("timing FP32 path (FP32 storage, reference operation order)")
     wall_f, _, tf_f, tb_f = time_path(op_fp32, vol32, None, sino32, data, recon,
                                       max(1, args.steps // 2), 1)
     log(f"  fwd {tf_f:.2f} ms  bp {tb_f:.2f} ms")
@@ -546,14 +581,41 @@ def main():
                                                                sub)
 
     value = nupd / (wall_m / args.steps) / 1e9
+    fast32_value = nupd / (wall_q / max(1, args.steps // 2)) / 1e9
     fp32_value = nupd / (wall_f / max(1, args.steps // 2)) / 1e9
-    # dominant kernel of the mixed step and its roofline
-    if tb_m >= tf_m:
-        kname, t_k, bpv = "k_backproject_brick<linear,fast,f32>", tb_m, 4  # f32 volume written
-    else:
-        kname, t_k, bpv = "k_project_views<linear,fast,f16>", tf_m, 2      # f16 volume read
-    alg_bytes = (nupd / 2) * 4 * bpv
-    achieved = alg_bytes / (t_k * 1e-3) / 1e9
+    # the dominant kernel of the mixed step is the adjoint (the
+    # column-separable kernel: every view of the scan has e_v along z)
+    kname, t_k = "k_bp_sep<4> (column-separable fast adjoint, f32 out)", tb_m
+    cnt = read_counters()
+    rates = issue_rates()
+    log("VALU issue rates (wavefront instructions/s): " +
+        ", ".join(f"{k} {v:.3e}" for k, v in rates.items()))
+    valu, v64 = cnt.get("valu_insts_per_launch"), cnt.get("valu_f64_insts_per_launch")
+    roof = {"bound": "valu", "kernel": kname, "kernel_ms": round(t_k, 3),
+            "unit": "Tlane-op/s", "peak": None, "achieved": None, "frac": None,
+            "valu_insts_per_launch": valu, "valu_f64_insts_per_launch": v64,
+            "issue_rates_per_cu_clock": {k: round(v / (256 * 2.4e9), 3)
+                                         for k, v in rates.items()},
+            "published_fp32_vector_tflops": 157.3}
+    if valu and v64 is not None:
+        # the kernel's instruction mix at the measured issue rates: its FP64
+        # instructions at the FP64 rate, the others at the FP32 rate
+        t_min = v64 / rates["f64_fma"] + (valu - v64) / rates["f32_fma"]
+        ach = valu * 64 / (t_k * 1e-3) / 1e12
+        peak = valu * 64 / t_min / 1e12
+        roof.update({"achieved": round(ach, 3), "peak": round(peak, 3),
+                     "frac": round(ach / peak, 4), "valu_bound_ms": round(1e3 * t_min, 3)})
+    # compulsory HBM bytes of the adjoint: the sinogram and indicator read
+    # once, the f32 volume written once
+    ns_, nu_, nv_ = op_mixed.sino_shape
+    comp = ns_ * nu_ * nv_ * 5 + recon.numel() * 4
+    traffic = cnt.get("traffic_bytes_per_launch")
+    roof["hbm"] = {"peak": HBM_PEAK_GBS, "unit": "GB/s", "compulsory_bytes": comp,
+                   "compulsory_GBs": round(comp / (t_k * 1e-3) / 1e9, 1),
+                   "traffic": traffic,
+                   "measured_GBs": round(traffic / (t_k * 1e-3) / 1e9, 1) if traffic else None,
+                   "frac": round(traffic / (t_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                   if traffic else None}
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -577,13 +639,14 @@ def main():
                    "updates_per_step": nupd},
         "kernels_ms": {"convert_mixed": round(tc_m, 4), "fwd_mixed": round(tf_m, 3),
                        "bp_mixed": round(tb_m, 3),
+                       "fwd_fast_f32": round(tf_q, 3), "bp_fast_f32": round(tb_q, 3),
                        "fwd_fp32": round(tf_f, 3), "bp_fp32": round(tb_f, 3)},
+        "fast_f32_value": round(fast32_value, 3),
         "fp32_value": round(fp32_value, 3),
+        "mixed_over_fast_f32": round(value / fast32_value, 3),
         "mixed_over_fp32": round(value / fp32_value, 3),
         "rel_l2_vs_fp64": None,
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic()},
+        "roofline": roof,
     }
     if not args.no_cpu:
         log("CPU leg: reference path on a bounded sample; FP64 oracle check of the GPU outputs")

@@ -26,6 +26,7 @@ _G = "ffffffiii"        # dx, dy, dz, x0, y0, z0, nx, ny, nz
 
 SIGNATURES = {
     "ct2_version": "",
+    "ct2_calib_issue": "iiiPS",
     "ct2_last_error": "",
     "ct2_desc_init": "P" + _V + "iiiffffff",
     "ct2_project": "PPPPPS",

@@ -488,6 +488,14 @@ int ct2_stencil_art(const ct2_desc* desc, const ct2_stencil* st, const float* w,
                     const float* small, const float* weights, const int8_t* indsino,
                     const float* frame_vectors, float* vol, ct2_stream_t stream);
 
+/* Issue-rate calibration for the roofline figures (no counterpart in the
+ * reference): `blocks` workgroups of 256 threads, each thread running `iters`
+ * iterations of 8 independent chains of one VALU instruction kind (kind 0:
+ * FP32 FMA, 1: FP64 FMA, 2: 32-bit integer rotate), exactly 8 such instructions
+ * per iteration; out[blocks*256] receives the chains' sums.  The caller
+ * times the launch: blocks*4*iters*8 wavefront instructions. */
+int ct2_calib_issue(int kind, int blocks, int iters, float* out, ct2_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
