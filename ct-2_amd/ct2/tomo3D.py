@@ -79,16 +79,21 @@ def _is_tensor(x):
 
 
 def _xp_sum_sq(a, b, w=None):
-    """sqrt(sum(w*(a-b)^2)) in float64 for numpy or torch."""
+    """sqrt(sum(w*(a-b)^2)) for numpy or torch, as emil_code/tomo3D.py:319-324
+    and :946-951 evaluate it: the difference and its square in the arrays'
+    own precision (float32 for float32 images and sinograms), the weight
+    (0 or 1, so exact) and the sum in float64."""
     if _is_tensor(a) or _is_tensor(b):
         torch = _torch()
-        a = torch.as_tensor(a).to(dtype=torch.float64)
-        b = torch.as_tensor(b, device=a.device).to(dtype=torch.float64)
-        d = (a - b) ** 2
+        a = torch.as_tensor(a)
+        b = torch.as_tensor(b, device=a.device)
+        d = a - b
+        d = (d * d).to(torch.float64)
         if w is not None:
             d = d * torch.as_tensor(w, device=a.device).to(torch.float64)
         return float(torch.sqrt(d.sum()))
-    d = (np.asarray(a, np.float64).ravel() - np.asarray(b, np.float64).ravel()) ** 2
+    d = np.asarray(a).ravel() - np.asarray(b).ravel()
+    d = (d * d).astype(np.float64)
     if w is not None:
         d = d * np.asarray(w, np.float64).ravel()
     return float(np.sqrt(d.sum(dtype=np.float64)))
