@@ -447,6 +447,78 @@ def recon_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def art_workload(args):
+    """The CPU-runnable workload C1: a 64^3 Shepp-Logan phantom (1 mm
+    voxels), circular cone beam of 90 views over 360 deg (R = 500 mm,
+    SDD = 1000 mm), 96x96 detector at 1.5 mm; its projection, then 10 ART
+    sweeps (artstep_linear, relaxation 0.5) from zero.  Timed on the GPU with
+    the dataflow schedule (the default) and with the single-workgroup level
+    schedule (CT2_ART_FLOW=0), which must give the same bits; beside them
+    the reference Fortran (oracle/_ref, 1 thread; the FP32 C restatement
+    when it is not built) on one sweep, the bounded CPU sample."""
+    import numpy as np
+    import torch
+    import oracle
+    from ct2 import configs3D, phantoms
+    from ct2 import view_proj3D as vp
+    ns, nu, nv, du = 90, 96, 96, 1.5
+    fv = configs3D.frame_vectors("circular_conebeam", {"radius": 500.0,
+                                                      "source_to_detector": 1000.0},
+                                 ns, 2 * np.pi, 0.0, 0)
+    vol = phantoms.shepp_logan(64, 32.0)
+    geo = (ns, nu, nv, du, du, -72.0, -72.0)
+    g2 = (1.0, 1.0, 1.0, -32.0, -32.0, -32.0, 64, 64, 64)
+    sino_h = vp.sinoproj_linear(None, fv, *geo, vol, *g2)
+    sino = torch.from_numpy(sino_h).cuda()
+    fvd = torch.from_numpy(np.ascontiguousarray(fv, np.float32)).cuda()
+
+    def sweeps(n, flow):
+        if flow:
+            os.environ.pop("CT2_ART_FLOW", None)
+        else:
+            os.environ["CT2_ART_FLOW"] = "0"
+        x = torch.zeros(vol.shape, dtype=torch.float32, device="cuda")
+        x = vp.artstep_linear(0.5, sino, None, fvd, *geo, x, *g2)   # warm-up
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            x = vp.artstep_linear(0.5, sino, None, fvd, *geo, x, *g2)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        os.environ.pop("CT2_ART_FLOW", None)
+        return t, x.cpu().numpy()
+
+    t_flow, x_flow = sweeps(10, True)
+    t_level, x_level = sweeps(10, False)
+    line = {"metric": "seconds per 10 ART sweeps (64^3 Shepp-Logan, 90 views, 96^2 detector, "
+                      "relaxation 0.5)",
+            "value": round(t_flow, 4), "unit": "s", "n_gpus": 1, "steps": 10, "warmup": 1,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: 10-ellipsoid Shepp-Logan phantom and its "
+                                    "projections",
+            "config": {"workload": "C1: ART from zero on the phantom's cone-beam projections"},
+            "ms_per_sweep": {"dataflow": round(1e3 * t_flow / 10, 2),
+                             "level_schedule": round(1e3 * t_level / 10, 2)},
+            "dataflow_equals_level_schedule": bool(np.array_equal(x_flow, x_level))}
+    if not args.no_cpu:
+        mods, kind = ((oracle.reference(), "reference") if oracle.ref_available()
+                      else (oracle.restated("f32"), "port"))
+        x0 = np.zeros_like(vol)
+        t0 = time.perf_counter()
+        x1 = mods.view_proj3D.artstep_linear(0.5, sino_h, None, fv, *geo, x0, *g2)
+        t1 = time.perf_counter() - t0
+        _, g1 = sweeps(1, True)
+        line["cpu_baseline"] = {"value": round(10 * t1, 3), "unit": "s", "cores": 1,
+                                "kind": kind, "sample": f"one sweep ({t1:.2f} s) x 10, single "
+                                "thread" + (", reference Fortran compiled by amdflang -O2 from "
+                                            "emil_code/view_proj3D.f" if kind == "reference"
+                                            else ", FP32 C restatement")}
+        line["gpu_over_cpu"] = round(10 * t1 / t_flow, 1)
+        line["rel_l2_one_sweep_vs_cpu"] = float(np.linalg.norm(g1 - x1) / np.linalg.norm(x1))
+    print(json.dumps(line), flush=True)
+
+
 def asd_workload(args):
     """ASD-POCS at 256^3: the recon workload's noisy data; 10 outer
     iterations of a 10-subset SART sweep, the two distances, TV value, the
@@ -512,10 +584,11 @@ def main():
     ap.add_argument("--profile-once", action="store_true",
                     help="for counter runs: one warm-up and one mixed fwd+bp step, then a "
                          "512 MiB device copy that calibrates FETCH_SIZE/WRITE_SIZE; no JSON")
-    ap.add_argument("--workload", choices=["c2", "batch", "recon", "asd"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batch", "recon", "asd", "art"], default="c2",
                     help="c2: the headline fwd+bp metric (default); batch: 64-problem batched "
                          "SART in a HIP graph; recon: FDK + 20 EM-TV iterations at 256^3; "
-                         "asd: ASD-POCS iterations (SART + TV) in a HIP graph at 256^3")
+                         "asd: ASD-POCS iterations (SART + TV) in a HIP graph at 256^3; "
+                         "art: C1, 10 ART sweeps on the GPU beside the reference Fortran")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the "
@@ -534,6 +607,9 @@ def main():
         return
     if args.workload == "recon":
         recon_workload(args)
+        return
+    if args.workload == "art":
+        art_workload(args)
         return
     fv, geo, grid = geometry()
     log("building phantom")
