@@ -15,6 +15,11 @@ product).
   tomo3D.py:1031-1043), then the gf and reg_fac rules (:321-324).  It
   returns the final image and the records (iter, ddist, posttv, ierr) that
   the reference writes to derrEvol.txt.
+* ``asd_pocs_sart`` restates ct2.solvers.AsdPocs: the same loop with the
+  data step a sweep of ordered-subset SART (oracle/sart.py, nsub interleaved
+  view subsets, relaxation reg_fac), the image and the step sizes gf and
+  reg_fac in the oracle's precision, and every distance with its difference
+  in float64 (ct2_distance).
 * ``em_tv`` restates ct2.solvers.em_tv: the loop of examples/arttest.py:110-128
   with its EM line (:117, emstep_linear, view_proj3D.f:2507-2831;
   tomo3D.py:1394-1435) in place of ART, followed by jawPhantomCCB.py's TV
@@ -136,3 +141,46 @@ def em_tv(o, x, b, ind, fv, geo, g2, n_em, n_tv=5, eps=1e-8, alpha=0.2, red_fact
         if gdist >= conv * adist:
             gf = ft(gf * ft(red_fact))
     return x, hist
+
+
+def asd_pocs_sart(o, x, b, ind, fv, geo, g2, max_iter, nsub=10, ngrad=20, gf=0.2, reg_fac=1.0,
+                  red_fact=0.98, red_reg=0.995, conv=0.99, iter_eps=500, truth=None, eps=1e-9):
+    """ct2.solvers.AsdPocs from image x (not modified) with data b, in the
+    precision of ``o``; returns (image, records, (gf, reg_fac, meps))."""
+    from . import sart as osart
+    ns, nu, nv, du, dv, u0, v0 = geo
+    dx, dy, dz, x0, y0, z0, nx, ny, nz = g2
+    ft = o.dt
+    x = np.array(x, ft).reshape(nx, ny, nz)
+    b = np.asarray(b, ft).reshape(ns, nu, nv)
+    used = np.ones((ns, nu, nv), np.float64) if ind is None else np.asarray(ind, np.float64)
+    gf, reg_fac = ft(gf), ft(reg_fac)
+    meps = -1.0
+    records = []
+    for it in range(1, max_iter + 1):
+        pre = x.copy()
+        x = osart.sart_sweeps(o, 0, 0, reg_fac, 1, nsub, b, ind, fv, *geo, x, *g2).astype(ft)
+        zero_edges(x)
+        make_positive(x)
+        adist = dist64(x, pre)
+        if it == 1:
+            gf = ft(gf * ft(adist))
+        tv = o.total_var3d(0.0, x, nx, ny, nz, eps)
+        proj = o.sinoproj(0, 0, ind, fv, *geo, x, *g2)
+        r = np.asarray(proj, np.float64) - np.asarray(b, np.float64)
+        ddist = float(np.sqrt(np.sum(used * r * r)))
+        if it == iter_eps:
+            meps = ddist
+        ierr = dist64(x, truth) if truth is not None else float("nan")
+        records.append((it, ddist, tv, ierr))
+        pre = x.copy()
+        for _ in range(ngrad):
+            g, _ = o.total_var_grad3d(0.0, x, nx, ny, nz, 1, eps)
+            x = x - gf * g.astype(ft)
+            zero_edges(x)
+            make_positive(x)
+        gdist = dist64(x, pre)
+        if gdist >= conv * adist and ddist > meps:
+            gf = ft(gf * ft(red_fact))
+        reg_fac = ft(reg_fac * ft(red_reg))
+    return x, records, (float(gf), float(reg_fac), meps)

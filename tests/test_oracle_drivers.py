@@ -11,6 +11,9 @@ than by restating them:
   exactly while gf = gf0 * adist_1 * red_fact^k for some k <= iterations;
 * with no TV steps the loop is ART sweeps with edges and positivity (one
   iteration, so reg_fac is still reg_fac0);
+* the SART variant (AsdPocs) with no TV steps is one ordered-subset SART
+  sweep with edges and positivity, its reg_fac schedule is exact in
+  float32, and its data distances fall on consistent data;
 * EM+TV started at the truth of consistent data stays there (EM's fixed
   point makes the first image change, and so the TV step size, vanish: a
   step size taken from anything else moves the image), and the TV steps
@@ -105,3 +108,25 @@ def test_em_tv_fixed_point_and_tv_descent(o32):
     tv = lambda v: o32.total_var3d(0.0, v, *v.shape)
     assert tv(x_tv) < tv(x_em)
     assert x_tv.min() >= 0.0
+
+
+def test_asd_pocs_sart_plumbing(o32):
+    from oracle import sart as osart
+    ind, fv, geo, vol, g2 = _case()
+    b = o32.sinoproj(0, 0, ind, fv, *geo, vol, *g2)
+    x0 = np.zeros_like(vol)
+    x1, rec1, _ = odr.asd_pocs_sart(o32, x0, b, ind, fv, geo, g2, max_iter=1, nsub=3, ngrad=0,
+                                    reg_fac=0.9)
+    want = osart.sart_sweeps(o32, 0, 0, 0.9, 1, 3, b, ind, fv, *geo, x0, *g2)
+    odr.zero_edges(want)
+    odr.make_positive(want)
+    assert np.array_equal(x1, want)
+    x, rec, (gf, reg_fac, meps) = odr.asd_pocs_sart(o32, x0, b, ind, fv, geo, g2, max_iter=4,
+                                                    nsub=3, ngrad=3, reg_fac=1.0)
+    rf = np.float32(1.0)
+    for _ in range(4):
+        rf = np.float32(rf * np.float32(0.995))
+    assert np.float32(reg_fac) == rf
+    d = [r[1] for r in rec]
+    assert all(d1 < d0 for d0, d1 in zip(d, d[1:]))
+    assert x.min() >= 0.0 and not x[:2].any()
