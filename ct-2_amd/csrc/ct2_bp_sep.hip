@@ -17,11 +17,24 @@
 // B(l) k is linear in the row (the ray's z slope is linear in the detector
 // v coordinate and the plane's distance is shared), with B(l) of one sign
 // over the planes; rows are stored in ascending z.  The rows of cell c (c <=
-// uc < c+1) are then the stored rows [r_c, r_c+1), r_c the first row whose uc
-// (the forward's FP32 value) reaches c: the linear model puts r_c within a
-// hundredth of a row of (c - A)/B, and one look at the forward's uc of the
-// nearest row settles it, so every row lands in the cell the forward samples
-// it in.  With FP64 prefix sums PS(r) = sum_{k<r} td_k and PK(r) = sum_{k<r}
+// uc < c+1) are then the stored rows [r_c, r_c+1), r_c = floor(t_c) + 1 for
+// the crossing t_c = (c - A)/B, evaluated in FP32 (ceil(t_c) but at integers,
+// where the choice is open anyway).  Linear interpolation is
+// continuous across a cell boundary, so a row that lies within the FP32
+// error of t_c from it and is taken in the neighbouring cell gets weights
+// that extend that cell's interpolation by as much (about 1e-4 of a cell at
+// most), the order of the forward's own rounding of uc.  The volume's two z
+// faces, c = 0 and c = nc-1, are cut-offs instead: the forward samples no
+// row outside [0, nc-1).  Their rows r_0 and r_nc-1, the first whose forward
+// uc (its FP32 value) reaches the face, are found exactly with the table:
+// the forward's own set-up of the row nearest the crossing, whose uc lies
+// within its rounding (below 1e-3 of a cell) of the linear value, while a
+// row is at least 1/48 of a cell from the next (view_separable), settles
+// whether that row or the next is the first.  Every crossing is clamped to
+// [r_0, r_nc-1] (the true rows increase with c, so this only moves an
+// estimate towards them), and the faces' own estimates are pushed past the
+// ends, so they take r_0 and r_nc-1.  With FP64
+// prefix sums PS(r) = sum_{k<r} td_k and PK(r) = sum_{k<r}
 // td_k k, cell c receives S = PS(r_c+1) - PS(r_c) and
 //     H = sum td_k (uc_k - c) = (A - c) S + B (PK(r_c+1) - PK(r_c)),
 // its lower voxel S - H and its upper voxel H.  The weights use the exact
@@ -41,7 +54,7 @@
 //
 // Each view's columns are taken SEP_CHUNK at a time, one per wavefront: the
 // wavefront sets the column up (FP64, the forward's expressions) and writes
-// its table to LDS: per row the forward's (uc0, duc) and the prefix sums.
+// its table of prefix sums to LDS.
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -56,22 +69,20 @@ constexpr int SEP_MAXV = 512;                   // views per launch (one LDS wor
 constexpr int SEP_FILLB = 4;                    // table rows per batch of loads
 
 // Per (column, tile plane): the linear model uc = A + B kk over stored rows
-// kk (FP64, and A's FP32 value and B's reciprocal), and the forward's b cell
-// and weight.
+// kk (FP64, B > 0); one past the crossing of z, t(z) + 1 = z rB + nA1
+// (FP32); the face rows r_0 and r_nc-1 (as floats); the forward's b cell and
+// its two weights.
 struct SepPlane {
     double A, B;
-    float Af, rB, wb;
+    float rB, nA1, rbot, rtop, w0, w1;
     int ib;
 };
 
-// LDS per column: the prefix pairs (PS, PK) of entries 0 .. nv, then the
-// position pairs (uc0, duc) of rows -1 .. nv as entries 0 .. nv+1 (the two
-// ends are sentinels below and above every row).  (Lanes whose entries lie
-// a multiple of 8 apart share LDS banks; spreading them with spare slots
-// costs more address arithmetic than the conflicts cost: 37.9 against
-// 35.0 ms at 256^3 x 360 views.)
+// LDS per column: the prefix pairs (PS, PK) of entries 0 .. nv.  (Lanes whose
+// entries lie a multiple of 8 apart share LDS banks; spreading them with
+// spare slots costs more address arithmetic than the conflicts cost.)
 __host__ __device__ inline size_t sep_col_bytes(int nv) {
-    return ((size_t)16 * (nv + 1) + (size_t)8 * (nv + 2) + 15) & ~(size_t)15;
+    return (size_t)16 * (nv + 1);
 }
 size_t sep_lds_bytes(int nv) { return SEP_CHUNK * sep_col_bytes(nv); }
 
@@ -153,12 +164,20 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
     const double da = (double)g.d[A], oa = (double)g.o[A];
     const double ob = (double)g.o[Bx], oc = (double)g.o[2];
     const double idb = rcp_f64((double)g.d[Bx]), idc = rcp_f64((double)g.d[2]);
+    const double dacd = da * idc;   // the forward's (double)da * idc
+    const double dvd = (double)dt.dv, v0d = (double)dt.v0;
     const float gxa = fabsf(g.d[0]), gya = fabsf(g.d[1]);
     const float trav_a = fabsf(g.d[A]);
     // this lane's z cells c0 .. c0+CPL-1 (cell c is the forward's corner
     // pair (c, c+1), valid for 0 <= c <= nc-2)
     const int c0 = lane * CPL;
-    const float c0f = (float)c0, ztop = (float)(nc - 1), nvh = (float)nv + 0.5f;
+    const float c0f = (float)c0;
+    // the lane's boundaries c0 .. c0+CPL as searched: the faces pushed past
+    // the ends of the clamp [r_0, r_nc-1]
+    float zs[CPL + 1];
+#pragma unroll
+    for (int bq = 0; bq <= CPL; ++bq)
+        zs[bq] = c0 + bq == 0 ? -1e30f : (c0 + bq >= nc - 1 ? 1e30f : c0f + (float)bq);
     // stored rows per lane in the table fill (odd, so that the lanes' LDS
     // rows fall in different banks)
     const int R = ((nv + 63) >> 6) | 1;
@@ -198,7 +217,7 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
         const unsigned vw = (unsigned)__builtin_amdgcn_readfirstlane((int)s_view[pv]);
         if (vw == ~0u) continue;   // not separable (the brick kernel takes it), or no column reaches the tile
         const int jlo = (int)(vw & 0xffffu), jhi = (int)(vw >> 16);
-        const double Sa = f[A], Sb = f[Bx], Sc = f[2];
+        const double Sa = f[A], Sb = f[Bx], Sc = f[2], evz = f[14];
         const double ta = fma(0.5, da, oa) - Sa;   // plane 0 centre from the source
         const double ScOc = Sc - oc;
         for (int cb = jlo; cb <= jhi; cb += SEP_CHUNK) {
@@ -208,7 +227,6 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                 const int j = cb + wave;
                 double2* pre = reinterpret_cast<double2*>(reinterpret_cast<char*>(s_tab) +
                                                           wave * colb);
-                float2* pos = reinterpret_cast<float2*>(pre + (nv + 1));
                 unsigned mask = 0u;
                 if (j <= jhi && j < dt.nu) {
                     const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
@@ -226,10 +244,11 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                         const float r1f = (float)r1;
                         const float r1sq = 1.0f + r1f * r1f;
                         const double X2 = fma((double)f[11], u, (double)f[8]);
-                        const double v0c = fma(0.5, (double)dt.dv, (double)dt.v0);
+                        const double v0c = fma(0.5, dvd, v0d);
                         // z slope of row k: R0 + R1 k (real arithmetic)
-                        const double R0 = (fma((double)f[14], v0c, X2) - Sc) * ia;
-                        const double R1 = (double)f[14] * (double)dt.dv * ia;
+                        const double R0 = (fma(evz, v0c, X2) - Sc) * ia;
+                        const double R1 = evz * dvd * ia;
+                        const float R0f = (float)R0, R1f = (float)R1;
                         // rows in ascending z: B(l) has one sign over the planes
                         const bool rev = R1 * fma((double)P0, da, ta) * idc < 0.0;
                         // lane q < nP: tile plane P0 + q
@@ -240,7 +259,8 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                             const float ubl = __builtin_fmaf((float)l, dub, ub0);
                             SepPlane sp;
                             sp.ib = floor_i(ubl);
-                            sp.wb = frac_pos(ubl);
+                            sp.w1 = frac_pos(ubl);
+                            sp.w0 = 1.0f - sp.w1;
                             ok = sp.ib >= lo && sp.ib <= hi;
                             // uc of row k on plane l: A(l) + B(l) k, in z cells
                             const double T = fma((double)l, da, ta);
@@ -250,8 +270,26 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                                 sp.A = fma(sp.B, (double)(nv - 1), sp.A);
                                 sp.B = -sp.B;
                             }
-                            sp.Af = (float)sp.A;
-                            sp.rB = __builtin_amdgcn_rcpf((float)sp.B);
+                            const double rBd = rcp_f64(sp.B);
+                            sp.rB = (float)rBd;
+                            sp.nA1 = (float)fma(-sp.A, rBd, 1.0);
+                            // the face rows: m nearest the crossing (clamped to
+                            // the detector), then m or m+1 by the forward's uc
+                            // of row m (setup_fast_ax)
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) {
+                                const double zc = e ? (double)(nc - 1) : 0.0;
+                                const int m = (int)fmin(fmax(rint((zc - sp.A) * rBd), 0.0),
+                                                        (double)(nv - 1));
+                                const int k = rev ? nv - 1 - m : m;
+                                const double v = fma((double)k + 0.5, dvd, v0d);
+                                const double r2 = (fma(evz, v, X2) - Sc) * ia;
+                                const float uc0 = (float)fma(fma(r2, ta, Sc) - oc, idc, -0.5);
+                                const float duc = (float)(r2 * dacd);
+                                const float rf = (float)(m + (__builtin_fmaf((float)l, duc, uc0) <
+                                                              (float)zc ? 1 : 0));
+                                if (e) sp.rtop = rf; else sp.rbot = rf;
+                            }
                             if (ok) s_pl[wave][lane] = sp;
                         }
                         mask = (unsigned)__ballot(ok);
@@ -277,23 +315,19 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                                     const int kk = kb + i0 + u;
                                     if (i0 + u >= R || kk >= nv) continue;
                                     const int k = rev ? nv - 1 - kk : kk;
-                                    const double v = fma((double)k + 0.5, (double)dt.dv, (double)dt.v0);
-                                    const double r2 = (fma((double)f[14], v, X2) - Sc) * ia;
-                                    // setup_fast_ax's c-axis start and step of this ray
-                                    float2 pc;
-                                    pc.x = (float)fma(fma(r2, ta, Sc) - oc, idc, -0.5);
-                                    pc.y = (float)(r2 * (da * idc));
                                     float val = 0.0f;
                                     if (use[u] == 1) {
-                                        const float r2f = (float)r2;
-                                        // the forward's trav (setup_fast_ax), times the datum
+                                        // the forward's trav (setup_fast_ax) with the
+                                        // ray's z slope from its affine form in the row
+                                        // (within 3e-8 of the forward's trav, below
+                                        // fast_sqrt's own rounding), times the datum
+                                        const float r2f = __builtin_fmaf((float)k, R1f, R0f);
                                         val = trav_a * fast_sqrt(r1sq + r2f * r2f) * dat[u];
                                     }
                                     const double td = (double)val;
                                     s0 += td;
                                     s1 = fma(td, (double)kk, s1);
                                     pre[kk + 1] = make_double2(s0, s1);
-                                    pos[kk + 1] = pc;
                                 }
                             }
                             // this lane's rows follow the sums of the lanes below
@@ -309,8 +343,6 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                             }
                             if (lane == 0) {
                                 pre[0] = make_double2(0.0, 0.0);
-                                pos[0] = make_float2(-1e30f, 0.0f);
-                                pos[nv + 1] = make_float2(1e30f, 0.0f);
                             }
                         }
                     }
@@ -319,38 +351,34 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
             }
             __syncthreads();
             // ---- samples on this wavefront's planes, column by column ----
+#pragma unroll
             for (int c = 0; c < SEP_CHUNK; ++c) {
                 const unsigned mask = (unsigned)__builtin_amdgcn_readfirstlane((int)s_mask[c]);
                 const double2* pre = reinterpret_cast<const double2*>(
                     reinterpret_cast<const char*>(s_tab) + c * colb);
-                const float2* pos = reinterpret_cast<const float2*>(pre + (nv + 1));
 #pragma unroll
                 for (int pl = 0; pl < SEP_PPW; ++pl) {
                     const int q = wave * SEP_PPW + pl;
                     if (!((mask >> q) & 1u)) continue;   // uniform
                     const SepPlane& sp = s_pl[c][q];
                     const double Ad = sp.A, Bd = sp.B;
-                    const float rB = uniform_f(sp.rB);
-                    // nearest row to (z - A) / B, as (z - A) / B + 1/2 truncated
-                    const float nAr = uniform_f(__builtin_fmaf(-sp.Af, rB, 0.5f));
-                    const float wb = uniform_f(sp.wb);
+                    const float rB = uniform_f(sp.rB), nA1 = uniform_f(sp.nA1);
+                    const float rbot = uniform_f(sp.rbot), rtop = uniform_f(sp.rtop);
                     const int ib = __builtin_amdgcn_readfirstlane(sp.ib);
                     const float lf = (float)(P0 + q);
                     // cell boundaries z = c0 .. c0+CPL (at most nc-1, so the
                     // cells from nc-1 up, which the forward never samples, are
                     // empty): the first stored row whose uc reaches z, and the
                     // prefix sums there
+                    int r[CPL + 1];
+#pragma unroll
+                    for (int bq = 0; bq <= CPL; ++bq)
+                        r[bq] = (int)__builtin_amdgcn_fmed3f(__builtin_fmaf(zs[bq], rB, nA1), rbot,
+                                                            rtop);
                     double PS[CPL + 1], PK[CPL + 1];
 #pragma unroll
                     for (int bq = 0; bq <= CPL; ++bq) {
-                        const float zf = fminf(c0f + (float)bq, ztop);
-                        // m in [0, nv]: the nearest row, or 0 when that is row
-                        // -1 (whose uc lies below every z anyway)
-                        const int m = (int)__builtin_amdgcn_fmed3f(
-                            __builtin_fmaf(zf, rB, nAr), 0.0f, nvh);
-                        const float2 pm = pos[m + 1];
-                        const int r = m + (__builtin_fmaf(lf, pm.y, pm.x) < zf ? 1 : 0);
-                        const double2 P = pre[r];
+                        const double2 P = pre[r[bq]];
                         PS[bq] = P.x;
                         PK[bq] = P.y;
                     }
@@ -368,12 +396,16 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                     // b rows of the tile: ib takes 1-wb, ib+1 takes wb, the
                     // others 0 (uniform weights, so every row is one FMA with
                     // a scalar operand)
-                    const float w0 = 1.0f - wb, w1 = wb;
+                    // (selected on the scalar unit from the record's two weights)
+                    const int w0b = __builtin_amdgcn_readfirstlane(__float_as_int(sp.w0));
+                    const int w1b = __builtin_amdgcn_readfirstlane(__float_as_int(sp.w1));
                     const int jj = ib - Q0;
                     float wr[SEP_TB];
 #pragma unroll
-                    for (int bi = 0; bi < SEP_TB; ++bi)
-                        wr[bi] = bi == jj ? w0 : (bi == jj + 1 ? w1 : 0.0f);
+                    for (int bi = 0; bi < SEP_TB; ++bi) {
+                        const unsigned dj = (unsigned)(bi - jj);
+                        wr[bi] = __int_as_float(dj == 0u ? w0b : (dj == 1u ? w1b : 0));
+                    }
 #pragma unroll
                     for (int cc = 0; cc < CPL; ++cc) {
                         const float G = (float)(lo[cc] + (cc == 0 ? carry : hi[cc - 1]));
@@ -442,7 +474,7 @@ __global__ void k_sep_zero(float* __restrict__ vol, size_t n) {
 // tile.  Views that fail view_separable contribute nothing
 // here; the caller adds them with the brick kernel.  Requires linear
 // sampling, f32 output, the default C-order strides, nz <= 256, nu < 65536
-// and column tables within 60 KiB of LDS (nv <= 638).
+// and column tables within 60 KiB of LDS (nv <= 959).
 bool sep_supported(const ct2_desc* d, const Grid& g) {
     return d->arith == CT2_ARITH_FAST && d->interp == CT2_LINEAR && d->vol_dtype == CT2_F32 &&
            d->axis == CT2_AXIS_FREE && g.n[2] <= 256 && d->nv >= 1 &&

@@ -7,6 +7,13 @@ The route weighs each sample by its z coordinate in exact arithmetic (prefix
 sums over a column's rows), where the fast forward rounds that coordinate to
 FP32 sample by sample: each z weight differs by at most half an ulp of the
 coordinate (below 2^-24 * nz), with signs that vary from sample to sample.
+A row within the FP32 error of the cell-boundary search from a boundary
+(about 2e-5 of the rows) may be taken in the neighbouring cell, with its
+weights extended past [0, 1] by that error (below 1e-3 of a cell); at the
+volume's two z faces, where the forward cuts rays off, the cut is the
+forward's own, row for row, which the per-voxel comparison with the brick
+kernel (the transpose of the forward's own sampling) checks: one row cut
+wrongly moves a face voxel by a whole sample, above 1e-2 of its value here.
 So the route is held to the FP64 oracle (relative L2 <= 2e-5; it measures
 about 2e-6, closer than the brick kernel, whose z weights carry the FP32
 rounding) and to the brick adjoint (relative L2 <= 2e-5, the same bar; up to
@@ -171,6 +178,33 @@ def test_separable_adjoint_vs_brick(name):
     e = rel_l2(sep, brk)
     print(f"{name}: rel L2 separable vs brick {e:.2e}")
     assert e <= 2e-5, e
+
+
+# the cone covers more than the volume's z extent on every plane, so rays
+# cross both z faces; the first case has the magnitudes of a 256^3 scan (4.7e7
+# face cuts, of which the FP32 crossing alone would misplace hundreds)
+FACE_CASES = {
+    "faces_256x256x24_360views": dict(shape=(256, 256, 24), ns=360, nu=256, nv=256, du=1.6,
+                                      dv=1.6, R=500.0, sdd=1000.0),
+    "faces_reversed_rows": dict(shape=(30, 26, 24), ns=40, nu=44, nv=64, dv=0.9, flip_v=True),
+    "faces_shifted": dict(shape=(31, 27, 25), ns=40, nu=44, nv=70, dv=0.9, v_shift=-9.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FACE_CASES))
+def test_separable_adjoint_faces_per_voxel(name):
+    """Per voxel, against the brick kernel: the z faces are cut where the
+    forward cuts them (positive data, so no voxel is a cancelling sum)."""
+    fv, geo, g2, ind, rng = _case(**FACE_CASES[name])
+    y = rng.uniform(0.5, 1.5, geo[:3]).astype(np.float32)
+    sep, _ = _run(fv, geo, g2, ind, y)
+    brk, _ = _run(fv, geo, g2, ind, y, brick=(10, 10, 24))
+    scale = np.abs(brk).max()
+    err = np.abs(sep - brk)
+    faces = max(err[:, :, 0].max(), err[:, :, -1].max()) / scale
+    print(f"{name}: max voxel error vs brick {err.max() / scale:.2e}, on the z faces {faces:.2e}")
+    assert np.abs(brk[:, :, 0]).max() > 0.1 * scale and np.abs(brk[:, :, -1]).max() > 0.1 * scale
+    assert err.max() <= 1e-4 * scale
 
 
 def test_separable_adjoint_is_deterministic():
