@@ -434,6 +434,319 @@ __global__ __launch_bounds__(256) void k_project_rays(Grid g, int nr, int strict
     sino[r] = out;
 }
 
+// ---- column-separable fast forward (views whose detector rows run along z) ----
+// With e_v parallel to z (view_separable), the rays of detector column j
+// share their driving axis a (x or y) and the fast set-up of the other
+// transverse axis b: on plane l every one of them samples b cell ib(l) with
+// weight wb(l) = frac(fma(l, dub, ub0)).  The fast bilinear sample lerps
+// along b first and then along z, so the b lerp of the two voxel rows
+// (l, ib, :) and (l, ib+1, :) is the same for every ray of the column:
+//     Rw(c) = lerp_b(v(l, ib, c), v(l, ib+1, c)),
+// and a ray's sample on plane l is lerp_z(Rw(ic), Rw(ic+1), wc).  One
+// wavefront takes one column.  Per batch of planes it blends the rows into
+// its own LDS (coalesced row loads, the b lerp once per (column, plane, z)
+// instead of once per ray; the next batch's rows are loaded while the
+// current one is sampled), then each lane marches RPL rays of the column
+// through those rows.  Every sample, its arithmetic and the order of the
+// compensated sum are those of march_fast_batched, so the projections are
+// bit-identical to k_project_views'.  A wavefront whose view is not
+// separable, or whose rays unexpectedly disagree on a, ub0 or dub, runs the
+// per-ray march of k_project_views instead (same rows per call, so the same
+// results).
+constexpr int FS_WAVES = 4;   // detector columns per workgroup, one per wavefront
+constexpr int FS_P = 8;       // planes per batch (one row quad per lane and plane: nz <= 256)
+constexpr int FS_RPLMAX = 8;  // rays per lane at most
+// LDS floats per wavefront: the batch's blended rows, then the rays' weights
+constexpr int FS_WSLAB = FS_P * 256 + 64 * FS_RPLMAX;
+
+// Four consecutive values of a volume row along z: the raw load, and the
+// lerp across b with the fast sample's arithmetic for the storage type (as
+// lerp_cell: (1-wb) x0 + wb x1 by mixed-precision FMAs for f16 and for the
+// f16 pairs' own values, x0 + wb (x1 - x0) otherwise).
+struct __align__(8) h16x4 { __half v[4]; };
+struct __align__(8) bf16x4 { hip_bfloat16 v[4]; };
+
+__device__ __forceinline__ float blend_half(__half x0, __half x1, float wb, float wb1) {
+    return __builtin_fmaf(wb, __half2float(x1), __builtin_fmaf(wb1, __half2float(x0), 0.0f));
+}
+
+template <typename VT> struct RowQuad;
+template <> struct RowQuad<float> {
+    using raw = float4;
+    __device__ __forceinline__ static float4 blend(const raw& a, const raw& b, float wb) {
+        return make_float4(__builtin_fmaf(wb, b.x - a.x, a.x), __builtin_fmaf(wb, b.y - a.y, a.y),
+                           __builtin_fmaf(wb, b.z - a.z, a.z), __builtin_fmaf(wb, b.w - a.w, a.w));
+    }
+};
+template <> struct RowQuad<__half> {
+    using raw = h16x4;
+    __device__ __forceinline__ static float4 blend(const raw& a, const raw& b, float wb) {
+        const float wb1 = 1.0f - wb;
+        return make_float4(blend_half(a.v[0], b.v[0], wb, wb1), blend_half(a.v[1], b.v[1], wb, wb1),
+                           blend_half(a.v[2], b.v[2], wb, wb1), blend_half(a.v[3], b.v[3], wb, wb1));
+    }
+};
+template <> struct RowQuad<h16x2> {
+    // four pairs as one 16-byte load; each pair's own value is its low half
+    using raw = uint4;
+    __device__ __forceinline__ static __half lo(unsigned w) {
+        return __ushort_as_half((unsigned short)(w & 0xffffu));
+    }
+    __device__ __forceinline__ static float4 blend(const raw& a, const raw& b, float wb) {
+        const float wb1 = 1.0f - wb;
+        return make_float4(blend_half(lo(a.x), lo(b.x), wb, wb1), blend_half(lo(a.y), lo(b.y), wb, wb1),
+                           blend_half(lo(a.z), lo(b.z), wb, wb1), blend_half(lo(a.w), lo(b.w), wb, wb1));
+    }
+};
+template <> struct RowQuad<hip_bfloat16> {
+    using raw = bf16x4;
+    __device__ __forceinline__ static float4 blend(const raw& a, const raw& b, float wb) {
+        float4 o;
+        float* op = &o.x;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float x0 = static_cast<float>(a.v[i]), x1 = static_cast<float>(b.v[i]);
+            op[i] = __builtin_fmaf(wb, x1 - x0, x0);
+        }
+        return o;
+    }
+};
+
+// Orders one wavefront's LDS writes before its later reads (and its reads
+// before later writes) without a workgroup barrier: the hardware keeps a
+// wavefront's LDS operations in order, the fences keep the compiler from
+// moving them across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroup b of the grid runs on XCD b % 8 (round-robin dispatch): give
+// each XCD a contiguous range of tiles, so the columns that run together on
+// one XCD are neighbours and share their volume rows in its L2.
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nblk) {
+    const unsigned x = b & 7u, i = b >> 3, q = nblk >> 3, rm = nblk & 7u;
+    return (x < rm ? x * (q + 1) : rm * (q + 1) + (x - rm) * q) + i;
+}
+
+// Kahan step of one sample (the compensated sum of march_fast_batched).
+__device__ __forceinline__ void kahan_add(float val, float& sum, float& comp, bool ok) {
+    const float y = val - comp;
+    const float ts = sum + y;
+    const float cn = (ts - sum) - y;
+    sum = ok ? ts : sum;
+    comp = ok ? cn : comp;
+}
+
+// Grid: one workgroup per (view, chunk of 64*RPL rows, FS_WAVES columns),
+// ncg column groups and nrc row chunks per view.  Requires nz % 4 == 0,
+// nz <= 256, ny*nz < 2^23, a volume below 4 GiB with the default strides
+// and a 16-byte aligned base; LDS
+// FS_WAVES * FS_WSLAB floats.
+// 16-bit rows take half the registers of the batch in flight: four
+// wavefronts per SIMD then fit without spills (5.9 against 6.7 ms for the
+// f16 volume at 256^3 x 360 x 512^2); 32-bit rows at that limit spill.
+#ifndef CT2_FS_WPE16
+#define CT2_FS_WPE16 4
+#endif
+#ifndef CT2_FS_WPE32
+#define CT2_FS_WPE32 1
+#endif
+template <typename VT>
+constexpr int fs_waves_per_eu() {
+    return sizeof(typename RowQuad<VT>::raw) == 8 ? CT2_FS_WPE16 : CT2_FS_WPE32;
+}
+
+template <typename VT, int RPL>
+__global__ __launch_bounds__(64 * FS_WAVES)
+__attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(Grid g, Det dt, int view0,
+                                                               int vstride, int ncg, int nrc,
+                                                               float* __restrict__ sino,
+                                                               const int8_t* __restrict__ ind,
+                                                               const float* __restrict__ fv,
+                                                               const VT* __restrict__ vol) {
+    using RQ = RowQuad<VT>;
+    using raw = typename RQ::raw;
+    extern __shared__ float s_rows[];
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const int cg = (int)(t % (unsigned)ncg);
+    const unsigned rest = t / (unsigned)ncg;
+    const int rc = (int)(rest % (unsigned)nrc), pv = (int)(rest / (unsigned)nrc);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = cg * FS_WAVES + wave;
+    if (j >= dt.nu) return;   // the whole wavefront; no workgroup barrier follows
+    const int p = view0 + pv * vstride;
+    const int k0 = rc * 64 * RPL;
+    const size_t col = ((size_t)p * dt.nu + j) * dt.nv;
+
+    bool sep = view_separable(fv + (size_t)p * 18, g, dt);
+    float uc0[RPL], duc[RPL];
+    float* rows = s_rows + (size_t)wave * FS_WSLAB;
+    float* trav = rows + FS_P * 256;   // ray weights, lane-private (no sync needed)
+    float ub0 = 0.0f, dub = 0.0f;
+    int a = 0;
+    if (sep) {
+        // setup_fast_ax's expressions, split into the column's part and each
+        // row's: with e_v = (0, 0, e_vz) the bin's x and y do not depend on
+        // the row (fma(0, v, x) = x), so neither do a, ub0 and dub
+        const ViewFrame F = load_frame(fv, p);
+        const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
+        const double dd0 = fma((double)F.eu[0], u, (double)F.d[0]) - (double)F.s[0];
+        const double dd1 = fma((double)F.eu[1], u, (double)F.d[1]) - (double)F.s[1];
+        const double X2 = fma((double)F.eu[2], u, (double)F.d[2]);
+        const double S2 = F.s[2], ev2 = F.ev[2];
+        auto ddz = [&](int k) {
+            return fma(ev2, fma((double)k + 0.5, (double)dt.dv, (double)dt.v0), X2) - S2;
+        };
+        a = fast_axis(g, dd0, dd1, ddz(k0), false);   // row k0 exists
+        const double dda = a == 0 ? dd0 : dd1, ddb = a == 0 ? dd1 : dd0;
+        const float da = a == 0 ? g.d[0] : g.d[1], oa = a == 0 ? g.o[0] : g.o[1];
+        const float ob = a == 0 ? g.o[1] : g.o[0];
+        const double Sa = a == 0 ? F.s[0] : F.s[1], Sb = a == 0 ? F.s[1] : F.s[0];
+        const double ia = rcp_f64(dda);
+        const double id0 = rcp_f64((double)g.d[0]), id1 = rcp_f64((double)g.d[1]);
+        const double idb = a == 0 ? id1 : id0, idc = rcp_f64((double)g.d[2]);
+        const double r1 = ddb * ia;
+        const double ta = fma(0.5, (double)da, (double)oa) - Sa;
+        ub0 = (float)fma(fma(r1, ta, Sb) - (double)ob, idb, -0.5);
+        dub = (float)(r1 * ((double)da * idb));
+        const double dacd = (double)da * idc;
+        const float r1f = (float)r1;
+        const float r1sq = 1.0f + r1f * r1f;
+        const float tra = fabsf(da);
+        bool agree = true;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int k = k0 + 64 * r + lane;
+            const bool active = k < dt.nv;
+            const bool use = active && (ind == nullptr || ind[col + k] == 1);
+            const double d2 = ddz(k);
+            agree &= !active || fast_axis(g, dd0, dd1, d2, false) == a;
+            const double r2 = d2 * ia;
+            const float r2f = (float)r2;
+            // a ray that is not used marches cell 0 of every plane (in
+            // bounds) with weight 0 and is stored as 0
+            uc0[r] = use ? (float)fma(fma(r2, ta, S2) - (double)g.o[2], idc, -0.5) : 0.5f;
+            duc[r] = use ? (float)(r2 * dacd) : 0.0f;
+            trav[64 * r + lane] = use ? tra * fast_sqrt(r1sq + r2f * r2f) : 0.0f;
+        }
+        sep = __all(agree) && a < 2;
+    }
+    if (!sep) {
+        // per-ray march, 64 consecutive rows per call as in k_project_views
+#pragma unroll 1
+        for (int r = 0; r < RPL; ++r) {
+            const int k = k0 + 64 * r + lane;
+            const bool active = k < dt.nv;
+            const bool use = active && (ind == nullptr || ind[col + k] == 1);
+            const float out =
+                project_lane<CT2_LINEAR, CT2_ARITH_FAST, VT>(g, dt, p, j, k, use, false, fv, vol);
+            if (active) sino[col + k] = out;
+        }
+        return;
+    }
+    const int na = a == 0 ? g.n[0] : g.n[1], nb = a == 0 ? g.n[1] : g.n[0];
+    // element strides of a and b (below 2^23, and the volume below 4 GiB:
+    // signed 24-bit multiplies and 32-bit byte offsets)
+    const int sa = a == 0 ? g.s[0] : g.s[1], sb = a == 0 ? g.s[1] : g.s[0];
+    const int nc = g.n[2];
+    // planes whose b cell is inside the volume: an interval, since
+    // fma(l, dub, ub0) is monotonic in l
+    int lo = 0x7fffffff, hi = -1;
+    for (int l = lane; l < na; l += 64) {
+        const int ib = floor_i(__builtin_fmaf((float)l, dub, ub0));
+        if ((unsigned)ib < (unsigned)(nb - 1)) { lo = min(lo, l); hi = max(hi, l); }
+    }
+    const int lb0 = wave_min(lo), lb1 = wave_max(hi);
+    float sum[RPL], comp[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) sum[r] = comp[r] = 0.0f;
+    const unsigned hc = (unsigned)(nc - 1);
+    const int c4 = 4 * lane;   // this lane's z values c4 .. c4+3 of every row
+    // the row quads of the batch starting at plane lb (a plane past the
+    // volume, or a lane past nz, loads element 0 and is not used)
+    raw x0[FS_P], x1[FS_P];
+    const char* vb = reinterpret_cast<const char*>(vol);
+    const unsigned esz = (unsigned)sizeof(VT);
+    auto load_batch = [&](int lb) {
+#pragma unroll
+        for (int q = 0; q < FS_P; ++q) {
+            const int l = lb + q;
+            const bool ok = l <= lb1 && c4 < nc;
+            const int ib = floor_i(__builtin_fmaf((float)l, dub, ub0));
+            const unsigned o = ok ? (unsigned)(__mul24(l, sa) + __mul24(ib, sb) + c4) * esz : 0u;
+            x0[q] = *reinterpret_cast<const raw*>(vb + o);
+            x1[q] = *reinterpret_cast<const raw*>(vb + o + (ok ? (unsigned)sb * esz : 0u));
+        }
+    };
+    if (lb0 <= lb1) load_batch(lb0);
+    for (int lb = lb0; lb <= lb1; lb += FS_P) {
+        const int np = min(FS_P, lb1 - lb + 1);
+        float lfq[FS_P];   // the batch's plane indices as floats (exact), in scalar registers
+#pragma unroll
+        for (int q = 0; q < FS_P; ++q) lfq[q] = uniform_f((float)(lb + q));
+        // the batch's rows, lerped across b, into LDS
+#pragma unroll
+        for (int q = 0; q < FS_P; ++q) {
+            if (q < np && c4 < nc)
+                *reinterpret_cast<float4*>(rows + q * nc + c4) =
+                    RQ::blend(x0[q], x1[q], frac_pos(__builtin_fmaf(lfq[q], dub, ub0)));
+        }
+        wave_lds_sync();
+        if (lb + FS_P <= lb1) load_batch(lb + FS_P);   // in flight while this batch is sampled
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            // every ray of this slot inside the volume on the batch's first
+            // and last plane is inside on every plane between (the valid
+            // planes of a ray are an interval): then no sample needs a mask
+            const int ia = floor_i(__builtin_fmaf(lfq[0], duc[r], uc0[r]));
+            const int ie = floor_i(__builtin_fmaf(lfq[FS_P - 1], duc[r], uc0[r]));
+            if (np == FS_P && __all((unsigned)ia < hc && (unsigned)ie < hc)) {
+#pragma unroll
+                for (int h = 0; h < FS_P; h += 4) {
+                    float val[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float uc = __builtin_fmaf(lfq[h + q], duc[r], uc0[r]);
+                        const int ic = floor_i(uc);
+                        const float* row = rows + (h + q) * nc;
+                        const float vi1 = row[ic], vi2 = row[ic + 1];
+                        val[q] = __builtin_fmaf(frac_pos(uc), vi2 - vi1, vi1);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) kahan_add(val[q], sum[r], comp[r], true);
+                }
+            } else {
+#pragma unroll
+                for (int h = 0; h < FS_P; h += 4) {
+                    float val[4];
+                    bool ok[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float uc = __builtin_fmaf(lfq[h + q], duc[r], uc0[r]);
+                        const int ic = floor_i(uc);
+                        ok[q] = (unsigned)ic < hc && h + q < np;
+                        const int icc = ok[q] ? ic : 0;
+                        const float* row = rows + (h + q) * nc;
+                        const float vi1 = row[icc], vi2 = row[icc + 1];
+                        val[q] = __builtin_fmaf(frac_pos(uc), vi2 - vi1, vi1);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) kahan_add(val[q], sum[r], comp[r], ok[q]);
+                }
+            }
+        }
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        const int k = k0 + 64 * r + lane;
+        const float tw = trav[64 * r + lane];
+        if (k < dt.nv) sino[col + k] = tw != 0.0f ? sum[r] * tw : 0.0f;
+    }
+}
+
 // ---------------------------------------------------------------- host ----
 template <int INTERP, int ARITH, typename VT, int TU>
 static void launch_tile(const Grid& g, const Det& dt, const ViewSet& vs, int zonly, float* sino,
@@ -508,9 +821,66 @@ static std::map<TileKey, int>& tile_cache() {
     return *m;
 }
 
+// The column-separable forward takes fast linear projections along free
+// axes of volumes below 4 GiB with nz % 4 == 0, nz <= 256, ny*nz < 2^23 and
+// a 16-byte aligned base
+// (CT2_FWD_SEP=0 keeps k_project_views, which gives the same results).
+static bool fsep_supported(const ct2_desc* d, const Grid& g, const void* vol) {
+    if (d->arith != CT2_ARITH_FAST || d->interp != CT2_LINEAR || d->axis != CT2_AXIS_FREE)
+        return false;
+    if (g.n[2] % 4 != 0 || g.n[2] > 256 || ((uintptr_t)vol & 15u) != 0) return false;
+    const size_t esz = d->vol_dtype == CT2_F32 || d->vol_dtype == CT2_F16Z2 ? 4 : 2;
+    if (g.s[0] >= (1 << 23) || g.n[0] >= (1 << 23) || g.n[1] >= (1 << 23) ||
+        (size_t)g.n[0] * (size_t)g.s[0] * esz >= (1ull << 32))
+        return false;
+    const char* env = getenv("CT2_FWD_SEP");
+    return !(env && env[0] == '0');
+}
+
+template <typename VT, int RPL>
+static void launch_fsep_t(const Grid& g, const Det& dt, const ViewSet& vs, float* sino,
+                          const int8_t* ind, const float* fv, const void* vol, hipStream_t st) {
+    const int ncg = (dt.nu + FS_WAVES - 1) / FS_WAVES;
+    const int nrc = (dt.nv + 64 * RPL - 1) / (64 * RPL);
+    const size_t nblk = (size_t)ncg * nrc * vs.nviews;
+    const size_t lds = (size_t)FS_WAVES * FS_WSLAB * sizeof(float);
+    hipLaunchKernelGGL((k_project_sep<VT, RPL>), dim3((unsigned)nblk), dim3(64 * FS_WAVES), lds,
+                       st, g, dt, vs.view0, vs.vstride, ncg, nrc, sino, ind, fv, (const VT*)vol);
+}
+
+template <typename VT>
+static void launch_fsep_rpl(const Grid& g, const Det& dt, const ViewSet& vs, float* sino,
+                            const int8_t* ind, const float* fv, const void* vol,
+                            hipStream_t st) {
+    if (dt.nv <= 64) launch_fsep_t<VT, 1>(g, dt, vs, sino, ind, fv, vol, st);
+    else if (dt.nv <= 128) launch_fsep_t<VT, 2>(g, dt, vs, sino, ind, fv, vol, st);
+    else if (dt.nv <= 256) launch_fsep_t<VT, 4>(g, dt, vs, sino, ind, fv, vol, st);
+    else launch_fsep_t<VT, 8>(g, dt, vs, sino, ind, fv, vol, st);
+}
+
+static int launch_fsep(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
+                       float* sino, const int8_t* ind, const float* fv, const void* vol,
+                       hipStream_t st) {
+    if ((size_t)((dt.nu + FS_WAVES - 1) / FS_WAVES) * ((dt.nv + 63) / 64) * vs.nviews >=
+        (1ull << 31)) {
+        set_error("ct2_project: too many detector tiles");
+        return CT2_EINVAL;
+    }
+    switch (d->vol_dtype) {
+        case CT2_F32: launch_fsep_rpl<float>(g, dt, vs, sino, ind, fv, vol, st); break;
+        case CT2_F16: launch_fsep_rpl<__half>(g, dt, vs, sino, ind, fv, vol, st); break;
+        case CT2_BF16: launch_fsep_rpl<hip_bfloat16>(g, dt, vs, sino, ind, fv, vol, st); break;
+        case CT2_F16Z2: launch_fsep_rpl<h16x2>(g, dt, vs, sino, ind, fv, vol, st); break;
+        default: set_error("ct2_project: unknown vol_dtype %d", d->vol_dtype); return CT2_EINVAL;
+    }
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
 static int tuned_project(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                          float* s, const int8_t* ind, const float* fv, const void* vol,
                          hipStream_t st) {
+    if (fsep_supported(d, g, vol)) return launch_fsep(d, g, dt, vs, s, ind, fv, vol, st);
     int dev = 0;
     CT2_HIP(hipGetDevice(&dev));
     TileKey key;
