@@ -159,7 +159,7 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
     const int i0 = A == 0 ? P0 : Q0, j0 = A == 0 ? Q0 : P0;
     const int ni = A == 0 ? nP : nQ, nj = A == 0 ? nQ : nP;
     const int nb = g.n[Bx], nc = g.n[2];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     // the fast set-up's grid constants (setup_fast_ax), FP64
     const double da = (double)g.d[A], oa = (double)g.o[A];
     const double ob = (double)g.o[Bx], oc = (double)g.o[2];
@@ -292,6 +292,17 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                             }
                             if (ok) s_pl[wave][lane] = sp;
                         }
+                        if (!ok && lane < SEP_TA) {
+                            // a plane that misses the tile (or lies past the
+                            // volume): every boundary clamps to row 0 (PS =
+                            // PK = 0) and the weights are 0, so sampling it
+                            // adds nothing
+                            SepPlane sp;
+                            sp.A = sp.B = 0.0;
+                            sp.rB = sp.nA1 = sp.rbot = sp.rtop = sp.w0 = sp.w1 = 0.0f;
+                            sp.ib = Q0;
+                            s_pl[wave][lane] = sp;
+                        }
                         mask = (unsigned)__ballot(ok);
                         if (mask != 0u) {
                             // the table: lane takes stored rows [lane*R, lane*R + R)
@@ -356,10 +367,12 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                 const unsigned mask = (unsigned)__builtin_amdgcn_readfirstlane((int)s_mask[c]);
                 const double2* pre = reinterpret_cast<const double2*>(
                     reinterpret_cast<const char*>(s_tab) + c * colb);
+                // the wavefront's planes together, when either samples the
+                // tile (the record of a plane that misses it adds nothing)
+                if (((mask >> (wave * SEP_PPW)) & ((1u << SEP_PPW) - 1u)) == 0u) continue;
 #pragma unroll
                 for (int pl = 0; pl < SEP_PPW; ++pl) {
                     const int q = wave * SEP_PPW + pl;
-                    if (!((mask >> q) & 1u)) continue;   // uniform
                     const SepPlane& sp = s_pl[c][q];
                     const double Ad = sp.A, Bd = sp.B;
                     const float rB = uniform_f(sp.rB), nA1 = uniform_f(sp.nA1);
