@@ -258,7 +258,7 @@ def issue_rates(blocks=2048, iters=20000):
     return rates
 
 
-def read_counters(name="bp_sep_c2_counters.txt"):
+def read_counters(name="sep_kernels_c2_counters.txt"):
     """Per-launch figures of the dominant kernel (the column-separable
     adjoint) from the committed rocprofv3 counter summary: lines
     "key: value" (valu_insts_per_launch, traffic_bytes_per_launch, ...)."""
