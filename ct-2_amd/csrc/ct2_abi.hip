@@ -102,6 +102,20 @@ int ct2_backproject(const ct2_desc* d, void* vol, const float* sino, const int8_
     return launch_backproject(d, vol, sino, ind, fv, (hipStream_t)stream);
 }
 
+int ct2_backproject_ws(const ct2_desc* d, void* vol, const float* sino, const int8_t* ind,
+                       const float* fv, void* work, size_t work_bytes, ct2_stream_t stream) {
+    if (!d) { set_error("ct2_backproject_ws: null descriptor"); return CT2_EINVAL; }
+    SepWork w;
+    w.ptr = work;
+    w.bytes = work ? work_bytes : 0;
+    return launch_backproject(d, vol, sino, ind, fv, (hipStream_t)stream, w);
+}
+
+size_t ct2_backproject_work_bytes(const ct2_desc* d) {
+    if (!d) { set_error("ct2_backproject_work_bytes: null descriptor"); return 0; }
+    return sep_work_bytes(d);
+}
+
 #define VIEW_ARGS                                                                              \
     int ns, int nu, int nv, float du, float dv, float u0, float v0
 #define GRID_ARGS                                                                              \

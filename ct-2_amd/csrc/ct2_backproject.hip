@@ -741,7 +741,7 @@ static int3 default_brick(const ct2_desc* d) {
 template <int INTERP, int ARITH, typename OT>
 static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                         void* vol, const float* sino, const int8_t* ind, const float* fv,
-                        hipStream_t st) {
+                        hipStream_t st, SepWork w) {
     int3 b = default_brick(d);
     b.x = min(b.x, g.n[0]); b.y = min(b.y, g.n[1]); b.z = min(b.z, g.n[2]);
     // FP64 brick accumulator with a one-voxel halo on every side
@@ -768,7 +768,7 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
     if constexpr (ARITH == CT2_ARITH_FAST && INTERP == CT2_LINEAR &&
                   std::is_same<OT, float>::value) {
         if (sep_route(d, g)) {
-            int rc = launch_backproject_sep(g, dt, vs, (float*)vol, sino, ind, fv, st);
+            int rc = launch_backproject_sep(g, dt, vs, (float*)vol, sino, ind, fv, st, w);
             if (rc) return rc;
             hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), dim3(nb.x * nb.y * nb.z),
                                dim3(BRICK_THREADS), lds, st, g, dt, vs.view0, vs.nviews,
@@ -789,11 +789,11 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
 template <int INTERP, int ARITH>
 static int dispatch_out(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                         void* vol, const float* sino, const int8_t* ind, const float* fv,
-                        hipStream_t st) {
+                        hipStream_t st, SepWork w) {
     switch (d->vol_dtype) {
-        case CT2_F32: return launch_brick<INTERP, ARITH, float>(d, g, dt, vs, vol, sino, ind, fv, st);
-        case CT2_F16: return launch_brick<INTERP, ARITH, __half>(d, g, dt, vs, vol, sino, ind, fv, st);
-        case CT2_BF16: return launch_brick<INTERP, ARITH, hip_bfloat16>(d, g, dt, vs, vol, sino, ind, fv, st);
+        case CT2_F32: return launch_brick<INTERP, ARITH, float>(d, g, dt, vs, vol, sino, ind, fv, st, w);
+        case CT2_F16: return launch_brick<INTERP, ARITH, __half>(d, g, dt, vs, vol, sino, ind, fv, st, w);
+        case CT2_BF16: return launch_brick<INTERP, ARITH, hip_bfloat16>(d, g, dt, vs, vol, sino, ind, fv, st, w);
     }
     set_error("ct2_backproject: unknown vol_dtype %d", d->vol_dtype);
     return CT2_EINVAL;
@@ -801,7 +801,7 @@ static int dispatch_out(const ct2_desc* d, const Grid& g, const Det& dt, const V
 
 static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                        void* vol, const float* sn, const int8_t* ind, const float* fv,
-                       hipStream_t st);
+                       hipStream_t st, SepWork w = SepWork());
 
 // ---- brick autotuner ----
 // The best brick depends on the geometry (footprint per view, LDS bank
@@ -891,7 +891,7 @@ static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt,
 }
 
 int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int8_t* ind,
-                       const float* fv, hipStream_t st) {
+                       const float* fv, hipStream_t st, SepWork w) {
     Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
     Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
     int rc;
@@ -907,22 +907,22 @@ int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int
     // no brick is tuned for it)
     if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 && !sep_route(d, g))
         return tuned_backproject(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
-    return launch_with(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
+    return launch_with(d, g, dt, vs, vol, (const float*)sino, ind, fv, st, w);
 }
 
 // One launch with the brick of *d (0 entries take the defaults).
 static int launch_with(const ct2_desc* d, const Grid& g, const Det& dt, const ViewSet& vs,
                        void* vol, const float* sn, const int8_t* ind, const float* fv,
-                       hipStream_t st) {
+                       hipStream_t st, SepWork w) {
     int rc;
     if (d->arith == CT2_ARITH_EXACT)
         rc = d->interp == CT2_LINEAR
-                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, vs, vol, sn, ind, fv, st)
-                 : dispatch_out<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, vs, vol, sn, ind, fv, st);
+                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_EXACT>(d, g, dt, vs, vol, sn, ind, fv, st, w)
+                 : dispatch_out<CT2_NEAREST, CT2_ARITH_EXACT>(d, g, dt, vs, vol, sn, ind, fv, st, w);
     else if (d->arith == CT2_ARITH_FAST)
         rc = d->interp == CT2_LINEAR
-                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, vs, vol, sn, ind, fv, st)
-                 : dispatch_out<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, vs, vol, sn, ind, fv, st);
+                 ? dispatch_out<CT2_LINEAR, CT2_ARITH_FAST>(d, g, dt, vs, vol, sn, ind, fv, st, w)
+                 : dispatch_out<CT2_NEAREST, CT2_ARITH_FAST>(d, g, dt, vs, vol, sn, ind, fv, st, w);
     else {
         set_error("ct2_backproject: unknown arith %d", d->arith);
         return CT2_EINVAL;

@@ -55,6 +55,8 @@
 // Each view's columns are taken SEP_CHUNK at a time, one per wavefront: the
 // wavefront sets the column up (FP64, the forward's expressions) and writes
 // its table of prefix sums to LDS.
+#include <mutex>
+
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -66,7 +68,7 @@ constexpr int SEP_TB = 8;                       // tile cells along b
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
 constexpr int SEP_CHUNK = SEP_WAVES;            // detector columns per LDS table set
 constexpr int SEP_MAXV = 512;                   // views per launch (one LDS word each)
-constexpr int SEP_FILLB = 4;                    // table rows per batch of loads
+constexpr int SEP_RMAX = 16;                    // table rows per lane in k_sep_tables
 
 // Per (column, tile plane): the linear model uc = A + B kk over stored rows
 // kk (FP64, B > 0); one past the crossing of z, t(z) + 1 = z rB + nA1
@@ -78,13 +80,23 @@ struct SepPlane {
     int ib;
 };
 
-// LDS per column: the prefix pairs (PS, PK) of entries 0 .. nv.  (Lanes whose
-// entries lie a multiple of 8 apart share LDS banks; spreading them with
-// spare slots costs more address arithmetic than the conflicts cost.)
+// A column's table: the prefix pairs (PS, PK) of entries 0 .. nv, 16 bytes
+// each, in the workspace (one per view and column) and in LDS (one per
+// wavefront).  (Lanes whose entries lie a multiple of 8 apart share LDS
+// banks; spreading them with spare slots costs more address arithmetic than
+// the conflicts cost.)
 __host__ __device__ inline size_t sep_col_bytes(int nv) {
     return (size_t)16 * (nv + 1);
 }
 size_t sep_lds_bytes(int nv) { return SEP_CHUNK * sep_col_bytes(nv); }
+
+// Whether a column's rows are stored reversed (uc decreasing in the row on
+// the tile's planes; the sign of the plane distance T is the same on every
+// plane of a separable view): one expression, so that k_sep_tables and the
+// tiles agree.
+__device__ __forceinline__ bool sep_rev(double R1, int P0, double da, double ta, double idc) {
+    return R1 * fma((double)P0, da, ta) * idc < 0.0;
+}
 
 // x from the lane CTRL names (DPP), 0 where that lane is out of range or
 // the row is masked off by RM.
@@ -140,13 +152,13 @@ __device__ void sep_column_range(const float* __restrict__ f, const Det& dt, dou
 enum { SEP_STORE = 0, SEP_ADD = 1, SEP_ATOMIC = 2 };
 
 // One tile of driving axis AX: tile number `tile` of that axis's tiling.
+// tabs: the column tables of views pv = 0 .. nviews-1 (k_sep_tables);
 // s_tab: the dynamic LDS tables; s_pl, s_mask, s_view: the kernel's.
 template <int AX, int CPL>
 __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& dt, int view0,
                                             int nviews, int vstride, int mode,
                                             float* __restrict__ vol,
-                                            const float* __restrict__ sino,
-                                            const int8_t* __restrict__ ind,
+                                            const double2* __restrict__ tabs,
                                             const float* __restrict__ fv, double* s_tab,
                                             SepPlane (*s_pl)[SEP_TA], unsigned* s_mask,
                                             unsigned* s_view) {
@@ -167,7 +179,6 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
     const double dacd = da * idc;   // the forward's (double)da * idc
     const double dvd = (double)dt.dv, v0d = (double)dt.v0;
     const float gxa = fabsf(g.d[0]), gya = fabsf(g.d[1]);
-    const float trav_a = fabsf(g.d[A]);
     // this lane's z cells c0 .. c0+CPL-1 (cell c is the forward's corner
     // pair (c, c+1), valid for 0 <= c <= nc-2)
     const int c0 = lane * CPL;
@@ -178,9 +189,6 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
 #pragma unroll
     for (int bq = 0; bq <= CPL; ++bq)
         zs[bq] = c0 + bq == 0 ? -1e30f : (c0 + bq >= nc - 1 ? 1e30f : c0f + (float)bq);
-    // stored rows per lane in the table fill (odd, so that the lanes' LDS
-    // rows fall in different banks)
-    const int R = ((nv + 63) >> 6) | 1;
 
     // the tile's cell box: sample points whose cells touch the tile
     const double xa = (double)g.o[0] + ((double)(i0 - 1) + 0.5) * g.d[0];
@@ -236,21 +244,26 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                     // the fast set-up's axis choice (fast_axis; z never drives here)
                     const bool xy = fabsf((float)dd0) * gya > fabsf((float)dd1) * gxa;
                     if ((xy ? 0 : 1) == A) {
+                        // the column's table, built once by k_sep_tables,
+                        // copied into LDS while its plane records are made
+                        const double2* src = tabs + ((size_t)pv * dt.nu + j) * (nv + 1);
+                        for (int e0 = 0; e0 <= nv; e0 += 64)
+                            if (e0 + lane <= nv)
+                                __builtin_amdgcn_global_load_lds(
+                                    (const __attribute__((address_space(1))) void*)(src + e0 + lane),
+                                    (__attribute__((address_space(3))) void*)(pre + e0), 16, 0, 0);
                         const double dda = A == 0 ? dd0 : dd1, ddb = A == 0 ? dd1 : dd0;
                         const double ia = rcp_f64(dda);
                         const double r1 = ddb * ia;
                         const float ub0 = (float)fma(fma(r1, ta, Sb) - ob, idb, -0.5);
                         const float dub = (float)(r1 * (da * idb));
-                        const float r1f = (float)r1;
-                        const float r1sq = 1.0f + r1f * r1f;
                         const double X2 = fma((double)f[11], u, (double)f[8]);
                         const double v0c = fma(0.5, dvd, v0d);
                         // z slope of row k: R0 + R1 k (real arithmetic)
                         const double R0 = (fma(evz, v0c, X2) - Sc) * ia;
                         const double R1 = evz * dvd * ia;
-                        const float R0f = (float)R0, R1f = (float)R1;
                         // rows in ascending z: B(l) has one sign over the planes
-                        const bool rev = R1 * fma((double)P0, da, ta) * idc < 0.0;
+                        const bool rev = sep_rev(R1, P0, da, ta, idc);
                         // lane q < nP: tile plane P0 + q
                         const int lo = max(Q0 - 1, 0), hi = min(Q0 + nQ - 1, nb - 2);
                         bool ok = false;
@@ -304,62 +317,11 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                             s_pl[wave][lane] = sp;
                         }
                         mask = (unsigned)__ballot(ok);
-                        if (mask != 0u) {
-                            // the table: lane takes stored rows [lane*R, lane*R + R)
-                            const size_t rbase = ((size_t)p * dt.nu + j) * nv;
-                            const int kb = lane * R;
-                            double s0 = 0.0, s1 = 0.0;
-                            // SEP_FILLB rows at a time: their loads first (rows
-                            // past the end read row nv-1 and are dropped)
-                            for (int i0 = 0; i0 < R; i0 += SEP_FILLB) {
-                                float dat[SEP_FILLB];
-                                int use[SEP_FILLB];
-#pragma unroll
-                                for (int u = 0; u < SEP_FILLB; ++u) {
-                                    const int kk = min(kb + i0 + u, nv - 1);
-                                    const size_t r = rbase + (rev ? nv - 1 - kk : kk);
-                                    dat[u] = sino[r];
-                                    use[u] = ind == nullptr ? 1 : (int)ind[r];
-                                }
-#pragma unroll
-                                for (int u = 0; u < SEP_FILLB; ++u) {
-                                    const int kk = kb + i0 + u;
-                                    if (i0 + u >= R || kk >= nv) continue;
-                                    const int k = rev ? nv - 1 - kk : kk;
-                                    float val = 0.0f;
-                                    if (use[u] == 1) {
-                                        // the forward's trav (setup_fast_ax) with the
-                                        // ray's z slope from its affine form in the row
-                                        // (within 3e-8 of the forward's trav, below
-                                        // fast_sqrt's own rounding), times the datum
-                                        const float r2f = __builtin_fmaf((float)k, R1f, R0f);
-                                        val = trav_a * fast_sqrt(r1sq + r2f * r2f) * dat[u];
-                                    }
-                                    const double td = (double)val;
-                                    s0 += td;
-                                    s1 = fma(td, (double)kk, s1);
-                                    pre[kk + 1] = make_double2(s0, s1);
-                                }
-                            }
-                            // this lane's rows follow the sums of the lanes below
-                            const double e0 = wave_scan_f64(s0) - s0;
-                            const double e1 = wave_scan_f64(s1) - s1;
-                            for (int i = 0; i < R; ++i) {
-                                const int kk = kb + i;
-                                if (kk >= nv) break;
-                                double2 t = pre[kk + 1];
-                                t.x += e0;
-                                t.y += e1;
-                                pre[kk + 1] = t;
-                            }
-                            if (lane == 0) {
-                                pre[0] = make_double2(0.0, 0.0);
-                            }
-                        }
                     }
                 }
                 if (lane == 0) s_mask[wave] = mask;
             }
+            __builtin_amdgcn_s_waitcnt(0);   // this wavefront's table copy has landed
             __syncthreads();
             // ---- samples on this wavefront's planes, column by column ----
 #pragma unroll
@@ -461,19 +423,100 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
 template <int CPL>
 __global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_bp_sep(
     Grid g, Det dt, int view0, int nviews, int vstride, int axes, int ntx, int mode,
-    float* __restrict__ vol, const float* __restrict__ sino, const int8_t* __restrict__ ind,
-    const float* __restrict__ fv) {
+    float* __restrict__ vol, const double2* __restrict__ tabs, const float* __restrict__ fv) {
     extern __shared__ double s_tab[];       // SEP_CHUNK column tables
     __shared__ SepPlane s_pl[SEP_CHUNK][SEP_TA];
     __shared__ unsigned s_mask[SEP_CHUNK];  // bit q: tile plane q samples the tile
     __shared__ unsigned s_view[SEP_MAXV];   // per view: jlo | jhi << 16, or ~0u
     const int b = (int)blockIdx.x;
     if ((axes & 1) && b < ntx)
-        bp_sep_tile<0, CPL>(b, g, dt, view0, nviews, vstride, mode, vol, sino, ind, fv, s_tab,
-                            s_pl, s_mask, s_view);
+        bp_sep_tile<0, CPL>(b, g, dt, view0, nviews, vstride, mode, vol, tabs, fv, s_tab, s_pl,
+                            s_mask, s_view);
     else
         bp_sep_tile<1, CPL>((axes & 1) ? b - ntx : b, g, dt, view0, nviews, vstride, mode, vol,
-                            sino, ind, fv, s_tab, s_pl, s_mask, s_view);
+                            tabs, fv, s_tab, s_pl, s_mask, s_view);
+}
+
+// The column tables of views view0 + pv*vstride (pv < nviews) of a separable
+// view set: one wavefront per (view, detector column), its driving axis a
+// from the fast set-up's choice, table entry kk+1 = the prefix sums over the
+// stored rows 0..kk (rows in ascending z) of td = trav * datum (0 for a ray
+// the indicator drops) and of td * kk, in FP64; entry 0 = (0, 0).  Lane l
+// sums the rows [l*R, l*R + R) and a wavefront scan adds the lanes below.
+// Views that are not separable are skipped (their tables are not read).
+__global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, int nviews,
+                                                    int vstride, double2* __restrict__ tabs,
+                                                    const float* __restrict__ sino,
+                                                    const int8_t* __restrict__ ind,
+                                                    const float* __restrict__ fv) {
+    const int lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int pv = (int)(item / dt.nu), j = (int)(item % dt.nu);
+    if (pv >= nviews) return;   // the whole wavefront
+    const int p = view0 + pv * vstride;
+    const float* f = fv + (size_t)p * 18;
+    if (!view_separable(f, g, dt)) return;
+    const int nv = dt.nv;
+    const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
+    // x and y of the bin do not depend on the row (e_v has no x, y)
+    const double dd0 = fma((double)f[9], u, (double)f[6]) - (double)f[0];
+    const double dd1 = fma((double)f[10], u, (double)f[7]) - (double)f[1];
+    // the fast set-up's axis choice (fast_axis; z never drives here)
+    const bool xy = fabsf((float)dd0) * fabsf(g.d[1]) > fabsf((float)dd1) * fabsf(g.d[0]);
+    const int a = xy ? 0 : 1;
+    const double dda = a == 0 ? dd0 : dd1, ddb = a == 0 ? dd1 : dd0;
+    const double da = (double)g.d[a], oa = (double)g.o[a];
+    const double Sa = f[a], Sc = f[2], evz = f[14];
+    const double ta = fma(0.5, da, oa) - Sa;
+    const double idc = rcp_f64((double)g.d[2]);
+    const double dvd = (double)dt.dv, v0d = (double)dt.v0;
+    const double ia = rcp_f64(dda);
+    const float r1f = (float)(ddb * ia);
+    const float r1sq = 1.0f + r1f * r1f;
+    const double X2 = fma((double)f[11], u, (double)f[8]);
+    const double v0c = fma(0.5, dvd, v0d);
+    // z slope of row k: R0 + R1 k (real arithmetic)
+    const double R0 = (fma(evz, v0c, X2) - Sc) * ia;
+    const double R1 = evz * dvd * ia;
+    const float R0f = (float)R0, R1f = (float)R1;
+    const bool rev = sep_rev(R1, 0, da, ta, idc);
+    const float trav_a = fabsf(g.d[a]);
+    const int R = ((nv + 63) >> 6) | 1;   // rows per lane (nv <= 959: R <= 15)
+    const int kb = lane * R;
+    const size_t rbase = ((size_t)p * dt.nu + j) * nv;
+    double2* tab = tabs + ((size_t)pv * dt.nu + j) * (nv + 1);
+    double ps[SEP_RMAX], pk[SEP_RMAX];
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < SEP_RMAX; ++i) {
+        const int kk = kb + i;
+        if (i < R && kk < nv) {
+            const int k = rev ? nv - 1 - kk : kk;
+            float val = 0.0f;
+            if (ind == nullptr || ind[rbase + k] == 1) {
+                // the forward's trav (setup_fast_ax) with the ray's z slope
+                // from its affine form in the row (within 3e-8 of the
+                // forward's trav, below fast_sqrt's own rounding), times the
+                // datum
+                const float r2f = __builtin_fmaf((float)k, R1f, R0f);
+                val = trav_a * fast_sqrt(r1sq + r2f * r2f) * sino[rbase + k];
+            }
+            const double td = (double)val;
+            s0 += td;
+            s1 = fma(td, (double)kk, s1);
+        }
+        ps[i] = s0;
+        pk[i] = s1;
+    }
+    // this lane's rows follow the sums of the lanes below
+    const double e0 = wave_scan_f64(s0) - s0;
+    const double e1 = wave_scan_f64(s1) - s1;
+#pragma unroll
+    for (int i = 0; i < SEP_RMAX; ++i) {
+        const int kk = kb + i;
+        if (i < R && kk < nv) tab[kk + 1] = make_double2(ps[i] + e0, pk[i] + e1);
+    }
+    if (lane == 0) tab[0] = make_double2(0.0, 0.0);
 }
 
 __global__ void k_sep_zero(float* __restrict__ vol, size_t n) {
@@ -495,49 +538,118 @@ bool sep_supported(const ct2_desc* d, const Grid& g) {
            g.s[1] == g.n[2] && g.s[0] == g.n[1] * g.n[2];
 }
 
+// The column tables of one view: nu tables of nv + 1 entries.
+static size_t sep_view_bytes(const Det& dt) { return (size_t)dt.nu * sep_col_bytes(dt.nv); }
+
+// Views whose tables an internally allocated workspace holds at most (the
+// 256^3 x 360 x 512^2 scan needs 1.5 GB for all of them).
+constexpr size_t SEP_WS_CAP = (size_t)4 << 30;
+
+size_t sep_work_bytes(const ct2_desc* d) {
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
+    ViewSet vs;
+    if (check_grid(g) || check_det(dt) || view_set(d, "ct2_backproject_work_bytes", vs)) return 0;
+    if (!sep_route(d, g)) return 0;
+    return (size_t)vs.nviews * sep_view_bytes(dt);
+}
+
+// gv views per group: each group's tables are built, then its x- and
+// y-driven tiles run.  One group: both axes in one launch, adding atomically
+// into the zeroed volume (two commuting adds per voxel).  More groups: per
+// group an x launch that stores (first group) or adds, then a y launch that
+// adds, so stream order fixes the order of the additions.
 template <int CPL>
 static int launch_sep_cpl(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
-                          const float* sino, const int8_t* ind, const float* fv, hipStream_t st) {
+                          const float* sino, const int8_t* ind, const float* fv, double2* tabs,
+                          int gv, hipStream_t st) {
     const size_t lds = sep_lds_bytes(dt.nv);
     const int ntx = ((g.n[0] + SEP_TA - 1) / SEP_TA) * ((g.n[1] + SEP_TB - 1) / SEP_TB);
     const int nty = ((g.n[0] + SEP_TB - 1) / SEP_TB) * ((g.n[1] + SEP_TA - 1) / SEP_TA);
-    if (vs.nviews <= SEP_MAXV) {
-        // one launch for both axes (a single tiling leaves the last round of
-        // workgroups on a fraction of the CUs): each tile adds its sums
-        // atomically into the zeroed volume, two adds per voxel
-        const size_t n = (size_t)g.n[0] * g.n[1] * g.n[2];
-        hipLaunchKernelGGL(k_sep_zero, dim3(2048), dim3(256), 0, st, vol, n);
-        hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(ntx + nty), dim3(SEP_THREADS), lds, st, g, dt,
-                           vs.view0, vs.nviews, vs.vstride, 3, ntx, (int)SEP_ATOMIC, vol, sino,
-                           ind, fv);
-    } else {
-        // SEP_MAXV views per launch pair; the first x launch stores, every
-        // later launch adds (stream order fixes the order of the additions)
-        for (int v = 0; v < vs.nviews; v += SEP_MAXV) {
-            const int nv = min(SEP_MAXV, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
+    for (int v = 0; v < vs.nviews; v += gv) {
+        const int nvg = min(gv, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
+        const long long items = (long long)nvg * dt.nu;   // one wavefront per (view, column)
+        hipLaunchKernelGGL(k_sep_tables, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, st, g,
+                           dt, v0, nvg, vs.vstride, tabs, sino, ind, fv);
+        if (gv >= vs.nviews) {
+            const size_t n = (size_t)g.n[0] * g.n[1] * g.n[2];
+            hipLaunchKernelGGL(k_sep_zero, dim3(2048), dim3(256), 0, st, vol, n);
+            hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(ntx + nty), dim3(SEP_THREADS), lds, st, g,
+                               dt, v0, nvg, vs.vstride, 3, ntx, (int)SEP_ATOMIC, vol, tabs, fv);
+        } else {
             hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(ntx), dim3(SEP_THREADS), lds, st, g, dt, v0,
-                               nv, vs.vstride, 1, ntx, v > 0 ? (int)SEP_ADD : (int)SEP_STORE,
-                               vol, sino, ind, fv);
+                               nvg, vs.vstride, 1, ntx, v > 0 ? (int)SEP_ADD : (int)SEP_STORE,
+                               vol, tabs, fv);
             hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(nty), dim3(SEP_THREADS), lds, st, g, dt, v0,
-                               nv, vs.vstride, 2, 0, (int)SEP_ADD, vol, sino, ind, fv);
+                               nvg, vs.vstride, 2, 0, (int)SEP_ADD, vol, tabs, fv);
         }
     }
     CT2_HIP(hipGetLastError());
     return CT2_OK;
 }
 
+// The stream-ordered pool keeps up to this much freed memory for the next
+// allocation instead of returning it at each synchronisation (its default is
+// 0), so that repeated adjoints without a caller workspace do not map and
+// unmap their tables every call.  Set once per device.
+static void keep_pool_memory(int dev) {
+    static std::mutex mu;
+    static bool done[64] = {};
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev < 0 || dev >= 64 || done[dev]) return;
+    done[dev] = true;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
+    uint64_t cur = 0, want = (uint64_t)SEP_WS_CAP;
+    if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &cur) == hipSuccess &&
+        cur < want)
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &want);
+}
+
 int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
-                           const float* sino, const int8_t* ind, const float* fv,
-                           hipStream_t st) {
+                           const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
+                           SepWork w) {
     const int cpl = (g.n[2] + 63) / 64;
-    switch (cpl) {
-        case 1: return launch_sep_cpl<1>(g, dt, vs, vol, sino, ind, fv, st);
-        case 2: return launch_sep_cpl<2>(g, dt, vs, vol, sino, ind, fv, st);
-        case 3: return launch_sep_cpl<3>(g, dt, vs, vol, sino, ind, fv, st);
-        case 4: return launch_sep_cpl<4>(g, dt, vs, vol, sino, ind, fv, st);
+    if (cpl < 1 || cpl > 4) {
+        set_error("separable adjoint: nz = %d beyond 256", g.n[2]);
+        return CT2_EINVAL;
     }
-    set_error("separable adjoint: nz = %d beyond 256", g.n[2]);
-    return CT2_EINVAL;
+    const size_t per = sep_view_bytes(dt);
+    void* own = nullptr;
+    int gv;
+    if (w.ptr) {
+        if (((uintptr_t)w.ptr & 15u) != 0) {
+            set_error("ct2_backproject: workspace must be 16-byte aligned");
+            return CT2_EINVAL;
+        }
+        const size_t fit = w.bytes / per;
+        if (fit < 1) {
+            set_error("ct2_backproject: workspace of %zu B holds no view's tables (%zu B)",
+                      w.bytes, per);
+            return CT2_EINVAL;
+        }
+        gv = (int)min((size_t)min(vs.nviews, SEP_MAXV), fit);
+    } else {
+        gv = (int)min((size_t)min(vs.nviews, SEP_MAXV), max((size_t)1, SEP_WS_CAP / per));
+        int dev = 0;
+        CT2_HIP(hipGetDevice(&dev));
+        keep_pool_memory(dev);
+        CT2_HIP(hipMallocAsync(&own, per * gv, st));
+        w.ptr = own;
+    }
+    double2* tabs = static_cast<double2*>(w.ptr);
+    int rc = CT2_EINVAL;
+    switch (cpl) {
+        case 1: rc = launch_sep_cpl<1>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
+        case 2: rc = launch_sep_cpl<2>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
+        case 3: rc = launch_sep_cpl<3>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
+        case 4: rc = launch_sep_cpl<4>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
+    }
+    if (own) {
+        const hipError_t e = hipFreeAsync(own, st);
+        if (rc == CT2_OK && e != hipSuccess) return hip_fail(e, "hipFreeAsync");
+    }
+    return rc;
 }
 
 }  // namespace ct2

@@ -676,8 +676,14 @@ int view_set(const ct2_desc* d, const char* who, ViewSet& vs);
 // ------------------------------------------------------ launchers (host) ----
 int launch_project(const ct2_desc* desc, void* sino, const int8_t* ind, const float* fv,
                    const void* vol, hipStream_t st);
+// Caller-owned device workspace of the column-separable adjoint (its column
+// tables); a null pointer lets the launch allocate one on the stream.
+struct SepWork {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
 int launch_backproject(const ct2_desc* desc, void* vol, const void* sino, const int8_t* ind,
-                       const float* fv, hipStream_t st);
+                       const float* fv, hipStream_t st, SepWork w = SepWork());
 int launch_ray_project(const ct2_desc* desc, float* sino, const int8_t* ind, const float* rv,
                        int nr, const float* vol, int strict, hipStream_t st);
 int launch_ray_backproject(const ct2_desc* desc, float* vol, const float* sino,
@@ -704,8 +710,11 @@ inline bool sep_route(const ct2_desc* d, const Grid& g) {
     return d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 && sep_supported(d, g);
 }
 int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
-                           const float* sino, const int8_t* ind, const float* fv,
-                           hipStream_t st);
+                           const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
+                           SepWork w);
+// Bytes of the separable adjoint's column tables for every view of the subset
+// (0 when the route does not apply).
+size_t sep_work_bytes(const ct2_desc* d);
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
                           hipStream_t st);
 int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,

@@ -31,6 +31,8 @@ SIGNATURES = {
     "ct2_desc_init": "P" + _V + "iiiffffff",
     "ct2_project": "PPPPPS",
     "ct2_backproject": "PPPPPS",
+    "ct2_backproject_ws": "PPPPPPzS",
+    "ct2_backproject_work_bytes": "P",
     "ct2_sart_step": "PfPPPPPS",
     "ct2_convert_volume": "PPiziuS",
     "ct2_batch_create": "PiPPPPPPPiiS",
@@ -127,6 +129,7 @@ def lib():
             fn.restype = C.c_int
         L.ct2_last_error.restype = C.c_char_p
         L.ct2_last_error.argtypes = []
+        L.ct2_backproject_work_bytes.restype = C.c_size_t
         _LIB = L
     return _LIB
 

@@ -20,7 +20,7 @@ import torch
 from . import _dev
 from ._lib import (CT2_ARITH_EXACT, CT2_ARITH_FAST, CT2_AXIS_FREE, CT2_AXIS_Z, CT2_BF16,
                    CT2_F16, CT2_F16Z2, CT2_F32, CT2_LINEAR, CT2_NEAREST, CT2_ROUND_NEAREST,
-                   CT2_ROUND_STOCHASTIC, call, ct2_desc)
+                   CT2_ROUND_STOCHASTIC, call, ct2_desc, lib)
 
 _DT = {torch.float32: CT2_F32, torch.float16: CT2_F16, torch.bfloat16: CT2_BF16}
 
@@ -70,6 +70,7 @@ class ConeBeamOperator:
         self.ind = _dev.ind_dev(indicator, (ns, nu, nv))
         self.device = self.fv.device
         self._sens = None
+        self._work = None   # the column-separable adjoint's tables (bytes)
 
     @classmethod
     def from_objects(cls, sino, image, **kw):
@@ -107,14 +108,21 @@ class ConeBeamOperator:
 
     def adjoint(self, sino, out=None, dtype=torch.float32, view0=0, nviews=0, vstride=1):
         """A^T sino -> volume written in `dtype` storage (over a view subset as
-        in ``forward``)."""
+        in ``forward``).  The column-separable route keeps its column tables
+        in a workspace the operator allocates once and reuses, so calls on
+        one operator are ordered on one stream (or serialised by the
+        caller)."""
         if sino.shape != self.sino_shape or sino.dtype != torch.float32 or not sino.is_contiguous():
             raise ValueError(f"sinogram must be a contiguous float32 {self.sino_shape} tensor")
         if out is None:
             out = torch.empty(self.shape, dtype=dtype, device=sino.device)
         d = self._desc(out.dtype, view0, nviews, vstride)
-        call("ct2_backproject", C.byref(d), _dev.ptr(out), _dev.ptr(sino), _dev.ptr(self.ind),
-             _dev.ptr(self.fv), _dev.stream_handle())
+        n = int(lib().ct2_backproject_work_bytes(C.byref(d)))
+        if n and (self._work is None or self._work.numel() < n):
+            self._work = torch.empty(n, dtype=torch.uint8, device=sino.device)
+        call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(sino), _dev.ptr(self.ind),
+             _dev.ptr(self.fv), _dev.ptr(self._work) if n else None, C.c_size_t(n),
+             _dev.stream_handle())
         return out
 
     def sensitivity(self):

@@ -77,9 +77,30 @@ int ct2_desc_init(ct2_desc* desc, int ns, int nu, int nv, float du, float dv, fl
 int ct2_project(const ct2_desc* desc, float* sino, const int8_t* ind, const float* fv,
                 const void* vol, ct2_stream_t stream);
 /* Exact adjoint of ct2_project (same matrix entries), written over vol
- * (storage desc->vol_dtype). */
+ * (storage desc->vol_dtype).  The column-separable route (fast linear
+ * arithmetic into float32, views whose detector rows run along z) builds
+ * per-(view, column) tables in device memory: this entry allocates them on
+ * the stream (hipMallocAsync, at most 4 GiB, freed on the stream; the
+ * device's default stream-ordered pool is set to keep up to 4 GiB of freed
+ * memory for the next call).  ct2_backproject_ws takes them from the caller
+ * instead. */
 int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
                     const float* fv, ct2_stream_t stream);
+/* ct2_backproject with a caller-owned device workspace (16-byte aligned) for
+ * the column-separable route's tables: work_bytes >= one view's tables
+ * (nu * (nv+1) * 16 bytes); ct2_backproject_work_bytes(desc) bytes hold the
+ * tables of every view of the subset, and a smaller workspace processes the
+ * views in groups.  The call allocates nothing; it must not overlap another
+ * call that uses the same workspace.  work == NULL behaves as
+ * ct2_backproject.  No counterpart in the reference (view_proj3D.f:1096-1370
+ * has no workspace); for callers that keep a workspace across calls (graph
+ * capture, iteration loops). */
+int ct2_backproject_ws(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
+                       const float* fv, void* work, size_t work_bytes, ct2_stream_t stream);
+/* Bytes of ct2_backproject_ws's workspace for every view of desc's subset; 0
+ * when the column-separable route does not apply (then no workspace is
+ * used). */
+size_t ct2_backproject_work_bytes(const ct2_desc* desc);
 /* float32 volume -> 16-bit storage (dtype CT2_F16 or CT2_BF16) for the
  * mixed-precision forward projection.  CT2_ROUND_STOCHASTIC picks the upper
  * or lower 16-bit neighbour with probability proportional to proximity,

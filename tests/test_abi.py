@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_functions():
     text = open(os.path.join(ROOT, "include", "ct2.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(?:int|const char\*)\s+(ct2_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(?:int|size_t|const char\*)\s+(ct2_\w+)\s*\(", text)))
 
 
 def test_header_parses():
@@ -34,7 +34,7 @@ def header_prototypes():
     text = open(os.path.join(ROOT, "include", "ct2.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     protos = {}
-    for m in re.finditer(r"\b(?:int|const char\*)\s+(ct2_\w+)\s*\(([^)]*)\)\s*;", text):
+    for m in re.finditer(r"\b(?:int|size_t|const char\*)\s+(ct2_\w+)\s*\(([^)]*)\)\s*;", text):
         args = [a.strip() for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
         sig = ""
         for a in args:
