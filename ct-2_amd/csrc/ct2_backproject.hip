@@ -542,7 +542,11 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det
                                                            int skip_sep, OT* __restrict__ vol,
                                                            const float* __restrict__ sino,
                                                            const int8_t* __restrict__ ind,
-                                                           const float* __restrict__ fv) {
+                                                           const float* __restrict__ fv,
+                                                           const int* __restrict__ nonsep) {
+    // the pass after the column-separable route: nothing to add when that
+    // route found every view separable
+    if (nonsep != nullptr && *nonsep == 0) return;
     extern __shared__ double acc[];
     // nsplit workgroups per brick, each over its own share of the views
     const int part = blockIdx.x % nsplit;
@@ -768,12 +772,17 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
     if constexpr (ARITH == CT2_ARITH_FAST && INTERP == CT2_LINEAR &&
                   std::is_same<OT, float>::value) {
         if (sep_route(d, g)) {
-            int rc = launch_backproject_sep(g, dt, vs, (float*)vol, sino, ind, fv, st, w);
+            void* own = nullptr;
+            int rc = sep_work_acquire(dt, vs, w, &own, st);
             if (rc) return rc;
-            hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), dim3(nb.x * nb.y * nb.z),
-                               dim3(BRICK_THREADS), lds, st, g, dt, vs.view0, vs.nviews,
-                               vs.vstride, zonly, b, nb, 1, 1, (OT*)vol, sino, ind, fv);
-            return CT2_OK;
+            rc = launch_backproject_sep(g, dt, vs, (float*)vol, sino, ind, fv, st, w);
+            if (rc == CT2_OK)
+                hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>),
+                                   dim3(nb.x * nb.y * nb.z), dim3(BRICK_THREADS), lds, st, g, dt,
+                                   vs.view0, vs.nviews, vs.vstride, zonly, b, nb, 1, 1, (OT*)vol,
+                                   sino, ind, fv, sep_nonsep_count(w));
+            const int rf = sep_work_release(own, st);
+            return rc ? rc : rf;
         }
     }
     const int nsplit = (ARITH == CT2_ARITH_FAST && std::is_same<OT, float>::value) ? 2 : 1;
@@ -782,7 +791,7 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
     dim3 grid(nb.x * nb.y * nb.z * nsplit);
     hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(BRICK_THREADS), lds, st, g, dt,
                        vs.view0, vs.nviews, vs.vstride, zonly, b, nb, nsplit, 0, (OT*)vol, sino,
-                       ind, fv);
+                       ind, fv, (const int*)nullptr);
     return CT2_OK;
 }
 

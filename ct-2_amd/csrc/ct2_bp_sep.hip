@@ -446,6 +446,7 @@ __global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))
 // Views that are not separable are skipped (their tables are not read).
 __global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, int nviews,
                                                     int vstride, double2* __restrict__ tabs,
+                                                    int* __restrict__ nonsep,
                                                     const float* __restrict__ sino,
                                                     const int8_t* __restrict__ ind,
                                                     const float* __restrict__ fv) {
@@ -455,7 +456,11 @@ __global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, i
     if (pv >= nviews) return;   // the whole wavefront
     const int p = view0 + pv * vstride;
     const float* f = fv + (size_t)p * 18;
-    if (!view_separable(f, g, dt)) return;
+    if (!view_separable(f, g, dt)) {
+        // one count per view, for the brick kernel's pass
+        if (j == 0 && lane == 0) atomicAdd(nonsep, 1);
+        return;
+    }
     const int nv = dt.nv;
     const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
     // x and y of the bin do not depend on the row (e_v has no x, y)
@@ -519,7 +524,9 @@ __global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, i
     if (lane == 0) tab[0] = make_double2(0.0, 0.0);
 }
 
-__global__ void k_sep_zero(float* __restrict__ vol, size_t n) {
+// Clears vol (n floats) and the count of views left to the brick kernel.
+__global__ void k_sep_zero(float* __restrict__ vol, size_t n, int* __restrict__ nonsep) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nonsep = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         vol[i] = 0.0f;
@@ -541,6 +548,10 @@ bool sep_supported(const ct2_desc* d, const Grid& g) {
 // The column tables of one view: nu tables of nv + 1 entries.
 static size_t sep_view_bytes(const Det& dt) { return (size_t)dt.nu * sep_col_bytes(dt.nv); }
 
+// The workspace: a header (the device count of views the route leaves to the
+// brick kernel), then the tables of a group of views.
+constexpr size_t SEP_WS_HEAD = 256;
+
 // Views whose tables an internally allocated workspace holds at most (the
 // 256^3 x 360 x 512^2 scan needs 1.5 GB for all of them).
 constexpr size_t SEP_WS_CAP = (size_t)4 << 30;
@@ -551,7 +562,7 @@ size_t sep_work_bytes(const ct2_desc* d) {
     ViewSet vs;
     if (check_grid(g) || check_det(dt) || view_set(d, "ct2_backproject_work_bytes", vs)) return 0;
     if (!sep_route(d, g)) return 0;
-    return (size_t)vs.nviews * sep_view_bytes(dt);
+    return SEP_WS_HEAD + (size_t)vs.nviews * sep_view_bytes(dt);
 }
 
 // gv views per group: each group's tables are built, then its x- and
@@ -561,19 +572,20 @@ size_t sep_work_bytes(const ct2_desc* d) {
 // adds, so stream order fixes the order of the additions.
 template <int CPL>
 static int launch_sep_cpl(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
-                          const float* sino, const int8_t* ind, const float* fv, double2* tabs,
-                          int gv, hipStream_t st) {
+                          const float* sino, const int8_t* ind, const float* fv, int* nonsep,
+                          double2* tabs, int gv, hipStream_t st) {
     const size_t lds = sep_lds_bytes(dt.nv);
     const int ntx = ((g.n[0] + SEP_TA - 1) / SEP_TA) * ((g.n[1] + SEP_TB - 1) / SEP_TB);
     const int nty = ((g.n[0] + SEP_TB - 1) / SEP_TB) * ((g.n[1] + SEP_TA - 1) / SEP_TA);
+    const bool one = gv >= vs.nviews;
+    const size_t n = one ? (size_t)g.n[0] * g.n[1] * g.n[2] : 0;
+    hipLaunchKernelGGL(k_sep_zero, dim3(one ? 2048 : 1), dim3(256), 0, st, vol, n, nonsep);
     for (int v = 0; v < vs.nviews; v += gv) {
         const int nvg = min(gv, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
         const long long items = (long long)nvg * dt.nu;   // one wavefront per (view, column)
         hipLaunchKernelGGL(k_sep_tables, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, st, g,
-                           dt, v0, nvg, vs.vstride, tabs, sino, ind, fv);
-        if (gv >= vs.nviews) {
-            const size_t n = (size_t)g.n[0] * g.n[1] * g.n[2];
-            hipLaunchKernelGGL(k_sep_zero, dim3(2048), dim3(256), 0, st, vol, n);
+                           dt, v0, nvg, vs.vstride, tabs, nonsep, sino, ind, fv);
+        if (one) {
             hipLaunchKernelGGL((k_bp_sep<CPL>), dim3(ntx + nty), dim3(SEP_THREADS), lds, st, g,
                                dt, v0, nvg, vs.vstride, 3, ntx, (int)SEP_ATOMIC, vol, tabs, fv);
         } else {
@@ -588,10 +600,10 @@ static int launch_sep_cpl(const Grid& g, const Det& dt, const ViewSet& vs, float
     return CT2_OK;
 }
 
-// The stream-ordered pool keeps up to this much freed memory for the next
-// allocation instead of returning it at each synchronisation (its default is
-// 0), so that repeated adjoints without a caller workspace do not map and
-// unmap their tables every call.  Set once per device.
+// The stream-ordered pool keeps up to SEP_WS_CAP of freed memory for the
+// next allocation instead of returning it at each synchronisation (its
+// default is 0), so that repeated adjoints without a caller workspace do not
+// map and unmap their tables every call.  Set once per device.
 static void keep_pool_memory(int dev) {
     static std::mutex mu;
     static bool done[64] = {};
@@ -606,6 +618,37 @@ static void keep_pool_memory(int dev) {
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &want);
 }
 
+int sep_work_acquire(const Det& dt, const ViewSet& vs, SepWork& w, void** own, hipStream_t st) {
+    *own = nullptr;
+    const size_t per = sep_view_bytes(dt);
+    if (w.ptr) {
+        if (((uintptr_t)w.ptr & 15u) != 0) {
+            set_error("ct2_backproject: workspace must be 16-byte aligned");
+            return CT2_EINVAL;
+        }
+        if (w.bytes < SEP_WS_HEAD + per) {
+            set_error("ct2_backproject: workspace of %zu B holds no view's tables (%zu B)",
+                      w.bytes, SEP_WS_HEAD + per);
+            return CT2_EINVAL;
+        }
+        return CT2_OK;
+    }
+    const size_t gv = min((size_t)min(vs.nviews, SEP_MAXV), max((size_t)1, SEP_WS_CAP / per));
+    int dev = 0;
+    CT2_HIP(hipGetDevice(&dev));
+    keep_pool_memory(dev);
+    CT2_HIP(hipMallocAsync(own, SEP_WS_HEAD + per * gv, st));
+    w.ptr = *own;
+    w.bytes = SEP_WS_HEAD + per * gv;
+    return CT2_OK;
+}
+
+int sep_work_release(void* own, hipStream_t st) {
+    if (own) CT2_HIP(hipFreeAsync(own, st));
+    return CT2_OK;
+}
+
+// w: a workspace from sep_work_acquire.
 int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
                            const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
                            SepWork w) {
@@ -615,41 +658,20 @@ int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, floa
         return CT2_EINVAL;
     }
     const size_t per = sep_view_bytes(dt);
-    void* own = nullptr;
-    int gv;
-    if (w.ptr) {
-        if (((uintptr_t)w.ptr & 15u) != 0) {
-            set_error("ct2_backproject: workspace must be 16-byte aligned");
-            return CT2_EINVAL;
-        }
-        const size_t fit = w.bytes / per;
-        if (fit < 1) {
-            set_error("ct2_backproject: workspace of %zu B holds no view's tables (%zu B)",
-                      w.bytes, per);
-            return CT2_EINVAL;
-        }
-        gv = (int)min((size_t)min(vs.nviews, SEP_MAXV), fit);
-    } else {
-        gv = (int)min((size_t)min(vs.nviews, SEP_MAXV), max((size_t)1, SEP_WS_CAP / per));
-        int dev = 0;
-        CT2_HIP(hipGetDevice(&dev));
-        keep_pool_memory(dev);
-        CT2_HIP(hipMallocAsync(&own, per * gv, st));
-        w.ptr = own;
+    if (!w.ptr || w.bytes < SEP_WS_HEAD + per) {
+        set_error("separable adjoint: no workspace");
+        return CT2_EINVAL;
     }
-    double2* tabs = static_cast<double2*>(w.ptr);
-    int rc = CT2_EINVAL;
+    const int gv = (int)min((size_t)min(vs.nviews, SEP_MAXV), (w.bytes - SEP_WS_HEAD) / per);
+    int* nonsep = static_cast<int*>(w.ptr);
+    double2* tabs = reinterpret_cast<double2*>(static_cast<char*>(w.ptr) + SEP_WS_HEAD);
     switch (cpl) {
-        case 1: rc = launch_sep_cpl<1>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
-        case 2: rc = launch_sep_cpl<2>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
-        case 3: rc = launch_sep_cpl<3>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
-        case 4: rc = launch_sep_cpl<4>(g, dt, vs, vol, sino, ind, fv, tabs, gv, st); break;
+        case 1: return launch_sep_cpl<1>(g, dt, vs, vol, sino, ind, fv, nonsep, tabs, gv, st);
+        case 2: return launch_sep_cpl<2>(g, dt, vs, vol, sino, ind, fv, nonsep, tabs, gv, st);
+        case 3: return launch_sep_cpl<3>(g, dt, vs, vol, sino, ind, fv, nonsep, tabs, gv, st);
+        case 4: return launch_sep_cpl<4>(g, dt, vs, vol, sino, ind, fv, nonsep, tabs, gv, st);
     }
-    if (own) {
-        const hipError_t e = hipFreeAsync(own, st);
-        if (rc == CT2_OK && e != hipSuccess) return hip_fail(e, "hipFreeAsync");
-    }
-    return rc;
+    return CT2_EINVAL;
 }
 
 }  // namespace ct2

@@ -486,6 +486,15 @@ __device__ inline bool view_separable(const float* __restrict__ f, const Grid& g
     return fabs((double)f[14] * (double)dt.dv / (double)g.d[2]) * dist >= maxlen / 48.0;
 }
 
+// Workgroup b of a grid of nblk runs on XCD b % 8 (the dispatcher deals
+// workgroups to the 8 XCDs in turn): the tile it takes, such that each XCD
+// gets a contiguous range of tiles, whose data then share its L2.  A
+// bijection of [0, nblk).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nblk) {
+    const unsigned x = b & 7u, i = b >> 3, q = nblk >> 3, rm = nblk & 7u;
+    return (x < rm ? x * (q + 1) : rm * (q + 1) + (x - rm) * q) + i;
+}
+
 // Restrict the inclusive plane range [l0,l1] to planes where the cell
 // coordinate u0 + l*du may lie in [lo, hi); the interval is widened by `pad`
 // cells and one plane so FP32 rounding of the exact per-sample test never
@@ -712,9 +721,18 @@ inline bool sep_route(const ct2_desc* d, const Grid& g) {
 int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
                            const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
                            SepWork w);
-// Bytes of the separable adjoint's column tables for every view of the subset
-// (0 when the route does not apply).
+// Bytes of the separable adjoint's workspace for every view of the subset (0
+// when the route does not apply): a 256-byte header (the count of views the
+// route leaves to the brick kernel) and the column tables.
 size_t sep_work_bytes(const ct2_desc* d);
+// A workspace for the route: the caller's (checked), or one allocated on the
+// stream (*own, released by sep_work_release after the launches that use
+// it).
+int sep_work_acquire(const Det& dt, const ViewSet& vs, SepWork& w, void** own, hipStream_t st);
+int sep_work_release(void* own, hipStream_t st);
+// The device count of views of the last separable launch that
+// view_separable rejected (in the workspace header).
+inline const int* sep_nonsep_count(const SepWork& w) { return static_cast<const int*>(w.ptr); }
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
                           hipStream_t st);
 int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,

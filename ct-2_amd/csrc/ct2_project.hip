@@ -522,14 +522,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Workgroup b of the grid runs on XCD b % 8 (round-robin dispatch): give
-// each XCD a contiguous range of tiles, so the columns that run together on
-// one XCD are neighbours and share their volume rows in its L2.
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nblk) {
-    const unsigned x = b & 7u, i = b >> 3, q = nblk >> 3, rm = nblk & 7u;
-    return (x < rm ? x * (q + 1) : rm * (q + 1) + (x - rm) * q) + i;
-}
-
 // Kahan step of one sample (the compensated sum of march_fast_batched).
 __device__ __forceinline__ void kahan_add(float val, float& sum, float& comp, bool ok) {
     const float y = val - comp;

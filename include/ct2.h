@@ -87,10 +87,10 @@ int ct2_project(const ct2_desc* desc, float* sino, const int8_t* ind, const floa
 int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
                     const float* fv, ct2_stream_t stream);
 /* ct2_backproject with a caller-owned device workspace (16-byte aligned) for
- * the column-separable route's tables: work_bytes >= one view's tables
- * (nu * (nv+1) * 16 bytes); ct2_backproject_work_bytes(desc) bytes hold the
- * tables of every view of the subset, and a smaller workspace processes the
- * views in groups.  The call allocates nothing; it must not overlap another
+ * the column-separable route: a 256-byte header and the column tables,
+ * work_bytes >= 256 + one view's tables (nu * (nv+1) * 16 bytes);
+ * ct2_backproject_work_bytes(desc) bytes hold the tables of every view of
+ * the subset, and a smaller workspace processes the views in groups.  The call allocates nothing; it must not overlap another
  * call that uses the same workspace.  work == NULL behaves as
  * ct2_backproject.  No counterpart in the reference (view_proj3D.f:1096-1370
  * has no workspace); for callers that keep a workspace across calls (graph
