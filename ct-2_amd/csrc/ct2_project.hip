@@ -98,6 +98,24 @@ __device__ __forceinline__ float lerp_cell(VT x00, VT x10, VT x01, VT x11, float
     return __builtin_fmaf(wc, vi2 - vi1, vi1);
 }
 
+// The fast ray sum: the samples of each batch of CT2_SUMB planes aligned at
+// multiples of CT2_SUMB (planes [8m, 8m+8)) are added in plane order in
+// FP32, and each batch's sum that holds a sample enters a compensated
+// (Kahan) sum.  A batch sum of at most eight samples carries rounding far
+// below the ray sum's, and the compensation costs one step per batch
+// instead of one per sample.  Every fast forward kernel sums this way, so
+// their projections agree bit for bit.
+constexpr int CT2_SUMB = 8;
+
+// Kahan step of one batch sum, taken where `any` (the batch held a sample).
+__device__ __forceinline__ void kahan_add(float part, float& sum, float& comp, bool any) {
+    const float y = part - comp;
+    const float ts = sum + y;
+    const float cn = (ts - sum) - y;
+    sum = any ? ts : sum;
+    comp = any ? cn : comp;
+}
+
 // march_fast for bilinear sampling of volumes with 32-bit byte offsets: the
 // planes are taken UB at a time and all their loads are issued before the
 // first is used, so a wavefront keeps 4*UB loads in flight instead of 4.  A
@@ -107,62 +125,65 @@ __device__ __forceinline__ float lerp_cell(VT x00, VT x10, VT x01, VT x11, float
 template <typename VT, bool ZC, int UB = CT2_FWD_BATCH>
 __device__ __forceinline__ float march_fast_batched(const RayFast& R, const VT* __restrict__ vol,
                                                     int l0, int l1) {
+    static_assert(CT2_SUMB % UB == 0, "plane steps within a batch");
     constexpr unsigned E = (unsigned)sizeof(VT);
     float sum = 0.0f, comp = 0.0f;
     const unsigned hb = (unsigned)(R.nb - 1), hc = (unsigned)(R.nc - 1);
     const unsigned db = (unsigned)R.sb * E;
     const unsigned dc = ZC ? E : (unsigned)R.sc * E;
-    float lf = (float)l0;
-    int base = l0 * R.sa;
-    for (int l = l0; l <= l1; l += UB, lf += (float)UB, base += UB * R.sa) {
-        float ub[UB], uc[UB];
-        bool ok[UB];
-        unsigned o[UB];
+    for (int m0 = l0 & ~(CT2_SUMB - 1); m0 <= l1; m0 += CT2_SUMB) {
+        float part = 0.0f;
+        bool any = false;
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            const float lfu = lf + (float)u;   // integer-valued: exact
-            ub[u] = __builtin_fmaf(lfu, R.dub, R.ub0);
-            uc[u] = __builtin_fmaf(lfu, R.duc, R.uc0);
-            const int ib = floor_i(ub[u]), ic = floor_i(uc[u]);
-            ok[u] = (l + u <= l1) & ((unsigned)ib < hb) & ((unsigned)ic < hc);
-            const int i00 = base + u * R.sa +
-                            (ZC ? __mul24(ib, R.sb) + ic : cell_offset<true>(ib, R.sb, ic, R.sc));
-            o[u] = ok[u] ? (unsigned)i00 * E : 0u;
-        }
-        // f16 pairs: along a contiguous c axis one aligned 4-byte load gives
-        // a corner and its c+1 neighbour
-        using XT = typename std::conditional<std::is_same<VT, h16x2>::value, __half, VT>::type;
-        XT x00[UB], x10[UB], x01[UB], x11[UB];
+        for (int h = 0; h < CT2_SUMB; h += UB) {
+            const int l = m0 + h;
+            float ub[UB], uc[UB];
+            bool ok[UB];
+            unsigned o[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            if constexpr (std::is_same<VT, h16x2>::value) {
-                const h16x2 p0 = *at_byte(vol, o[u]), p1 = *at_byte(vol, o[u] + db);
-                x00[u] = p0.lo;
-                x10[u] = p1.lo;
-                if constexpr (ZC) {
-                    x01[u] = p0.hi;
-                    x11[u] = p1.hi;
+            for (int u = 0; u < UB; ++u) {
+                const float lfu = (float)(l + u);   // integer-valued: exact
+                ub[u] = __builtin_fmaf(lfu, R.dub, R.ub0);
+                uc[u] = __builtin_fmaf(lfu, R.duc, R.uc0);
+                const int ib = floor_i(ub[u]), ic = floor_i(uc[u]);
+                ok[u] = (l + u >= l0) & (l + u <= l1) & ((unsigned)ib < hb) & ((unsigned)ic < hc);
+                const int i00 = (l + u) * R.sa + (ZC ? __mul24(ib, R.sb) + ic
+                                                     : cell_offset<true>(ib, R.sb, ic, R.sc));
+                o[u] = ok[u] ? (unsigned)i00 * E : 0u;
+            }
+            // f16 pairs: along a contiguous c axis one aligned 4-byte load gives
+            // a corner and its c+1 neighbour
+            using XT = typename std::conditional<std::is_same<VT, h16x2>::value, __half, VT>::type;
+            XT x00[UB], x10[UB], x01[UB], x11[UB];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                if constexpr (std::is_same<VT, h16x2>::value) {
+                    const h16x2 p0 = *at_byte(vol, o[u]), p1 = *at_byte(vol, o[u] + db);
+                    x00[u] = p0.lo;
+                    x10[u] = p1.lo;
+                    if constexpr (ZC) {
+                        x01[u] = p0.hi;
+                        x11[u] = p1.hi;
+                    } else {
+                        x01[u] = at_byte(vol, o[u] + dc)->lo;
+                        x11[u] = at_byte(vol, o[u] + db + dc)->lo;
+                    }
                 } else {
-                    x01[u] = at_byte(vol, o[u] + dc)->lo;
-                    x11[u] = at_byte(vol, o[u] + db + dc)->lo;
+                    x00[u] = *at_byte(vol, o[u]);
+                    x10[u] = *at_byte(vol, o[u] + db);
+                    x01[u] = *at_byte(vol, o[u] + dc);
+                    x11[u] = *at_byte(vol, o[u] + db + dc);
                 }
-            } else {
-                x00[u] = *at_byte(vol, o[u]);
-                x10[u] = *at_byte(vol, o[u] + db);
-                x01[u] = *at_byte(vol, o[u] + dc);
-                x11[u] = *at_byte(vol, o[u] + db + dc);
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const float val = lerp_cell(x00[u], x10[u], x01[u], x11[u], frac_pos(ub[u]),
+                                            frac_pos(uc[u]));
+                part += ok[u] ? val : 0.0f;
+                any |= ok[u];
             }
         }
-#pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            const float val = lerp_cell(x00[u], x10[u], x01[u], x11[u], frac_pos(ub[u]),
-                                        frac_pos(uc[u]));
-            const float y = val - comp;
-            const float t = sum + y;
-            const float cn = (t - sum) - y;
-            sum = ok[u] ? t : sum;
-            comp = ok[u] ? cn : comp;
-        }
+        kahan_add(part, sum, comp, any);
     }
     return sum * R.trav;
 }
@@ -177,47 +198,48 @@ __device__ __forceinline__ float march_fast(const RayFast& R, const VT* __restri
     const unsigned hc = (unsigned)(INTERP == CT2_LINEAR ? R.nc - 1 : (strict ? R.nc - 1 : R.nc));
     // [l0, l1] is wavefront-uniform: all lanes step the same plane together,
     // so one load instruction touches one plane's neighbourhood
-    // lf is the plane index as a float (exact), base its element offset
-    float lf = (float)l0;
-    int base = l0 * R.sa;
-    for (int l = l0; l <= l1; ++l, lf += 1.0f, base += R.sa) {
-        float ub = __builtin_fmaf(lf, R.dub, R.ub0);
-        float uc = __builtin_fmaf(lf, R.duc, R.uc0);
-        const int ib = floor_i(ub), ic = floor_i(uc);
-        if ((unsigned)ib >= hb || (unsigned)ic >= hc) continue;
-        // ZC: the c axis is the contiguous one (stride 1), so the c+1
-        // neighbours sit one element further, an immediate load offset
-        int i00 = base + (ZC ? (SMALL ? __mul24(ib, R.sb) : ib * R.sb) + ic
-                             : cell_offset<SMALL>(ib, R.sb, ic, R.sc));
-        float val;
-        if (INTERP == CT2_LINEAR) {
-            // (linear sampling reaches this loop only for volumes beyond
-            // 32-bit byte offsets; march_fast_batched covers the rest)
-            const int sc = ZC ? 1 : R.sc;
-            VT x00, x10, x01, x11;
-            if (SMALL) {
-                const unsigned o = (unsigned)i00 * (unsigned)sizeof(VT);
-                const unsigned db = (unsigned)R.sb * (unsigned)sizeof(VT);
-                const unsigned dc = (unsigned)sc * (unsigned)sizeof(VT);
-                x00 = *at_byte(vol, o);
-                x10 = *at_byte(vol, o + db);
-                x01 = *at_byte(vol, o + dc);
-                x11 = *at_byte(vol, o + db + dc);
+    for (int m0 = l0 & ~(CT2_SUMB - 1); m0 <= l1; m0 += CT2_SUMB) {
+        float part = 0.0f;
+        bool any = false;
+        for (int l = max(m0, l0); l <= min(m0 + CT2_SUMB - 1, l1); ++l) {
+            const float lf = (float)l;   // integer-valued: exact
+            float ub = __builtin_fmaf(lf, R.dub, R.ub0);
+            float uc = __builtin_fmaf(lf, R.duc, R.uc0);
+            const int ib = floor_i(ub), ic = floor_i(uc);
+            if ((unsigned)ib >= hb || (unsigned)ic >= hc) continue;
+            const int base = l * R.sa;
+            // ZC: the c axis is the contiguous one (stride 1), so the c+1
+            // neighbours sit one element further, an immediate load offset
+            int i00 = base + (ZC ? (SMALL ? __mul24(ib, R.sb) : ib * R.sb) + ic
+                                 : cell_offset<SMALL>(ib, R.sb, ic, R.sc));
+            float val;
+            if (INTERP == CT2_LINEAR) {
+                // (linear sampling reaches this loop only for volumes beyond
+                // 32-bit byte offsets; march_fast_batched covers the rest)
+                const int sc = ZC ? 1 : R.sc;
+                VT x00, x10, x01, x11;
+                if (SMALL) {
+                    const unsigned o = (unsigned)i00 * (unsigned)sizeof(VT);
+                    const unsigned db = (unsigned)R.sb * (unsigned)sizeof(VT);
+                    const unsigned dc = (unsigned)sc * (unsigned)sizeof(VT);
+                    x00 = *at_byte(vol, o);
+                    x10 = *at_byte(vol, o + db);
+                    x01 = *at_byte(vol, o + dc);
+                    x11 = *at_byte(vol, o + db + dc);
+                } else {
+                    x00 = vol[i00];
+                    x10 = vol[(size_t)i00 + R.sb];
+                    x01 = vol[(size_t)i00 + sc];
+                    x11 = vol[(size_t)i00 + R.sb + sc];
+                }
+                val = lerp_cell(x00, x10, x01, x11, frac_pos(ub), frac_pos(uc));
             } else {
-                x00 = vol[i00];
-                x10 = vol[(size_t)i00 + R.sb];
-                x01 = vol[(size_t)i00 + sc];
-                x11 = vol[(size_t)i00 + R.sb + sc];
+                val = Store<VT>::ld(vol, i00);
             }
-            val = lerp_cell(x00, x10, x01, x11, frac_pos(ub), frac_pos(uc));
-        } else {
-            val = Store<VT>::ld(vol, i00);
+            part += val;
+            any = true;
         }
-        // Kahan compensation
-        float y = val - comp;
-        float t = sum + y;
-        comp = (t - sum) - y;
-        sum = t;
+        kahan_add(part, sum, comp, any);
     }
     return sum * R.trav;
 }
@@ -447,14 +469,15 @@ __global__ __launch_bounds__(256) void k_project_rays(Grid g, int nr, int strict
 // its own LDS (coalesced row loads, the b lerp once per (column, plane, z)
 // instead of once per ray; the next batch's rows are loaded while the
 // current one is sampled), then each lane marches RPL rays of the column
-// through those rows.  Every sample, its arithmetic and the order of the
-// compensated sum are those of march_fast_batched, so the projections are
-// bit-identical to k_project_views'.  A wavefront whose view is not
+// through those rows.  Every sample, its arithmetic and the ray sum (the
+// batches of CT2_SUMB planes) are those of march_fast_batched, so the
+// projections are bit-identical to k_project_views'.  A wavefront whose view is not
 // separable, or whose rays unexpectedly disagree on a, ub0 or dub, runs the
 // per-ray march of k_project_views instead (same rows per call, so the same
 // results).
 constexpr int FS_WAVES = 4;   // detector columns per workgroup, one per wavefront
 constexpr int FS_P = 8;       // planes per batch (one row quad per lane and plane: nz <= 256)
+static_assert(FS_P == CT2_SUMB, "the row batches are the ray sum's batches");
 constexpr int FS_RPLMAX = 8;  // rays per lane at most
 // LDS floats per wavefront: the batch's blended rows, then the rays' weights
 constexpr int FS_WSLAB = FS_P * 256 + 64 * FS_RPLMAX;
@@ -520,15 +543,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Kahan step of one sample (the compensated sum of march_fast_batched).
-__device__ __forceinline__ void kahan_add(float val, float& sum, float& comp, bool ok) {
-    const float y = val - comp;
-    const float ts = sum + y;
-    const float cn = (ts - sum) - y;
-    sum = ok ? ts : sum;
-    comp = ok ? cn : comp;
 }
 
 // Grid: one workgroup per (view, chunk of 64*RPL rows, FS_WAVES columns),
@@ -665,23 +679,27 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
 #pragma unroll
         for (int q = 0; q < FS_P; ++q) {
             const int l = lb + q;
-            const bool ok = l <= lb1 && c4 < nc;
+            const bool ok = l >= lb0 && l <= lb1 && c4 < nc;
             const int ib = floor_i(__builtin_fmaf((float)l, dub, ub0));
             const unsigned o = ok ? (unsigned)(__mul24(l, sa) + __mul24(ib, sb) + c4) * esz : 0u;
             x0[q] = *reinterpret_cast<const raw*>(vb + o);
             x1[q] = *reinterpret_cast<const raw*>(vb + o + (ok ? (unsigned)sb * esz : 0u));
         }
     };
-    if (lb0 <= lb1) load_batch(lb0);
-    for (int lb = lb0; lb <= lb1; lb += FS_P) {
-        const int np = min(FS_P, lb1 - lb + 1);
+    // batches aligned at multiples of FS_P = CT2_SUMB planes: the batches of
+    // the fast ray sum
+    const int lbs = lb0 & ~(FS_P - 1);
+    if (lb0 <= lb1) load_batch(lbs);
+    for (int lb = lbs; lb <= lb1; lb += FS_P) {
+        // planes of the batch whose b cell is inside the volume: [qa, qe]
+        const int qa = max(lb0 - lb, 0), qe = min(lb1 - lb, FS_P - 1);
         float lfq[FS_P];   // the batch's plane indices as floats (exact), in scalar registers
 #pragma unroll
         for (int q = 0; q < FS_P; ++q) lfq[q] = uniform_f((float)(lb + q));
         // the batch's rows, lerped across b, into LDS
 #pragma unroll
         for (int q = 0; q < FS_P; ++q) {
-            if (q < np && c4 < nc)
+            if (q >= qa && q <= qe && c4 < nc)
                 *reinterpret_cast<float4*>(rows + q * nc + c4) =
                     RQ::blend(x0[q], x1[q], frac_pos(__builtin_fmaf(lfq[q], dub, ub0)));
         }
@@ -694,7 +712,9 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
             // planes of a ray are an interval): then no sample needs a mask
             const int ia = floor_i(__builtin_fmaf(lfq[0], duc[r], uc0[r]));
             const int ie = floor_i(__builtin_fmaf(lfq[FS_P - 1], duc[r], uc0[r]));
-            if (np == FS_P && __all((unsigned)ia < hc && (unsigned)ie < hc)) {
+            float part = 0.0f;
+            bool any = true;
+            if (qa == 0 && qe == FS_P - 1 && __all((unsigned)ia < hc && (unsigned)ie < hc)) {
 #pragma unroll
                 for (int h = 0; h < FS_P; h += 4) {
                     float val[4];
@@ -707,9 +727,10 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
                         val[q] = __builtin_fmaf(frac_pos(uc), vi2 - vi1, vi1);
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) kahan_add(val[q], sum[r], comp[r], true);
+                    for (int q = 0; q < 4; ++q) part += val[q];
                 }
             } else {
+                any = false;
 #pragma unroll
                 for (int h = 0; h < FS_P; h += 4) {
                     float val[4];
@@ -718,16 +739,20 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
                     for (int q = 0; q < 4; ++q) {
                         const float uc = __builtin_fmaf(lfq[h + q], duc[r], uc0[r]);
                         const int ic = floor_i(uc);
-                        ok[q] = (unsigned)ic < hc && h + q < np;
+                        ok[q] = (unsigned)ic < hc && h + q >= qa && h + q <= qe;
                         const int icc = ok[q] ? ic : 0;
                         const float* row = rows + (h + q) * nc;
                         const float vi1 = row[icc], vi2 = row[icc + 1];
                         val[q] = __builtin_fmaf(frac_pos(uc), vi2 - vi1, vi1);
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) kahan_add(val[q], sum[r], comp[r], ok[q]);
+                    for (int q = 0; q < 4; ++q) {
+                        part += ok[q] ? val[q] : 0.0f;
+                        any |= ok[q];
+                    }
                 }
             }
+            kahan_add(part, sum[r], comp[r], any);
         }
         wave_lds_sync();
     }
