@@ -72,12 +72,13 @@ constexpr int SEP_RMAX = 16;                    // table rows per lane in k_sep_
 
 // Per (column, tile plane): the linear model uc = A + B kk over stored rows
 // kk (FP64, B > 0); one past the crossing of z, t(z) + 1 = z rB + nA1
-// (FP32); the face rows r_0 and r_nc-1 (as floats); the forward's b cell and
-// its two weights.
+// (FP32); the face rows r_0 and r_nc-1 (as floats); the weight of each b row
+// of the tile from the forward's b cell ib and weight wb: 1-wb at ib, wb at
+// ib+1, 0 elsewhere.
 struct SepPlane {
     double A, B;
-    float rB, nA1, rbot, rtop, w0, w1;
-    int ib;
+    float rB, nA1, rbot, rtop;
+    float wr[SEP_TB];
 };
 
 // A column's table: the prefix pairs (PS, PK) of entries 0 .. nv, 16 bytes
@@ -271,10 +272,14 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                             const int l = P0 + lane;
                             const float ubl = __builtin_fmaf((float)l, dub, ub0);
                             SepPlane sp;
-                            sp.ib = floor_i(ubl);
-                            sp.w1 = frac_pos(ubl);
-                            sp.w0 = 1.0f - sp.w1;
-                            ok = sp.ib >= lo && sp.ib <= hi;
+                            const int ibl = floor_i(ubl);
+                            const float w1 = frac_pos(ubl), w0 = 1.0f - w1;
+                            ok = ibl >= lo && ibl <= hi;
+#pragma unroll
+                            for (int bi = 0; bi < SEP_TB; ++bi) {
+                                const int dj = bi - (ibl - Q0);
+                                sp.wr[bi] = dj == 0 ? w0 : (dj == 1 ? w1 : 0.0f);
+                            }
                             // uc of row k on plane l: A(l) + B(l) k, in z cells
                             const double T = fma((double)l, da, ta);
                             sp.A = fma(fma(R0, T, ScOc), idc, -0.5);
@@ -312,8 +317,9 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                             // adds nothing
                             SepPlane sp;
                             sp.A = sp.B = 0.0;
-                            sp.rB = sp.nA1 = sp.rbot = sp.rtop = sp.w0 = sp.w1 = 0.0f;
-                            sp.ib = Q0;
+                            sp.rB = sp.nA1 = sp.rbot = sp.rtop = 0.0f;
+#pragma unroll
+                            for (int bi = 0; bi < SEP_TB; ++bi) sp.wr[bi] = 0.0f;
                             s_pl[wave][lane] = sp;
                         }
                         mask = (unsigned)__ballot(ok);
@@ -339,7 +345,6 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                     const double Ad = sp.A, Bd = sp.B;
                     const float rB = uniform_f(sp.rB), nA1 = uniform_f(sp.nA1);
                     const float rbot = uniform_f(sp.rbot), rtop = uniform_f(sp.rtop);
-                    const int ib = __builtin_amdgcn_readfirstlane(sp.ib);
                     const float lf = (float)(P0 + q);
                     // cell boundaries z = c0 .. c0+CPL (at most nc-1, so the
                     // cells from nc-1 up, which the forward never samples, are
@@ -369,18 +374,11 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                     // the cell below (the previous lane's last cell for cc = 0)
                     const double carry = dpp_f64<0x138, 0xf>(hi[CPL - 1]);   // wave_shr:1
                     // b rows of the tile: ib takes 1-wb, ib+1 takes wb, the
-                    // others 0 (uniform weights, so every row is one FMA with
-                    // a scalar operand)
-                    // (selected on the scalar unit from the record's two weights)
-                    const int w0b = __builtin_amdgcn_readfirstlane(__float_as_int(sp.w0));
-                    const int w1b = __builtin_amdgcn_readfirstlane(__float_as_int(sp.w1));
-                    const int jj = ib - Q0;
+                    // others 0, as the record holds them (the same on every
+                    // lane, so every row is one FMA per z cell)
                     float wr[SEP_TB];
 #pragma unroll
-                    for (int bi = 0; bi < SEP_TB; ++bi) {
-                        const unsigned dj = (unsigned)(bi - jj);
-                        wr[bi] = __int_as_float(dj == 0u ? w0b : (dj == 1u ? w1b : 0));
-                    }
+                    for (int bi = 0; bi < SEP_TB; ++bi) wr[bi] = sp.wr[bi];
 #pragma unroll
                     for (int cc = 0; cc < CPL; ++cc) {
                         const float G = (float)(lo[cc] + (cc == 0 ? carry : hi[cc - 1]));
