@@ -12,6 +12,17 @@
 // write-back; row sums A1 and the per-subset column sums A_s^T 1 are computed
 // once when the plan is made.  The entry points after creation allocate
 // nothing and synchronise nothing, so they can be captured.
+//
+// A plan of ONE problem whose forward and adjoint both take the
+// column-separable kernels (fast linear arithmetic, detector rows along z)
+// runs SART subset by subset through them instead: the forward of the subset
+// (ct2_project's route), the residual, the adjoint into a plan-owned
+// correction volume with a plan-owned column-table workspace (ct2_backproject's
+// route), and the update.  Row and column sums come from the same kernels, so
+// each subset step equals ct2_sart_step on that subset bit for bit.  At
+// 256^3 x 360 views x 512^2 (10 subsets) the batched kernels take 5.6 ms per
+// subset adjoint and 1.55 ms per subset forward, the separable ones about
+// 1.8 and 0.6 ms.
 #include <string.h>
 
 #include <algorithm>
@@ -35,6 +46,12 @@ struct ct2_batch {
     unsigned long long* d_queue = nullptr;   // work queues: [0,1] forward, [2,3] adjoint
     bool has_proj = false, has_adj = false;
     size_t bytes = 0;
+    // one-problem plans on the column-separable kernels
+    bool sep = false;
+    ct2_desc sd{};             // the problem's descriptor (subset fields set per call)
+    float* corr = nullptr;     // A_s^T r of the current subset
+    void* work = nullptr;      // column tables of the largest subset
+    size_t work_bytes = 0;
 };
 
 namespace {
@@ -48,7 +65,83 @@ void release(ct2_batch* b) {
     hipFree(b->resid);
     hipFree(b->colsum);
     hipFree(b->d_queue);
+    hipFree(b->corr);
+    hipFree(b->work);
     delete b;
+}
+
+// r = sart_resid(b, A_s x, rowsum, used) over the views of subset view0,
+// in place on the forward's output
+__global__ void k_plan_resid(int view0, int vstride, size_t per_view,
+                             const float* __restrict__ sino, const int8_t* __restrict__ ind,
+                             const float* __restrict__ rowsum, float* __restrict__ resid) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= per_view) return;
+    const size_t q = (size_t)(view0 + blockIdx.y * vstride) * per_view + e;
+    resid[q] = sart_resid(sino[q], resid[q], rowsum[q], ind == nullptr || ind[q] == 1);
+}
+
+// x = sart_update(x, lambda, corr, colsum); lambda_dev, when given, holds
+// the relaxation
+__global__ void k_plan_update(size_t n, float lambda, const float* __restrict__ lambda_dev,
+                              float* __restrict__ vol, const float* __restrict__ corr,
+                              const float* __restrict__ colsum) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (lambda_dev) lambda = *lambda_dev;
+    vol[q] = sart_update(vol[q], lambda, corr[q], colsum[q]);
+}
+
+ct2_desc subset_desc(const ct2_batch* b, int sub) {
+    ct2_desc d = b->sd;
+    d.view0 = sub;
+    d.nviews = 0;
+    d.vstride = b->nsub;
+    return d;
+}
+
+// One SART subset of a one-problem separable plan.
+int sep_sart_subset(ct2_batch* b, int sub, float lambda, const float* lambda_dev,
+                    hipStream_t st) {
+    const BatchProb& P = b->h[0];
+    const int nv = subset_views(P.dt.ns, sub, b->nsub);
+    if (nv == 0) return CT2_OK;
+    const ct2_desc d = subset_desc(b, sub);
+    int rc = launch_project(&d, P.resid, P.ind, P.fv, P.vol, st);
+    if (rc) return rc;
+    const size_t per_view = (size_t)P.dt.nu * P.dt.nv;
+    hipLaunchKernelGGL(k_plan_resid, dim3((unsigned)((per_view + 255) / 256), (unsigned)nv),
+                       dim3(256), 0, st, sub, b->nsub, per_view, P.sino, P.ind, P.rowsum,
+                       P.resid);
+    SepWork w;
+    w.ptr = b->work;
+    w.bytes = b->work_bytes;
+    if ((rc = launch_backproject(&d, b->corr, P.resid, P.ind, P.fv, st, w))) return rc;
+    const size_t n = (size_t)P.nvox;
+    hipLaunchKernelGGL(k_plan_update, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                       lambda, lambda_dev, P.vol, b->corr, P.colsum + (size_t)sub * n);
+    return CT2_OK;
+}
+
+// Row sums over every view and the column sums of each subset, by the
+// separable kernels (resid serves as the ones sinogram).
+int sep_sums(ct2_batch* b, const float* ones, size_t nsino, hipStream_t st) {
+    const BatchProb& P = b->h[0];
+    ct2_desc all = b->sd;
+    all.view0 = 0;
+    all.nviews = 0;
+    all.vstride = 1;
+    int rc = launch_project(&all, P.rowsum, P.ind, P.fv, ones, st);
+    if (!rc) rc = launch_fill(P.resid, 1.0f, nsino, st);
+    SepWork w;
+    w.ptr = b->work;
+    w.bytes = b->work_bytes;
+    for (int s = 0; s < b->nsub && !rc; ++s) {
+        if (subset_views(P.dt.ns, s, b->nsub) == 0) continue;
+        const ct2_desc d = subset_desc(b, s);
+        rc = launch_backproject(&d, P.colsum + (size_t)s * P.nvox, P.resid, P.ind, P.fv, st, w);
+    }
+    return rc;
 }
 
 // Item offsets of every subset row (and the all-views row) for tile height tu.
@@ -221,6 +314,19 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
         if ((rc = upload(b->d_queue, zero, sizeof zero, st))) return rc;
     }
     b->bytes = sizeof(float) * (2 * nsino_tot + nvox_tot * b->nsub);
+    if (b->nprob == 1) {
+        const char* env = getenv("CT2_PLAN_SEP");
+        b->sd = descs[0];
+        const ct2_desc d0 = subset_desc(b, 0);
+        b->sep = !(env && env[0] == '0') && project_sep_route(&d0, vol[0]) &&
+                 sep_route(&d0, b->h[0].g);
+        if (b->sep) {
+            b->work_bytes = sep_work_bytes(&d0);   // subset 0 has the most views
+            CT2_HIP(hipMalloc(&b->corr, sizeof(float) * nvox_tot));
+            CT2_HIP(hipMalloc(&b->work, b->work_bytes));
+            b->bytes += sizeof(float) * nvox_tot + b->work_bytes;
+        }
+    }
     size_t so = 0, vo = 0;
     for (int p = 0; p < b->nprob; ++p) {
         BatchProb& P = b->h[p];
@@ -236,7 +342,8 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
     if (!rc) rc = set_bp(b, b->bdim, st);
     if (!rc) rc = set_fwd(b, 4, st);
     const char* env = getenv("CT2_AUTOTUNE");
-    if (!rc && tune && !(env && env[0] == '0')) {
+    // (a separable plan's SART uses neither the tile nor the brick)
+    if (!rc && tune && !(env && env[0] == '0') && !(b->sep && !b->has_proj && !b->has_adj)) {
         // tile height of the forward: time the all-views row-sum projection
         float best = 1e30f;
         int best_tu = 4;
@@ -262,9 +369,13 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
         if (!rc) rc = set_bp(b, best_b, st);
     }
     // row sums over every view, then the column sums of each subset
-    if (!rc) rc = fwd(b, BFWD_ROWSUM, 0, ones, st);
-    if (!rc) rc = launch_fill(b->resid, 1.0f, nsino_tot, st);
-    for (int s = 0; s < b->nsub && !rc; ++s) rc = bp(b, BBP_COLSUM, s, 0.0f, st);
+    if (b->sep) {
+        if (!rc) rc = sep_sums(b, ones, nsino_tot, st);
+    } else {
+        if (!rc) rc = fwd(b, BFWD_ROWSUM, 0, ones, st);
+        if (!rc) rc = launch_fill(b->resid, 1.0f, nsino_tot, st);
+        for (int s = 0; s < b->nsub && !rc; ++s) rc = bp(b, BBP_COLSUM, s, 0.0f, st);
+    }
     hipError_t he = hipStreamSynchronize(st);
     hipFree(ones);
     if (rc) return rc;
@@ -344,8 +455,13 @@ static int batch_sart(const char* who, ct2_batch* plan, float lambda, const floa
     hipStream_t st = (hipStream_t)stream;
     for (int it = 0; it < niter; ++it)
         for (int s = 0; s < plan->nsub; ++s) {
-            int rc = fwd(plan, BFWD_RESID, s, nullptr, st);
-            if (!rc) rc = bp(plan, BBP_SART, s, lambda, st, lambda_dev);
+            int rc;
+            if (plan->sep) {
+                rc = sep_sart_subset(plan, s, lambda, lambda_dev, st);
+            } else {
+                rc = fwd(plan, BFWD_RESID, s, nullptr, st);
+                if (!rc) rc = bp(plan, BBP_SART, s, lambda, st, lambda_dev);
+            }
             if (rc) return rc;
         }
     CT2_HIP(hipGetLastError());

@@ -685,6 +685,9 @@ int view_set(const ct2_desc* d, const char* who, ViewSet& vs);
 // ------------------------------------------------------ launchers (host) ----
 int launch_project(const ct2_desc* desc, void* sino, const int8_t* ind, const float* fv,
                    const void* vol, hipStream_t st);
+// The forward of *desc on the volume at vol runs the column-separable kernel
+// (which neither tunes nor synchronises).
+bool project_sep_route(const ct2_desc* desc, const void* vol);
 // Caller-owned device workspace of the column-separable adjoint (its column
 // tables); a null pointer lets the launch allocate one on the stream.
 struct SepWork {

@@ -956,6 +956,11 @@ static int tuned_project(const ct2_desc* d, const Grid& g, const Det& dt, const 
     return CT2_OK;
 }
 
+bool project_sep_route(const ct2_desc* d, const void* vol) {
+    Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);
+    return fsep_supported(d, g, vol);
+}
+
 int launch_project(const ct2_desc* d, void* sino, const int8_t* ind, const float* fv,
                    const void* vol, hipStream_t st) {
     Grid g = make_grid(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, d->x0, d->y0, d->z0);

@@ -141,7 +141,13 @@ int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const i
  * (graph-capturable).  Calls on one plan must not overlap: the plan's
  * kernels hand out work items from a counter in the plan, which each launch
  * leaves at zero for the next one, so a plan is used on one stream at a
- * time (and a launch that fails part-way leaves the plan unusable). */
+ * time (and a launch that fails part-way leaves the plan unusable).
+ * A plan of one problem whose forward and adjoint both take the
+ * column-separable kernels (fast linear arithmetic, free axis, rows along
+ * z, nz % 4 == 0, nz <= 256, 16-byte aligned volume; CT2_PLAN_SEP=0 turns
+ * this off) runs SART through them instead, with a correction volume and
+ * the column tables of one subset allocated at creation: each subset step
+ * then equals ct2_sart_step bit for bit, and still allocates nothing. */
 typedef struct ct2_batch ct2_batch;
 int ct2_batch_create(ct2_batch** plan, int nprob, const ct2_desc* descs,
                      const float* const* frame_vectors, const int8_t* const* indicators,

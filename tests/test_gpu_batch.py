@@ -7,7 +7,9 @@ the forward equals ConeBeamOperator.forward bit for bit (same device code);
 the adjoint equals ConeBeamOperator.adjoint to LDS summation order (1e-6);
 SART over interleaved subsets equals ct2_sart_step with view0 = s,
 vstride = nsub applied problem by problem (1e-5); replays of a captured SART
-iteration equal eager iterations.
+iteration equal eager iterations.  A plan of one problem on the
+column-separable kernels gives ct2_sart_step's updates bit for bit, and agrees
+with the batched kernels (CT2_PLAN_SEP=0) to FP32 summation order.
 """
 import numpy as np
 import pytest
@@ -125,3 +127,108 @@ def test_batch_rejects_bad_input():
     with pytest.raises(CT2Error, match="no projection outputs"):
         plan.project()
     plan.close()
+
+
+# ---- one-problem plans on the column-separable kernels ----
+def _sep_problem(shape=(36, 28, 40), ns=20, nu=50, nv=44, tilt_every=0, seed=21):
+    from ct2 import configs3D, phantoms
+    from ct2.operator import ConeBeamOperator
+    rng = np.random.default_rng(seed)
+    fv = configs3D.frame_vectors("circular_conebeam", {"radius": 300.0, "source_to_detector": 600.0},
+                                 ns, 2 * np.pi, 0.2, 0).astype(np.float64)
+    for p in range(0, ns, tilt_every or ns + 1):
+        # e_v leans 2 degrees towards e_u: that view's rows no longer run along z
+        eu, ev = fv[p, 9:12].copy(), fv[p, 12:15].copy()
+        a = np.deg2rad(2.0)
+        fv[p, 12:15] = np.cos(a) * ev + np.sin(a) * eu
+        fv[p, 9:12] = np.cos(a) * eu - np.sin(a) * ev
+    ind = (rng.random((ns, nu, nv)) > 0.04).astype(np.int8)
+    hw = np.array(shape) / 2.0
+    op = ConeBeamOperator(fv.astype(np.float32), ns, nu, nv, 1.6, 1.6, -nu * 0.8, -nv * 0.8,
+                          shape, (1.0, 1.0, 1.0), tuple(-hw), arith="fast", indicator=ind)
+    n = max(shape)
+    cube = phantoms.random_ellipsoids(n, 0.45 * n, 6, seed)[:shape[0], :shape[1], :shape[2]]
+    v = torch.from_numpy(np.ascontiguousarray(cube)).cuda()
+    return op, v, op.forward(v)
+
+
+def _sart_steps(op, sino, nsub, lam, niter):
+    x = torch.zeros(op.shape, dtype=torch.float32, device="cuda")
+    for _ in range(niter):
+        for s in range(min(nsub, op.sino_shape[0])):
+            op.sart_step(sino, x, lam, view0=s, nviews=0, vstride=nsub)
+    return x
+
+
+@pytest.mark.parametrize("case", ["plain", "tilted_views", "more_subsets_than_views"])
+def test_one_problem_plan_sart_equals_sart_step(case):
+    """A one-problem plan on the separable kernels gives ct2_sart_step's
+    subset updates bit for bit (the same forward, residual, adjoint, row and
+    column sums); tilted views go through the brick pass of the adjoint in
+    both."""
+    from ct2.batch import BatchPlan
+    kw = {"plain": {}, "tilted_views": dict(tilt_every=3),
+          "more_subsets_than_views": dict(ns=3)}[case]
+    op, v, sino = _sep_problem(**kw)
+    nsub, lam, niter = 4, 0.9, 2
+    x = torch.zeros_like(v)
+    plan = BatchPlan([op], [sino], [x], nsub=nsub)
+    info = plan.info()
+    nvox, nsino = v.numel(), sino.numel()
+    # the route's extra memory: a correction volume and one subset's tables
+    assert info["bytes"] > 4 * (2 * nsino + nvox * nsub) + 4 * nvox
+    plan.sart(lam, niter)
+    ref = _sart_steps(op, sino, nsub, lam, niter)
+    assert torch.equal(x, ref)
+    assert rel_l2(x.cpu().numpy(), v.cpu().numpy()) < 0.9
+    plan.close()
+
+
+def test_one_problem_plan_routes_agree(monkeypatch):
+    """CT2_PLAN_SEP=0 keeps the batched kernels for the one-problem plan:
+    the two routes agree to FP32 summation order."""
+    from ct2.batch import BatchPlan
+    op, v, sino = _sep_problem(seed=22)
+    xs, xb = torch.zeros_like(v), torch.zeros_like(v)
+    ps = BatchPlan([op], [sino], [xs], nsub=5, tune=False)
+    monkeypatch.setenv("CT2_PLAN_SEP", "0")
+    pb = BatchPlan([op], [sino], [xb], nsub=5, tune=False)
+    ps.sart(1.0, 2)
+    pb.sart(1.0, 2)
+    e = rel_l2(xs.cpu().numpy(), xb.cpu().numpy())
+    assert 0 < e < 1e-5, e
+    assert pb.info()["bytes"] < ps.info()["bytes"]
+    ps.close()
+    pb.close()
+
+
+def test_one_problem_plan_graph_replays_with_device_lambda():
+    """ct2_batch_sart_dev on the separable route, captured once and replayed
+    with the relaxation changed on the device between replays, equals the
+    eager calls bit for bit."""
+    from ct2 import _dev
+    from ct2._lib import call
+    from ct2.batch import BatchPlan
+    from ct2.solvers import capture
+    op, v, sino = _sep_problem(shape=(32, 32, 48), ns=30, nu=56, nv=60, seed=23)
+    xe, xg = torch.zeros_like(v), torch.zeros_like(v)
+    pe = BatchPlan([op], [sino], [xe], nsub=6, tune=False)
+    pg = BatchPlan([op], [sino], [xg], nsub=6, tune=False)
+    lam = torch.ones((1,), dtype=torch.float32, device="cuda")
+
+    def step(plan):
+        call("ct2_batch_sart_dev", plan._h, _dev.ptr(lam), 1, _dev.stream_handle())
+
+    lam.fill_(1.0)
+    step(pg)
+    g = capture(lambda: step(pg))
+    step(pe)
+    for k, val in enumerate((0.8, 0.6, 0.5)):
+        lam.fill_(val)
+        g.replay()
+        step(pe)
+        torch.cuda.synchronize()
+        assert torch.equal(xg, xe), k
+    assert rel_l2(xe.cpu().numpy(), v.cpu().numpy()) < 0.5
+    pe.close()
+    pg.close()
