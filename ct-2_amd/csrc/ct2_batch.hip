@@ -23,6 +23,18 @@
 // 256^3 x 360 views x 512^2 (10 subsets) the batched kernels take 5.6 ms per
 // subset adjoint and 1.55 ms per subset forward, the separable ones about
 // 1.8 and 0.6 ms.
+//
+// In a plan of several problems, SART's adjoint takes the column-separable
+// tiles for every problem whose views all have their rows along z (fast
+// linear arithmetic, nz <= 256, at most 512 views per subset): one launch
+// builds the column tables of all those problems' subset views, then per
+// z-cell class (nz up to 64, 128, 192, 256) one launch runs their x-driven
+// tiles, storing the sums in a plan-owned correction volume, and one their
+// y-driven tiles, adding their sums and applying the update.  Each problem's
+// voxel sums are then those of ct2_backproject's two commuting atomic adds,
+// so these problems' SART steps equal ct2_sart_step bit for bit too.  The
+// persistent brick kernel keeps the other problems (its offsets leave the
+// separable ones out), the all-view backprojection, and the forward.
 #include <string.h>
 
 #include <algorithm>
@@ -52,6 +64,16 @@ struct ct2_batch {
     float* corr = nullptr;     // A_s^T r of the current subset
     void* work = nullptr;      // column tables of the largest subset
     size_t work_bytes = 0;
+    // several problems: the separable tiles for problems psep[p]
+    std::vector<char> psep;
+    bool sepb = false, brick_any = true;
+    SepBatchProb* d_sprobs = nullptr;
+    long long* d_stoff = nullptr;    // per subset: table items of each problem (nprob+1)
+    long long* d_sxoff = nullptr;    // per (subset, class, axis): tiles of each problem
+    std::vector<long long> stab_items, stiles;
+    int max_nv[5] = {0, 0, 0, 0, 0};
+    void* sep_mem = nullptr;         // tables and correction volumes of those problems
+    int* d_nonsep = nullptr;
 };
 
 namespace {
@@ -67,8 +89,94 @@ void release(ct2_batch* b) {
     hipFree(b->d_queue);
     hipFree(b->corr);
     hipFree(b->work);
+    hipFree(b->d_sprobs);
+    hipFree(b->d_stoff);
+    hipFree(b->d_sxoff);
+    hipFree(b->sep_mem);
+    hipFree(b->d_nonsep);
     delete b;
 }
+
+// Item offsets of every subset row (and the all-views row) for tile height tu.
+std::vector<long long> fwd_offsets(const ct2_batch* b, int tu) {
+    const int n1 = b->nprob + 1;
+    std::vector<long long> off((size_t)(b->nsub + 1) * n1, 0);
+    for (int row = 0; row <= b->nsub; ++row) {
+        long long acc = 0;
+        for (int p = 0; p < b->nprob; ++p) {
+            const Det& dt = b->h[p].dt;
+            off[(size_t)row * n1 + p] = acc;
+            int nv = row == b->nsub ? dt.ns : subset_views(dt.ns, row, b->nsub);
+            acc += (long long)nv * ((dt.nu + tu - 1) / tu) * ((dt.nv + 63) / 64);
+        }
+        off[(size_t)row * n1 + b->nprob] = acc;
+    }
+    return off;
+}
+
+std::vector<long long> bp_offsets(ct2_batch* b, int3 bd) {
+    const int n1 = b->nprob + 1;
+    std::vector<long long> off((size_t)(b->nsub + 1) * n1, 0);
+    for (int p = 0; p < b->nprob; ++p) {
+        const Grid& g = b->h[p].g;
+        b->h[p].nbr = make_int3((g.n[0] + bd.x - 1) / bd.x, (g.n[1] + bd.y - 1) / bd.y,
+                                (g.n[2] + bd.z - 1) / bd.z);
+    }
+    for (int row = 0; row <= b->nsub; ++row) {
+        long long acc = 0;
+        for (int p = 0; p < b->nprob; ++p) {
+            const BatchProb& P = b->h[p];
+            off[(size_t)row * n1 + p] = acc;
+            int nv = row == b->nsub ? P.dt.ns : subset_views(P.dt.ns, row, b->nsub);
+            // (the separable tiles take these problems' subsets)
+            if (row < b->nsub && b->sepb && b->psep[p]) nv = 0;
+            if (nv > 0) acc += (long long)P.nbr.x * P.nbr.y * P.nbr.z;
+        }
+        off[(size_t)row * n1 + b->nprob] = acc;
+    }
+    return off;
+}
+
+int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    CT2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    CT2_HIP(hipStreamSynchronize(st));
+    return CT2_OK;
+}
+
+int persistent_grid(int blocks_per_cu, int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    int g = std::max(1, blocks_per_cu) * cus;
+    return std::max(8, g & ~7);
+}
+
+int set_fwd(ct2_batch* b, int tu, hipStream_t st) {
+    std::vector<long long> off = fwd_offsets(b, tu);
+    int rc = upload(b->d_foff, off.data(), off.size() * sizeof(long long), st);
+    if (rc) return rc;
+    int occ = 0;
+    if ((rc = batch_project_occupancy(b->interp, b->arith, tu, BFWD_RESID, &occ))) return rc;
+    b->tu = tu;
+    b->fgrid = persistent_grid(occ, b->device);
+    return CT2_OK;
+}
+
+int set_bp(ct2_batch* b, int3 bd, hipStream_t st) {
+    std::vector<long long> off = bp_offsets(b, bd);
+    int rc;
+    if ((rc = upload(b->d_probs, b->h.data(), sizeof(BatchProb) * b->nprob, st))) return rc;
+    if ((rc = upload(b->d_boff, off.data(), off.size() * sizeof(long long), st))) return rc;
+    int occ = 0;
+    if ((rc = batch_backproject_occupancy(b->interp, b->arith, BBP_SART, bd, &occ))) return rc;
+    b->bdim = bd;
+    b->bgrid = persistent_grid(occ, b->device);
+    return CT2_OK;
+}
+
+const long long* frow(const ct2_batch* b, int row) { return b->d_foff + (size_t)row * (b->nprob + 1); }
+const long long* brow(const ct2_batch* b, int row) { return b->d_boff + (size_t)row * (b->nprob + 1); }
 
 // r = sart_resid(b, A_s x, rowsum, used) over the views of subset view0,
 // in place on the forward's output
@@ -123,6 +231,133 @@ int sep_sart_subset(ct2_batch* b, int sub, float lambda, const float* lambda_dev
     return CT2_OK;
 }
 
+const long long* sxrow(const ct2_batch* b, int sub, int cls, int ax) {
+    return b->d_sxoff + ((size_t)(sub * 4 + cls - 1) * 2 + ax) * (b->nprob + 1);
+}
+
+// Subset sub's adjoint on the separable tiles of the problems that take them
+// (kind BSEP_SART: update those estimates; BSEP_COLSUM: their column sums of
+// the subset, from resid holding ones).
+int sepb_subset(ct2_batch* b, int kind, int sub, float lambda, const float* lambda_dev,
+                hipStream_t st) {
+    const size_t n1 = (size_t)b->nprob + 1;
+    int rc = launch_batch_sep_tables(b->d_sprobs, b->d_stoff + (size_t)sub * n1, b->nprob, sub,
+                                     b->nsub, b->stab_items[sub], b->d_nonsep, st);
+    for (int cls = 1; cls <= 4 && !rc; ++cls)
+        for (int ax = 0; ax < 2 && !rc; ++ax)
+            rc = launch_batch_bp_sep(cls, ax, kind, b->d_sprobs, sxrow(b, sub, cls, ax), b->nprob,
+                                     sub, b->nsub,
+                                     b->stiles[((size_t)sub * 4 + cls - 1) * 2 + ax],
+                                     b->max_nv[cls], lambda, lambda_dev, st);
+    return rc;
+}
+
+// Which problems of a several-problem plan take the separable tiles, and
+// their tables, correction volumes and offsets (synchronises the stream).
+int sepb_setup(ct2_batch* b, const ct2_desc* descs, hipStream_t st) {
+    const char* env = getenv("CT2_PLAN_SEP");
+    if (env && env[0] == '0') return CT2_OK;
+    const int np = b->nprob, ns1 = b->nsub;
+    b->psep.assign(np, 0);
+    std::vector<char> cand(np, 0);
+    bool any = false;
+    for (int p = 0; p < np; ++p) {
+        const BatchProb& P = b->h[p];
+        cand[p] = sep_route(&descs[p], P.g) && subset_views(P.dt.ns, 0, ns1) <= 512;
+        any |= cand[p] != 0;
+    }
+    if (!any) return CT2_OK;
+    // every view of a candidate must be separable
+    CT2_HIP(hipMalloc(&b->d_nonsep, sizeof(int) * (np + 1)));
+    int rc = CT2_OK;
+    {
+        std::vector<int> zero(np + 1, 0);
+        CT2_HIP(hipMemcpyAsync(b->d_nonsep, zero.data(), sizeof(int) * (np + 1),
+                               hipMemcpyHostToDevice, st));
+        for (int p = 0; p < np && !rc; ++p)
+            if (cand[p]) rc = launch_count_nonsep(b->h[p].g, b->h[p].dt, b->h[p].fv, b->d_nonsep + p, st);
+        if (rc) return rc;
+        std::vector<int> cnt(np + 1, 0);
+        CT2_HIP(hipMemcpyAsync(cnt.data(), b->d_nonsep, sizeof(int) * (np + 1),
+                               hipMemcpyDeviceToHost, st));
+        CT2_HIP(hipStreamSynchronize(st));
+        for (int p = 0; p < np; ++p) b->psep[p] = cand[p] && cnt[p] == 0;
+    }
+    // memory: tables for each problem's largest subset (subset 0), and a
+    // correction volume
+    size_t total = 0;
+    std::vector<size_t> tab_off(np, 0), corr_off(np, 0);
+    for (int p = 0; p < np; ++p) {
+        if (!b->psep[p]) continue;
+        const BatchProb& P = b->h[p];
+        b->sepb = true;
+        tab_off[p] = total;
+        total += (size_t)subset_views(P.dt.ns, 0, ns1) * P.dt.nu * (P.dt.nv + 1) * 16;
+    }
+    if (!b->sepb) return CT2_OK;
+    for (int p = 0; p < np; ++p) {
+        if (!b->psep[p]) continue;
+        corr_off[p] = total;
+        total += ((size_t)b->h[p].nvox * sizeof(float) + 255) & ~(size_t)255;
+    }
+    CT2_HIP(hipMalloc(&b->sep_mem, total));
+    b->bytes += total;
+    b->brick_any = false;
+    std::vector<SepBatchProb> sp(np);
+    char* base = static_cast<char*>(b->sep_mem);
+    for (int p = 0; p < np; ++p) {
+        memset(&sp[p], 0, sizeof(SepBatchProb));
+        const BatchProb& P = b->h[p];
+        if (!b->psep[p]) { b->brick_any = true; continue; }
+        sp[p].g = P.g;
+        sp[p].dt = P.dt;
+        sp[p].nvox = P.nvox;
+        sp[p].fv = P.fv;
+        sp[p].ind = P.ind;
+        sp[p].resid = P.resid;
+        sp[p].tabs = reinterpret_cast<double2*>(base + tab_off[p]);
+        sp[p].corr = reinterpret_cast<float*>(base + corr_off[p]);
+        sp[p].vol = P.vol;
+        sp[p].colsum = P.colsum;
+        const int cls = sep_cpl(P.g);
+        b->max_nv[cls] = std::max(b->max_nv[cls], P.dt.nv);
+    }
+    // offsets: table items per subset; tiles per (subset, class, axis)
+    const size_t n1 = (size_t)np + 1;
+    std::vector<long long> toff((size_t)ns1 * n1), xoff((size_t)ns1 * 8 * n1);
+    b->stab_items.assign(ns1, 0);
+    b->stiles.assign((size_t)ns1 * 8, 0);
+    for (int s = 0; s < ns1; ++s) {
+        long long acc = 0;
+        for (int p = 0; p < np; ++p) {
+            toff[(size_t)s * n1 + p] = acc;
+            if (b->psep[p])
+                acc += (long long)subset_views(b->h[p].dt.ns, s, ns1) * b->h[p].dt.nu;
+        }
+        toff[(size_t)s * n1 + np] = acc;
+        b->stab_items[s] = acc;
+        for (int cls = 1; cls <= 4; ++cls)
+            for (int ax = 0; ax < 2; ++ax) {
+                const size_t row = ((size_t)s * 4 + cls - 1) * 2 + ax;
+                long long t = 0;
+                for (int p = 0; p < np; ++p) {
+                    xoff[row * n1 + p] = t;
+                    const BatchProb& P = b->h[p];
+                    if (b->psep[p] && sep_cpl(P.g) == cls && subset_views(P.dt.ns, s, ns1) > 0)
+                        t += sep_tiles(P.g, ax);
+                }
+                xoff[row * n1 + np] = t;
+                b->stiles[row] = t;
+            }
+    }
+    CT2_HIP(hipMalloc(&b->d_sprobs, sizeof(SepBatchProb) * np));
+    CT2_HIP(hipMalloc(&b->d_stoff, sizeof(long long) * toff.size()));
+    CT2_HIP(hipMalloc(&b->d_sxoff, sizeof(long long) * xoff.size()));
+    if ((rc = upload(b->d_sprobs, sp.data(), sizeof(SepBatchProb) * np, st))) return rc;
+    if ((rc = upload(b->d_stoff, toff.data(), sizeof(long long) * toff.size(), st))) return rc;
+    return upload(b->d_sxoff, xoff.data(), sizeof(long long) * xoff.size(), st);
+}
+
 // Row sums over every view and the column sums of each subset, by the
 // separable kernels (resid serves as the ones sinogram).
 int sep_sums(ct2_batch* b, const float* ones, size_t nsino, hipStream_t st) {
@@ -144,84 +379,6 @@ int sep_sums(ct2_batch* b, const float* ones, size_t nsino, hipStream_t st) {
     return rc;
 }
 
-// Item offsets of every subset row (and the all-views row) for tile height tu.
-std::vector<long long> fwd_offsets(const ct2_batch* b, int tu) {
-    const int n1 = b->nprob + 1;
-    std::vector<long long> off((size_t)(b->nsub + 1) * n1, 0);
-    for (int row = 0; row <= b->nsub; ++row) {
-        long long acc = 0;
-        for (int p = 0; p < b->nprob; ++p) {
-            const Det& dt = b->h[p].dt;
-            off[(size_t)row * n1 + p] = acc;
-            int nv = row == b->nsub ? dt.ns : subset_views(dt.ns, row, b->nsub);
-            acc += (long long)nv * ((dt.nu + tu - 1) / tu) * ((dt.nv + 63) / 64);
-        }
-        off[(size_t)row * n1 + b->nprob] = acc;
-    }
-    return off;
-}
-
-std::vector<long long> bp_offsets(ct2_batch* b, int3 bd) {
-    const int n1 = b->nprob + 1;
-    std::vector<long long> off((size_t)(b->nsub + 1) * n1, 0);
-    for (int p = 0; p < b->nprob; ++p) {
-        const Grid& g = b->h[p].g;
-        b->h[p].nbr = make_int3((g.n[0] + bd.x - 1) / bd.x, (g.n[1] + bd.y - 1) / bd.y,
-                                (g.n[2] + bd.z - 1) / bd.z);
-    }
-    for (int row = 0; row <= b->nsub; ++row) {
-        long long acc = 0;
-        for (int p = 0; p < b->nprob; ++p) {
-            const BatchProb& P = b->h[p];
-            off[(size_t)row * n1 + p] = acc;
-            int nv = row == b->nsub ? P.dt.ns : subset_views(P.dt.ns, row, b->nsub);
-            if (nv > 0) acc += (long long)P.nbr.x * P.nbr.y * P.nbr.z;
-        }
-        off[(size_t)row * n1 + b->nprob] = acc;
-    }
-    return off;
-}
-
-int upload(void* dst, const void* src, size_t bytes, hipStream_t st) {
-    CT2_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
-    CT2_HIP(hipStreamSynchronize(st));
-    return CT2_OK;
-}
-
-int persistent_grid(int blocks_per_cu, int device) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-        cus <= 0)
-        cus = 256;
-    int g = std::max(1, blocks_per_cu) * cus;
-    return std::max(8, g & ~7);
-}
-
-int set_fwd(ct2_batch* b, int tu, hipStream_t st) {
-    std::vector<long long> off = fwd_offsets(b, tu);
-    int rc = upload(b->d_foff, off.data(), off.size() * sizeof(long long), st);
-    if (rc) return rc;
-    int occ = 0;
-    if ((rc = batch_project_occupancy(b->interp, b->arith, tu, BFWD_RESID, &occ))) return rc;
-    b->tu = tu;
-    b->fgrid = persistent_grid(occ, b->device);
-    return CT2_OK;
-}
-
-int set_bp(ct2_batch* b, int3 bd, hipStream_t st) {
-    std::vector<long long> off = bp_offsets(b, bd);
-    int rc;
-    if ((rc = upload(b->d_probs, b->h.data(), sizeof(BatchProb) * b->nprob, st))) return rc;
-    if ((rc = upload(b->d_boff, off.data(), off.size() * sizeof(long long), st))) return rc;
-    int occ = 0;
-    if ((rc = batch_backproject_occupancy(b->interp, b->arith, BBP_SART, bd, &occ))) return rc;
-    b->bdim = bd;
-    b->bgrid = persistent_grid(occ, b->device);
-    return CT2_OK;
-}
-
-const long long* frow(const ct2_batch* b, int row) { return b->d_foff + (size_t)row * (b->nprob + 1); }
-const long long* brow(const ct2_batch* b, int row) { return b->d_boff + (size_t)row * (b->nprob + 1); }
 
 int fwd(ct2_batch* b, int mode, int sub, const float* ones, hipStream_t st) {
     const bool all = mode != BFWD_RESID;
@@ -336,13 +493,15 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
         so += (size_t)P.dt.ns * P.dt.nu * P.dt.nv;
         vo += (size_t)P.nvox;
     }
+    if (!b->sep && (rc = sepb_setup(b, descs, st))) return rc;
     float* ones = nullptr;
     CT2_HIP(hipMalloc(&ones, sizeof(float) * nvox_max));
     rc = launch_fill(ones, 1.0f, nvox_max, st);
     if (!rc) rc = set_bp(b, b->bdim, st);
     if (!rc) rc = set_fwd(b, 4, st);
     const char* env = getenv("CT2_AUTOTUNE");
-    // (a separable plan's SART uses neither the tile nor the brick)
+    // (a one-problem separable plan's SART uses neither the tile nor the
+    // brick; with every problem on the separable tiles, SART uses no brick)
     if (!rc && tune && !(env && env[0] == '0') && !(b->sep && !b->has_proj && !b->has_adj)) {
         // tile height of the forward: time the all-views row-sum projection
         float best = 1e30f;
@@ -359,7 +518,7 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
         best = 1e30f;
         int3 best_b = b->bdim;
         for (const auto& c : kBrickCandidates) {
-            if (rc) break;
+            if (rc || (!b->brick_any && !b->has_adj)) break;
             int3 bd = make_int3(c[0], c[1], c[2]);
             if ((rc = set_bp(b, bd, st))) break;
             float ms = 0;
@@ -374,7 +533,10 @@ int create(ct2_batch* b, const ct2_desc* descs, const float* const* fv, const in
     } else {
         if (!rc) rc = fwd(b, BFWD_ROWSUM, 0, ones, st);
         if (!rc) rc = launch_fill(b->resid, 1.0f, nsino_tot, st);
-        for (int s = 0; s < b->nsub && !rc; ++s) rc = bp(b, BBP_COLSUM, s, 0.0f, st);
+        for (int s = 0; s < b->nsub && !rc; ++s) {
+            if (b->brick_any) rc = bp(b, BBP_COLSUM, s, 0.0f, st);
+            if (!rc && b->sepb) rc = sepb_subset(b, BSEP_COLSUM, s, 0.0f, nullptr, st);
+        }
     }
     hipError_t he = hipStreamSynchronize(st);
     hipFree(ones);
@@ -460,7 +622,9 @@ static int batch_sart(const char* who, ct2_batch* plan, float lambda, const floa
                 rc = sep_sart_subset(plan, s, lambda, lambda_dev, st);
             } else {
                 rc = fwd(plan, BFWD_RESID, s, nullptr, st);
-                if (!rc) rc = bp(plan, BBP_SART, s, lambda, st, lambda_dev);
+                if (!rc && plan->brick_any) rc = bp(plan, BBP_SART, s, lambda, st, lambda_dev);
+                if (!rc && plan->sepb)
+                    rc = sepb_subset(plan, BSEP_SART, s, lambda, lambda_dev, st);
             }
             if (rc) return rc;
         }

@@ -149,8 +149,10 @@ __device__ void sep_column_range(const float* __restrict__ f, const Det& dt, dou
 }
 
 // Write-back of a tile's sums: store, add, or add atomically (two atomic
-// adds onto a zeroed voxel commute, so that result is deterministic too).
-enum { SEP_STORE = 0, SEP_ADD = 1, SEP_ATOMIC = 2 };
+// adds onto a zeroed voxel commute, so that result is deterministic too), or
+// (SART) add the stored sums of the other axis's tiles, corr, and update
+// vol with them: vol = sart_update(vol, lambda, corr + sum, colsum).
+enum { SEP_STORE = 0, SEP_ADD = 1, SEP_ATOMIC = 2, SEP_SART = 3 };
 
 // One tile of driving axis AX: tile number `tile` of that axis's tiling.
 // tabs: the column tables of views pv = 0 .. nviews-1 (k_sep_tables);
@@ -162,7 +164,8 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                                             const double2* __restrict__ tabs,
                                             const float* __restrict__ fv, double* s_tab,
                                             SepPlane (*s_pl)[SEP_TA], unsigned* s_mask,
-                                            unsigned* s_view) {
+                                            unsigned* s_view, const float* __restrict__ corr,
+                                            const float* __restrict__ colsum, float lambda) {
     constexpr int A = AX, Bx = 1 - AX;   // driving axis, other transverse axis
     const int nv = dt.nv;
     const size_t colb = sep_col_bytes(nv);
@@ -409,6 +412,8 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
                 const float v = acc[pl][bi][cc];
                 if (mode == SEP_ATOMIC)
                     atomicAdd(vol + gi, v);
+                else if (mode == SEP_SART)
+                    vol[gi] = sart_update(vol[gi], lambda, corr[gi] + v, colsum[gi]);
                 else
                     vol[gi] = mode == SEP_ADD ? vol[gi] + v : v;
             }
@@ -429,10 +434,44 @@ __global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))
     const int b = (int)blockIdx.x;
     if ((axes & 1) && b < ntx)
         bp_sep_tile<0, CPL>(b, g, dt, view0, nviews, vstride, mode, vol, tabs, fv, s_tab, s_pl,
-                            s_mask, s_view);
+                            s_mask, s_view, nullptr, nullptr, 0.0f);
     else
         bp_sep_tile<1, CPL>((axes & 1) ? b - ntx : b, g, dt, view0, nviews, vstride, mode, vol,
-                            tabs, fv, s_tab, s_pl, s_mask, s_view);
+                            tabs, fv, s_tab, s_pl, s_mask, s_view, nullptr, nullptr, 0.0f);
+}
+
+// Batched plans: the tiles of driving axis ax of several problems (first
+// tile of problem p at off[p]), subset sub of each (views sub + pv*nsub).
+// Kind BSEP_SART: x-driven tiles store their sums in corr, y-driven tiles
+// add theirs and update the estimate; BSEP_COLSUM: x-driven tiles store the
+// column sums of the subset, y-driven tiles add theirs.  With the x launch
+// first in stream order, every voxel ends with corr + sum = the same FP32
+// sum as the single-problem adjoint's two commuting atomic adds.
+template <int CPL>
+__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void
+k_batch_bp_sep(const SepBatchProb* __restrict__ probs, const long long* __restrict__ off,
+               int nprob, int sub, int nsub, int ax, int kind, float lambda,
+               const float* __restrict__ lambda_dev) {
+    extern __shared__ double s_tab[];
+    __shared__ SepPlane s_pl[SEP_CHUNK][SEP_TA];
+    __shared__ unsigned s_mask[SEP_CHUNK];
+    __shared__ unsigned s_view[SEP_MAXV];
+    const long long it = blockIdx.x;
+    const int pi = batch_find(off, nprob, it);
+    const SepBatchProb& P = probs[pi];
+    const int tile = (int)(it - off[pi]);
+    const int nviews = subset_views(P.dt.ns, sub, nsub);
+    float* cs = P.colsum + (size_t)sub * P.nvox;
+    if (kind == BSEP_SART && lambda_dev != nullptr) lambda = *lambda_dev;
+    if (ax == 0)
+        bp_sep_tile<0, CPL>(tile, P.g, P.dt, sub, nviews, nsub, (int)SEP_STORE,
+                            kind == BSEP_SART ? P.corr : cs, P.tabs, P.fv, s_tab, s_pl, s_mask,
+                            s_view, nullptr, nullptr, 0.0f);
+    else
+        bp_sep_tile<1, CPL>(tile, P.g, P.dt, sub, nviews, nsub,
+                            kind == BSEP_SART ? (int)SEP_SART : (int)SEP_ADD,
+                            kind == BSEP_SART ? P.vol : cs, P.tabs, P.fv, s_tab, s_pl, s_mask,
+                            s_view, P.corr, cs, lambda);
 }
 
 // The column tables of views view0 + pv*vstride (pv < nviews) of a separable
@@ -442,16 +481,15 @@ __global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))
 // the indicator drops) and of td * kk, in FP64; entry 0 = (0, 0).  Lane l
 // sums the rows [l*R, l*R + R) and a wavefront scan adds the lanes below.
 // Views that are not separable are skipped (their tables are not read).
-__global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, int nviews,
-                                                    int vstride, double2* __restrict__ tabs,
-                                                    int* __restrict__ nonsep,
-                                                    const float* __restrict__ sino,
-                                                    const int8_t* __restrict__ ind,
-                                                    const float* __restrict__ fv) {
+// One wavefront: the table of view view0 + pv*vstride, column j.
+__device__ __forceinline__ void sep_table_column(const Grid& g, const Det& dt, int view0,
+                                                 int vstride, int pv, int j,
+                                                 double2* __restrict__ tabs,
+                                                 int* __restrict__ nonsep,
+                                                 const float* __restrict__ sino,
+                                                 const int8_t* __restrict__ ind,
+                                                 const float* __restrict__ fv) {
     const int lane = threadIdx.x & 63;
-    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int pv = (int)(item / dt.nu), j = (int)(item % dt.nu);
-    if (pv >= nviews) return;   // the whole wavefront
     const int p = view0 + pv * vstride;
     const float* f = fv + (size_t)p * 18;
     if (!view_separable(f, g, dt)) {
@@ -520,6 +558,40 @@ __global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, i
         if (i < R && kk < nv) tab[kk + 1] = make_double2(ps[i] + e0, pk[i] + e1);
     }
     if (lane == 0) tab[0] = make_double2(0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_sep_tables(Grid g, Det dt, int view0, int nviews,
+                                                    int vstride, double2* __restrict__ tabs,
+                                                    int* __restrict__ nonsep,
+                                                    const float* __restrict__ sino,
+                                                    const int8_t* __restrict__ ind,
+                                                    const float* __restrict__ fv) {
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int pv = (int)(item / dt.nu), j = (int)(item % dt.nu);
+    if (pv >= nviews) return;   // the whole wavefront
+    sep_table_column(g, dt, view0, vstride, pv, j, tabs, nonsep, sino, ind, fv);
+}
+
+// Batched plans: one wavefront per (problem, view of subset sub, column),
+// the first item of problem p at off[p].
+__global__ __launch_bounds__(256) void k_batch_sep_tables(const SepBatchProb* __restrict__ probs,
+                                                          const long long* __restrict__ off,
+                                                          int nprob, int sub, int nsub,
+                                                          int* __restrict__ nonsep) {
+    const long long item = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= off[nprob]) return;   // the whole wavefront
+    const int pi = batch_find(off, nprob, item);
+    const SepBatchProb& P = probs[pi];
+    const long long local = item - off[pi];
+    const int pv = (int)(local / P.dt.nu), j = (int)(local % P.dt.nu);
+    sep_table_column(P.g, P.dt, sub, nsub, pv, j, P.tabs, nonsep, P.resid, P.ind, P.fv);
+}
+
+// One thread per view: the count of views view_separable rejects.
+__global__ void k_count_nonsep(Grid g, Det dt, const float* __restrict__ fv,
+                               int* __restrict__ count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < dt.ns && !view_separable(fv + (size_t)p * 18, g, dt)) atomicAdd(count, 1);
 }
 
 // Clears vol (n floats) and the count of views left to the brick kernel.
@@ -670,6 +742,60 @@ int launch_backproject_sep(const Grid& g, const Det& dt, const ViewSet& vs, floa
         case 4: return launch_sep_cpl<4>(g, dt, vs, vol, sino, ind, fv, nonsep, tabs, gv, st);
     }
     return CT2_EINVAL;
+}
+
+int sep_tiles(const Grid& g, int axis) {
+    return axis == 0 ? ((g.n[0] + SEP_TA - 1) / SEP_TA) * ((g.n[1] + SEP_TB - 1) / SEP_TB)
+                     : ((g.n[0] + SEP_TB - 1) / SEP_TB) * ((g.n[1] + SEP_TA - 1) / SEP_TA);
+}
+
+int launch_batch_sep_tables(const SepBatchProb* probs, const long long* off, int nprob, int sub,
+                            int nsub, long long items, int* nonsep, hipStream_t st) {
+    if (items <= 0) return CT2_OK;
+    if ((items + 3) / 4 >= (1ll << 31)) {
+        set_error("batched separable adjoint: too many table columns");
+        return CT2_EINVAL;
+    }
+    hipLaunchKernelGGL(k_batch_sep_tables, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, st,
+                       probs, off, nprob, sub, nsub, nonsep);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+template <int CPL>
+static void launch_bbs(long long tiles, size_t lds, const SepBatchProb* probs,
+                       const long long* off, int nprob, int sub, int nsub, int ax, int kind,
+                       float lambda, const float* lambda_dev, hipStream_t st) {
+    hipLaunchKernelGGL((k_batch_bp_sep<CPL>), dim3((unsigned)tiles), dim3(SEP_THREADS), lds, st,
+                       probs, off, nprob, sub, nsub, ax, kind, lambda, lambda_dev);
+}
+
+int launch_batch_bp_sep(int cpl, int ax, int kind, const SepBatchProb* probs,
+                        const long long* off, int nprob, int sub, int nsub, long long tiles,
+                        int max_nv, float lambda, const float* lambda_dev, hipStream_t st) {
+    if (tiles <= 0) return CT2_OK;
+    if (tiles >= (1ll << 31)) {
+        set_error("batched separable adjoint: too many tiles");
+        return CT2_EINVAL;
+    }
+    const size_t lds = sep_lds_bytes(max_nv);
+    switch (cpl) {
+        case 1: launch_bbs<1>(tiles, lds, probs, off, nprob, sub, nsub, ax, kind, lambda, lambda_dev, st); break;
+        case 2: launch_bbs<2>(tiles, lds, probs, off, nprob, sub, nsub, ax, kind, lambda, lambda_dev, st); break;
+        case 3: launch_bbs<3>(tiles, lds, probs, off, nprob, sub, nsub, ax, kind, lambda, lambda_dev, st); break;
+        case 4: launch_bbs<4>(tiles, lds, probs, off, nprob, sub, nsub, ax, kind, lambda, lambda_dev, st); break;
+        default: set_error("batched separable adjoint: %d z cells per lane", cpl); return CT2_EINVAL;
+    }
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+int launch_count_nonsep(const Grid& g, const Det& dt, const float* fv, int* count,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_count_nonsep, dim3((unsigned)((dt.ns + 255) / 256)), dim3(256), 0, st, g,
+                       dt, fv, count);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
 }
 
 }  // namespace ct2

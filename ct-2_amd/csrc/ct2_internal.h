@@ -658,6 +658,53 @@ enum { BBP_PLAIN = 0, BBP_COLSUM = 1, BBP_SART = 2 };
 inline __host__ __device__ int subset_views(int ns, int sub, int nsub) {
     return sub < ns ? (ns - sub + nsub - 1) / nsub : 0;
 }
+// The problem whose items hold item it: off[p] is the first item of problem p
+// (problems without items repeat the next offset), off[nprob] the total.
+__device__ __forceinline__ int batch_find(const long long* __restrict__ off, int nprob,
+                                          long long it) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= it) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// One problem of a batched plan on the column-separable adjoint
+// (ct2_bp_sep.hip): SART's adjoint of subset s reads resid over the subset's
+// views, its x-driven tiles store their sums in corr, and its y-driven tiles
+// add theirs and update vol with the column sums at colsum + s*nvox; the
+// column sums themselves are made the same way from a ones sinogram.
+struct SepBatchProb {
+    Grid g;
+    Det dt;
+    long long nvox;
+    const float* fv;
+    const int8_t* ind;
+    const float* resid;
+    double2* tabs;             // column tables of the problem's largest subset
+    float* corr;
+    float* vol;
+    float* colsum;
+};
+enum { BSEP_SART = 0, BSEP_COLSUM = 1 };
+// Tile counts of the x- and y-driven launches of a grid.
+int sep_tiles(const Grid& g, int axis);
+// z cells per lane of a grid's tiles (1 .. 4)
+inline int sep_cpl(const Grid& g) { return (g.n[2] + 63) / 64; }
+size_t sep_lds_bytes(int nv);
+// Column tables of subset sub of every problem: items (problem, view of the
+// subset, detector column), first item of problem p at off[p].
+int launch_batch_sep_tables(const SepBatchProb* probs, const long long* off, int nprob, int sub,
+                            int nsub, long long items, int* nonsep, hipStream_t st);
+// The tiles of driving axis ax (0: x, stores; 1: y, adds and updates) of
+// the problems whose tiles off numbers (those of z-cell class cpl), for kind
+// BSEP_SART or BSEP_COLSUM; max_nv sizes the LDS tables.
+int launch_batch_bp_sep(int cpl, int ax, int kind, const SepBatchProb* probs,
+                        const long long* off, int nprob, int sub, int nsub, long long tiles,
+                        int max_nv, float lambda, const float* lambda_dev, hipStream_t st);
+// Count of views p < ns that view_separable rejects, added to *count.
+int launch_count_nonsep(const Grid& g, const Det& dt, const float* fv, int* count,
+                        hipStream_t st);
 int launch_batch_project(int interp, int arith, int tu, int mode, const BatchProb* probs,
                          const long long* off, int nprob, int sub, int nsub, const float* ones,
                          unsigned long long* queue, int grid, hipStream_t st);

@@ -331,16 +331,6 @@ __global__ __launch_bounds__(64 * TU) void k_project_views(Grid g, Det dt, int v
 // off[p] is the first item of problem p and off[nprob] the total.  The grid
 // is the number of resident workgroups; each takes its next item from the
 // plan's work queue (queue_next), so the items' uneven costs balance out.
-__device__ __forceinline__ int batch_find(const long long* __restrict__ off, int nprob,
-                                          long long it) {
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= it) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
 template <int INTERP, int ARITH, int TU, int MODE>
 __global__ __launch_bounds__(64 * TU) void k_batch_project(const BatchProb* __restrict__ probs,
                                                            const long long* __restrict__ off,

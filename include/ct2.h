@@ -147,7 +147,14 @@ int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const i
  * z, nz % 4 == 0, nz <= 256, 16-byte aligned volume; CT2_PLAN_SEP=0 turns
  * this off) runs SART through them instead, with a correction volume and
  * the column tables of one subset allocated at creation: each subset step
- * then equals ct2_sart_step bit for bit, and still allocates nothing. */
+ * then equals ct2_sart_step bit for bit, and still allocates nothing.  In a
+ * plan of several problems, SART's adjoint runs on batched separable tiles
+ * for every problem whose views all have their rows along z (fast linear
+ * arithmetic, free axis, nz <= 256, at most 512 views per subset; creation
+ * counts the views on the device): those problems' steps equal
+ * ct2_sart_step bit for bit too, and creation allocates their column tables
+ * (largest subset) and a correction volume each.  CT2_PLAN_SEP=0 keeps the
+ * brick kernel for all. */
 typedef struct ct2_batch ct2_batch;
 int ct2_batch_create(ct2_batch** plan, int nprob, const ct2_desc* descs,
                      const float* const* frame_vectors, const int8_t* const* indicators,

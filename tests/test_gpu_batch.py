@@ -9,7 +9,9 @@ SART over interleaved subsets equals ct2_sart_step with view0 = s,
 vstride = nsub applied problem by problem (1e-5); replays of a captured SART
 iteration equal eager iterations.  A plan of one problem on the
 column-separable kernels gives ct2_sart_step's updates bit for bit, and agrees
-with the batched kernels (CT2_PLAN_SEP=0) to FP32 summation order.
+with the batched kernels (CT2_PLAN_SEP=0) to FP32 summation order; in plans of
+several problems, those whose SART adjoint runs on the batched separable tiles
+(every z-cell class) give ct2_sart_step's updates bit for bit as well.
 """
 import numpy as np
 import pytest
@@ -232,3 +234,113 @@ def test_one_problem_plan_graph_replays_with_device_lambda():
     assert rel_l2(xe.cpu().numpy(), v.cpu().numpy()) < 0.5
     pe.close()
     pg.close()
+
+
+# ---- several problems: SART's adjoint on the separable tiles ----
+def _mixed_class_problems(seed=31):
+    """Problems of every z-cell class of the separable tiles (nz up to 64,
+    128, 192, 256), one with indicator holes, one whose views include tilted
+    ones and one z-driven problem (both of those stay on the brick kernel)."""
+    from ct2 import configs3D, phantoms
+    from ct2.operator import ConeBeamOperator
+    rng = np.random.default_rng(seed)
+    specs = [((14, 12, 20), 9, 30, 36, False, False), ((12, 10, 70), 12, 24, 100, True, False),
+             ((10, 11, 130), 10, 24, 180, False, False), ((9, 8, 200), 11, 22, 270, False, False),
+             ((8, 8, 256), 7, 20, 340, False, False), ((16, 14, 24), 10, 30, 40, False, True)]
+    ops, vols, sinos, sep = [], [], [], []
+    for i, (shape, ns, nu, nv, holes, tilt) in enumerate(specs):
+        fv = configs3D.frame_vectors("circular_conebeam", {"radius": 300.0,
+                                                          "source_to_detector": 600.0},
+                                     ns, 2 * np.pi, 0.4 * i, 0).astype(np.float64)
+        if tilt:
+            for p in range(0, ns, 4):
+                eu, ev = fv[p, 9:12].copy(), fv[p, 12:15].copy()
+                a = np.deg2rad(2.0)
+                fv[p, 12:15] = np.cos(a) * ev + np.sin(a) * eu
+                fv[p, 9:12] = np.cos(a) * eu - np.sin(a) * ev
+        ind = (rng.random((ns, nu, nv)) > 0.05).astype(np.int8) if holes else None
+        hw = np.array(shape) / 2.0
+        op = ConeBeamOperator(fv.astype(np.float32), ns, nu, nv, 1.6, 1.6, -nu * 0.8, -nv * 0.8,
+                              shape, (1.0, 1.0, 1.0), tuple(-hw), arith="fast", indicator=ind)
+        n = max(shape)
+        cube = phantoms.random_ellipsoids(n, 0.45 * n, 6, seed + i)[:shape[0], :shape[1], :shape[2]]
+        v = torch.from_numpy(np.ascontiguousarray(cube)).cuda()
+        ops.append(op)
+        vols.append(v)
+        sinos.append(op.forward(v))
+        sep.append(not tilt)
+    fv = configs3D.frame_vectors("tomosynthesis", {"radius": 650.0, "center_to_detector": 20.0},
+                                 11, np.deg2rad(50), np.deg2rad(65), 1)
+    op = ConeBeamOperator(fv, 11, 60, 50, 1.0, 1.0, -30.0, -25.0, (48, 40, 10), (0.8, 0.8, 2.0),
+                          (-19.2, -16.0, 0.0), axis="z", arith="fast")
+    v = torch.from_numpy(rng.random((48, 40, 10)).astype(np.float32)).cuda()
+    ops.append(op)
+    vols.append(v)
+    sinos.append(op.forward(v))
+    sep.append(False)
+    return ops, vols, sinos, sep
+
+
+def test_batch_sart_separable_tiles_equal_sart_step():
+    """Problems on the separable tiles give ct2_sart_step's updates bit for
+    bit, in every z-cell class; the problems left to the brick kernel agree
+    to summation order; the plan's graph replays equal eager calls."""
+    from ct2.batch import BatchPlan
+    from ct2.solvers import capture
+    ops, vols, sinos, sep = _mixed_class_problems()
+    nsub, lam, niter = 3, 0.9, 2
+    xb = [torch.zeros_like(v) for v in vols]
+    plan = BatchPlan(ops, sinos, xb, nsub=nsub, tune=True)
+    plan.sart(lam, niter)
+    for i, op in enumerate(ops):
+        x = _sart_steps(op, sinos[i], nsub, lam, niter)
+        if sep[i]:
+            assert torch.equal(xb[i], x), (i, rel_l2(xb[i].cpu().numpy(), x.cpu().numpy()))
+        else:
+            assert rel_l2(xb[i].cpu().numpy(), x.cpu().numpy()) < 1e-5, i
+        assert rel_l2(x.cpu().numpy(), vols[i].cpu().numpy()) < 0.9
+    # graph replays with the relaxation read from the device
+    from ct2 import _dev
+    from ct2._lib import call
+    xg = [torch.zeros_like(v) for v in vols]
+    xe = [torch.zeros_like(v) for v in vols]
+    pg = BatchPlan(ops, sinos, xg, nsub=nsub, tune=False)
+    pe = BatchPlan(ops, sinos, xe, nsub=nsub, tune=False)
+    lamd = torch.full((1,), 1.0, dtype=torch.float32, device="cuda")
+
+    def step(p):
+        call("ct2_batch_sart_dev", p._h, _dev.ptr(lamd), 1, _dev.stream_handle())
+
+    step(pg)
+    g = capture(lambda: step(pg))
+    step(pe)
+    for val in (0.7, 0.5):
+        lamd.fill_(val)
+        g.replay()
+        step(pe)
+        torch.cuda.synchronize()
+        for a, b in zip(xg, xe):
+            assert torch.equal(a, b)
+    for p in (plan, pg, pe):
+        p.close()
+
+
+def test_batch_sart_separable_tiles_vs_bricks(monkeypatch):
+    """CT2_PLAN_SEP=0 keeps every problem on the brick kernel: the two plans
+    agree to FP32 summation order, and the separable one holds its tables and
+    correction volumes."""
+    from ct2.batch import BatchPlan
+    ops, vols, sinos, sep = _mixed_class_problems(seed=32)
+    xs = [torch.zeros_like(v) for v in vols]
+    xb = [torch.zeros_like(v) for v in vols]
+    ps = BatchPlan(ops, sinos, xs, nsub=4, tune=False)
+    monkeypatch.setenv("CT2_PLAN_SEP", "0")
+    pb = BatchPlan(ops, sinos, xb, nsub=4, tune=False)
+    ps.sart(1.0, 2)
+    pb.sart(1.0, 2)
+    for i in range(len(ops)):
+        e = rel_l2(xs[i].cpu().numpy(), xb[i].cpu().numpy())
+        assert e < 1e-5, (i, e)
+    assert pb.info()["bytes"] < ps.info()["bytes"]
+    ps.close()
+    pb.close()
