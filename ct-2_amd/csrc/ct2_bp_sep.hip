@@ -45,7 +45,7 @@
 // One launch takes the x-driven columns of every separable view, a second the
 // y-driven ones.  A workgroup owns SEP_TA consecutive planes of the driving
 // axis a, SEP_TB cells of the other axis b, and every z; wavefront w owns
-// planes a0 + 2w and a0 + 2w + 1.  Lane l owns the CPL z cells [l*CPL,
+// plane a0 + w.  Lane l owns the CPL z cells [l*CPL,
 // (l+1)*CPL) and voxels; the upper part of its last cell moves to the next
 // lane with one DPP lane shift.  Sums accumulate in FP32 registers per
 // (plane, b cell, z cell), so no LDS or global atomics are needed, and each
@@ -53,16 +53,20 @@
 // adds, in stream order, so the result is deterministic.
 //
 // Each view's columns are taken SEP_CHUNK at a time, one per wavefront: the
-// wavefront sets the column up (FP64, the forward's expressions) and writes
-// its table of prefix sums to LDS.
+// wavefront sets the column up (FP64, the forward's expressions) and copies
+// its table of prefix sums to LDS.  Eight wavefronts of one plane each, at
+// 128 VGPRs, let two workgroups (16 wavefronts) share a CU; four wavefronts
+// of two planes each need 168 VGPRs, which leaves 12 (16.8 against 16.2 ms
+// at 256^3 x 360 views x 512^2).
 #include <mutex>
 
 #include "ct2_internal.h"
 
 namespace ct2 {
 
-constexpr int SEP_PPW = 2;                      // planes per wavefront
-constexpr int SEP_WAVES = 4;
+constexpr int SEP_PPW = 1;                      // planes per wavefront
+constexpr int SEP_WAVES = 8;
+constexpr int SEP_WPE = 4;                      // wavefronts per SIMD (128 VGPRs)
 constexpr int SEP_TA = SEP_PPW * SEP_WAVES;     // tile planes along a
 constexpr int SEP_TB = 8;                       // tile cells along b
 constexpr int SEP_THREADS = 64 * SEP_WAVES;
@@ -176,12 +180,14 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
     const int ni = A == 0 ? nP : nQ, nj = A == 0 ? nQ : nP;
     const int nb = g.n[Bx], nc = g.n[2];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    // the fast set-up's grid constants (setup_fast_ax), FP64
-    const double da = (double)g.d[A], oa = (double)g.o[A];
-    const double ob = (double)g.o[Bx], oc = (double)g.o[2];
-    const double idb = rcp_f64((double)g.d[Bx]), idc = rcp_f64((double)g.d[2]);
-    const double dacd = da * idc;   // the forward's (double)da * idc
-    const double dvd = (double)dt.dv, v0d = (double)dt.v0;
+    // the fast set-up's grid constants (setup_fast_ax), FP64 (uniform: kept
+    // in scalar registers)
+    const double da = uniform_d((double)g.d[A]), oa = uniform_d((double)g.o[A]);
+    const double ob = uniform_d((double)g.o[Bx]), oc = uniform_d((double)g.o[2]);
+    const double idb = uniform_d(rcp_f64((double)g.d[Bx]));
+    const double idc = uniform_d(rcp_f64((double)g.d[2]));
+    const double dacd = uniform_d(da * idc);   // the forward's (double)da * idc
+    const double dvd = uniform_d((double)dt.dv), v0d = uniform_d((double)dt.v0);
     const float gxa = fabsf(g.d[0]), gya = fabsf(g.d[1]);
     // this lane's z cells c0 .. c0+CPL-1 (cell c is the forward's corner
     // pair (c, c+1), valid for 0 <= c <= nc-2)
@@ -229,9 +235,10 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
         const unsigned vw = (unsigned)__builtin_amdgcn_readfirstlane((int)s_view[pv]);
         if (vw == ~0u) continue;   // not separable (the brick kernel takes it), or no column reaches the tile
         const int jlo = (int)(vw & 0xffffu), jhi = (int)(vw >> 16);
-        const double Sa = f[A], Sb = f[Bx], Sc = f[2], evz = f[14];
-        const double ta = fma(0.5, da, oa) - Sa;   // plane 0 centre from the source
-        const double ScOc = Sc - oc;
+        const double Sa = uniform_d(f[A]), Sb = uniform_d(f[Bx]);
+        const double Sc = uniform_d(f[2]), evz = uniform_d(f[14]);
+        const double ta = uniform_d(fma(0.5, da, oa) - Sa);   // plane 0 centre from the source
+        const double ScOc = uniform_d(Sc - oc);
         for (int cb = jlo; cb <= jhi; cb += SEP_CHUNK) {
             __syncthreads();   // the previous chunk's samples are done with the tables
             // ---- wavefront w: column cb + w, its set-up and its table ----
@@ -424,7 +431,7 @@ __device__ __forceinline__ void bp_sep_tile(int tile, const Grid& g, const Det& 
 // axes bit 0: the first ntx workgroups take x-driven tiles; bit 1: the
 // (other) workgroups take y-driven tiles.
 template <int CPL>
-__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_bp_sep(
+__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(SEP_WPE))) void k_bp_sep(
     Grid g, Det dt, int view0, int nviews, int vstride, int axes, int ntx, int mode,
     float* __restrict__ vol, const double2* __restrict__ tabs, const float* __restrict__ fv) {
     extern __shared__ double s_tab[];       // SEP_CHUNK column tables
@@ -448,7 +455,7 @@ __global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))
 // first in stream order, every voxel ends with corr + sum = the same FP32
 // sum as the single-problem adjoint's two commuting atomic adds.
 template <int CPL>
-__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void
+__global__ __launch_bounds__(SEP_THREADS) __attribute__((amdgpu_waves_per_eu(SEP_WPE))) void
 k_batch_bp_sep(const SepBatchProb* __restrict__ probs, const long long* __restrict__ off,
                int nprob, int sub, int nsub, int ax, int kind, float lambda,
                const float* __restrict__ lambda_dev) {
@@ -607,11 +614,11 @@ __global__ void k_sep_zero(float* __restrict__ vol, size_t n, int* __restrict__ 
 // tile.  Views that fail view_separable contribute nothing
 // here; the caller adds them with the brick kernel.  Requires linear
 // sampling, f32 output, the default C-order strides, nz <= 256, nu < 65536
-// and column tables within 60 KiB of LDS (nv <= 959).
+// and column tables within 120 KiB of LDS (eight columns, nv <= 959).
 bool sep_supported(const ct2_desc* d, const Grid& g) {
     return d->arith == CT2_ARITH_FAST && d->interp == CT2_LINEAR && d->vol_dtype == CT2_F32 &&
            d->axis == CT2_AXIS_FREE && g.n[2] <= 256 && d->nv >= 1 &&
-           sep_lds_bytes(d->nv) <= 60 * 1024 && d->nu < 65536 && g.s[2] == 1 &&
+           sep_lds_bytes(d->nv) <= 120 * 1024 && d->nu < 65536 && g.s[2] == 1 &&
            g.s[1] == g.n[2] && g.s[0] == g.n[1] * g.n[2];
 }
 

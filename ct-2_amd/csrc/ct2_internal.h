@@ -343,6 +343,12 @@ struct ViewFrame {
 __device__ __forceinline__ float uniform_f(float x) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
+__device__ __forceinline__ double uniform_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)b);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 __device__ __forceinline__ ViewFrame load_frame(const float* __restrict__ fv, int p) {
     const float* f = fv + (size_t)p * 18;
