@@ -525,26 +525,6 @@ template <> struct RowQuad<hip_bfloat16> {
     }
 };
 
-// Four consecutive row values at byte offset o: one aligned load (ALN), or
-// element by element, an element at or past the row's end (c4 + i >= nc)
-// loading the first one again (it is never sampled).
-template <typename VT, bool ALN>
-__device__ __forceinline__ typename RowQuad<VT>::raw load_quad(const char* __restrict__ vb,
-                                                              unsigned o, int c4, int nc) {
-    using raw = typename RowQuad<VT>::raw;
-    static_assert(sizeof(raw) == 4 * sizeof(VT), "a quad is four elements");
-    if constexpr (ALN) {
-        return *reinterpret_cast<const raw*>(vb + o);
-    } else {
-        raw r;
-        VT* e = reinterpret_cast<VT*>(&r);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            e[i] = *reinterpret_cast<const VT*>(vb + (c4 + i < nc ? o + (unsigned)(i * sizeof(VT)) : o));
-        return r;
-    }
-}
-
 // Orders one wavefront's LDS writes before its later reads (and its reads
 // before later writes) without a workgroup barrier: the hardware keeps a
 // wavefront's LDS operations in order, the fences keep the compiler from
@@ -556,11 +536,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Grid: one workgroup per (view, chunk of 64*RPL rows, FS_WAVES columns),
-// ncg column groups and nrc row chunks per view.  Requires nz <= 256,
-// ny*nz < 2^23 and a volume below 4 GiB with the default strides; ALN: nz %
-// 4 == 0 and a base aligned for a quad load (else the row quads are loaded
-// element by element).  LDS FS_WAVES * FS_WSLAB floats; the blended rows
-// are nz rounded up to a multiple of 4 floats apart.
+// ncg column groups and nrc row chunks per view.  Requires nz % 4 == 0,
+// nz <= 256, ny*nz < 2^23, a volume below 4 GiB with the default strides
+// and a 16-byte aligned base; LDS
+// FS_WAVES * FS_WSLAB floats.
 // 16-bit rows take half the registers of the batch in flight: four
 // wavefronts per SIMD then fit without spills (5.9 against 6.7 ms for the
 // f16 volume at 256^3 x 360 x 512^2); 32-bit rows at that limit spill.
@@ -575,7 +554,7 @@ constexpr int fs_waves_per_eu() {
     return sizeof(typename RowQuad<VT>::raw) == 8 ? CT2_FS_WPE16 : CT2_FS_WPE32;
 }
 
-template <typename VT, int RPL, bool ALN>
+template <typename VT, int RPL>
 __global__ __launch_bounds__(64 * FS_WAVES)
 __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(Grid g, Det dt, int view0,
                                                                int vstride, int ncg, int nrc,
@@ -668,7 +647,6 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
     // signed 24-bit multiplies and 32-bit byte offsets)
     const int sa = a == 0 ? g.s[0] : g.s[1], sb = a == 0 ? g.s[1] : g.s[0];
     const int nc = g.n[2];
-    const int ncp = (nc + 3) & ~3;   // LDS row stride: quads stay aligned
     // planes whose b cell is inside the volume: an interval, since
     // fma(l, dub, ub0) is monotonic in l
     int lo = 0x7fffffff, hi = -1;
@@ -694,8 +672,8 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
             const bool ok = l >= lb0 && l <= lb1 && c4 < nc;
             const int ib = floor_i(__builtin_fmaf((float)l, dub, ub0));
             const unsigned o = ok ? (unsigned)(__mul24(l, sa) + __mul24(ib, sb) + c4) * esz : 0u;
-            x0[q] = load_quad<VT, ALN>(vb, o, c4, nc);
-            x1[q] = load_quad<VT, ALN>(vb, o + (ok ? (unsigned)sb * esz : 0u), c4, nc);
+            x0[q] = *reinterpret_cast<const raw*>(vb + o);
+            x1[q] = *reinterpret_cast<const raw*>(vb + o + (ok ? (unsigned)sb * esz : 0u));
         }
     };
     // batches aligned at multiples of FS_P = CT2_SUMB planes: the batches of
@@ -712,7 +690,7 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
 #pragma unroll
         for (int q = 0; q < FS_P; ++q) {
             if (q >= qa && q <= qe && c4 < nc)
-                *reinterpret_cast<float4*>(rows + q * ncp + c4) =
+                *reinterpret_cast<float4*>(rows + q * nc + c4) =
                     RQ::blend(x0[q], x1[q], frac_pos(__builtin_fmaf(lfq[q], dub, ub0)));
         }
         wave_lds_sync();
@@ -734,7 +712,7 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
                     for (int q = 0; q < 4; ++q) {
                         const float uc = __builtin_fmaf(lfq[h + q], duc[r], uc0[r]);
                         const int ic = floor_i(uc);
-                        const float* row = rows + (h + q) * ncp;
+                        const float* row = rows + (h + q) * nc;
                         const float vi1 = row[ic], vi2 = row[ic + 1];
                         val[q] = __builtin_fmaf(frac_pos(uc), vi2 - vi1, vi1);
                     }
@@ -753,7 +731,7 @@ __attribute__((amdgpu_waves_per_eu(fs_waves_per_eu<VT>()))) void k_project_sep(G
                         const int ic = floor_i(uc);
                         ok[q] = (unsigned)ic < hc && h + q >= qa && h + q <= qe;
                         const int icc = ok[q] ? ic : 0;
-                        const float* row = rows + (h + q) * ncp;
+                        const float* row = rows + (h + q) * nc;
                         const float vi1 = row[icc], vi2 = row[icc + 1];
                         val[q] = __builtin_fmaf(frac_pos(uc), vi2 - vi1, vi1);
                     }
@@ -851,18 +829,14 @@ static std::map<TileKey, int>& tile_cache() {
 }
 
 // The column-separable forward takes fast linear projections along free
-// axes of volumes below 4 GiB with nz <= 256 and ny*nz < 2^23
+// axes of volumes below 4 GiB with nz % 4 == 0, nz <= 256, ny*nz < 2^23 and
+// a 16-byte aligned base
 // (CT2_FWD_SEP=0 keeps k_project_views, which gives the same results).
-static size_t vol_elem_bytes(const ct2_desc* d) {
-    return d->vol_dtype == CT2_F32 || d->vol_dtype == CT2_F16Z2 ? 4 : 2;
-}
-
 static bool fsep_supported(const ct2_desc* d, const Grid& g, const void* vol) {
-    (void)vol;
     if (d->arith != CT2_ARITH_FAST || d->interp != CT2_LINEAR || d->axis != CT2_AXIS_FREE)
         return false;
-    if (g.n[2] > 256) return false;
-    const size_t esz = vol_elem_bytes(d);
+    if (g.n[2] % 4 != 0 || g.n[2] > 256 || ((uintptr_t)vol & 15u) != 0) return false;
+    const size_t esz = d->vol_dtype == CT2_F32 || d->vol_dtype == CT2_F16Z2 ? 4 : 2;
     if (g.s[0] >= (1 << 23) || g.n[0] >= (1 << 23) || g.n[1] >= (1 << 23) ||
         (size_t)g.n[0] * (size_t)g.s[0] * esz >= (1ull << 32))
         return false;
@@ -877,18 +851,8 @@ static void launch_fsep_t(const Grid& g, const Det& dt, const ViewSet& vs, float
     const int nrc = (dt.nv + 64 * RPL - 1) / (64 * RPL);
     const size_t nblk = (size_t)ncg * nrc * vs.nviews;
     const size_t lds = (size_t)FS_WAVES * FS_WSLAB * sizeof(float);
-    // quad loads: every row starts at a multiple of 4 elements from an
-    // aligned base
-    const bool aln = g.n[2] % 4 == 0 &&
-                     (uintptr_t)vol % sizeof(typename RowQuad<VT>::raw) == 0;
-    if (aln)
-        hipLaunchKernelGGL((k_project_sep<VT, RPL, true>), dim3((unsigned)nblk),
-                           dim3(64 * FS_WAVES), lds, st, g, dt, vs.view0, vs.vstride, ncg, nrc,
-                           sino, ind, fv, (const VT*)vol);
-    else
-        hipLaunchKernelGGL((k_project_sep<VT, RPL, false>), dim3((unsigned)nblk),
-                           dim3(64 * FS_WAVES), lds, st, g, dt, vs.view0, vs.vstride, ncg, nrc,
-                           sino, ind, fv, (const VT*)vol);
+    hipLaunchKernelGGL((k_project_sep<VT, RPL>), dim3((unsigned)nblk), dim3(64 * FS_WAVES), lds,
+                       st, g, dt, vs.view0, vs.vstride, ncg, nrc, sino, ind, fv, (const VT*)vol);
 }
 
 template <typename VT>

@@ -144,8 +144,8 @@ int ct2_sart_step(const ct2_desc* desc, float lambda, const float* sino, const i
  * time (and a launch that fails part-way leaves the plan unusable).
  * A plan of one problem whose forward and adjoint both take the
  * column-separable kernels (fast linear arithmetic, free axis, rows along
- * z, nz <= 256; CT2_PLAN_SEP=0 turns this off) runs SART through them
- * instead, with a correction volume and
+ * z, nz % 4 == 0, nz <= 256, 16-byte aligned volume; CT2_PLAN_SEP=0 turns
+ * this off) runs SART through them instead, with a correction volume and
  * the column tables of one subset allocated at creation: each subset step
  * then equals ct2_sart_step bit for bit, and still allocates nothing.  In a
  * plan of several problems, SART's adjoint runs on batched separable tiles

@@ -14,10 +14,10 @@ Cases: detector rows per column from 44 (one row per lane) to 600 (two row
 chunks), rows stored in reverse (e_v = -z), a shifted detector whose columns
 and rows miss the volume, indicator holes, view subsets, views whose
 detector is tilted (the route's wavefronts march those ray by ray), z sizes
-whose rows are not quads (nz % 4 != 0, down to nz = 1: the rows are loaded
-element by element), a volume whose base is not aligned for quad loads, a
-volume whose edge cuts the four-column workgroups, and the 256^3 x 512^2 scan
-at its full size on a few views.
+that the route does not take (nz % 4 != 0, down to nz = 1) and a volume whose
+base is not aligned for its quad loads (both stay on the per-ray march, whose
+bits the route must equal), a volume whose edge cuts the four-column
+workgroups, and the 256^3 x 512^2 scan at its full size on a few views.
 """
 from __future__ import annotations
 
@@ -105,7 +105,7 @@ CASES = {
     "rows600": dict(shape=(20, 20, 48), nu=9, nv=600, dv=0.1),
     "shifted_detector": dict(u_shift=14.0, v_shift=-20.0),
     "tilted_views": dict(ns=10, tilt_every=3),
-    "nz70_unaligned_rows": dict(shape=(30, 26, 70)),
+    "nz70_not_taken": dict(shape=(30, 26, 70)),
     "nz255": dict(shape=(9, 7, 255), nu=20, nv=180, R=500.0, sdd=1000.0),
     "nz1": dict(shape=(14, 12, 1), nv=6),
     "nz3": dict(shape=(14, 12, 3), nv=8, dv=0.8),
@@ -173,7 +173,7 @@ def test_separable_forward_full_size_views():
 
 def test_separable_forward_unaligned_base():
     """A volume whose base is one element past an aligned address (nz % 4
-    == 0): the rows are loaded element by element, with the same sums."""
+    == 0): the per-ray march takes it, with the route's sums."""
     fv, geo, g2, ind, vol = _case(shape=(20, 18, 48))
     for label, (vold, code) in _storages(vol).items():
         # one element (one f16 pair) past the tensor's aligned start
