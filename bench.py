@@ -573,6 +573,116 @@ def asd_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def _time_ms(fn, n=5):
+    """Best of n calls of fn, each bracketed by HIP events on the current stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return min(ts)
+
+
+def tomo_workload(args):
+    """The limited-angle tomosynthesis workload: 1024x768x64 volume (0.1 x 0.1
+    x 1 mm) of the breast-like phantom (seed 20260103), 25 views over +-25 deg
+    (R = 650 mm, detector 20 mm below the centre of rotation), 2048x1536
+    detector at 0.1 mm, z-driven projector and its transpose.  Times the
+    forward with f32, f16 and bf16 volume storage and the adjoint into f32,
+    in fast and in reference arithmetic (best of 5 calls each); the
+    precision of each output against FP64 is tests/test_gpu_tomo_precision's
+    table."""
+    import numpy as np
+    import torch
+    from ct2 import configs3D, phantoms
+    from ct2.operator import ConeBeamOperator, to_storage
+    nx, ny, nz, vox = 1024, 768, 64, (0.1, 0.1, 1.0)
+    ns, nu, nv, pitch = 25, 2048, 1536, 0.1
+    log("building the tomosynthesis phantom")
+    fv = configs3D.frame_vectors("tomosynthesis", {"radius": 650.0, "center_to_detector": 20.0},
+                                 ns, np.deg2rad(50.0), np.deg2rad(65.0), 1)
+    vol = torch.from_numpy(phantoms.breast_like((nx, ny, nz), vox, 20260103)).cuda()
+    ops = {a: ConeBeamOperator(fv, ns, nu, nv, pitch, pitch, -nu * pitch / 2, -nv * pitch / 2,
+                               (nx, ny, nz), vox, (-nx * vox[0] / 2, -ny * vox[1] / 2, 0.0),
+                               axis="z", arith=a) for a in ("fast", "exact")}
+    sino = torch.empty(ops["fast"].sino_shape, dtype=torch.float32, device="cuda")
+    out = torch.empty((nx, ny, nz), dtype=torch.float32, device="cuda")
+    v16 = to_storage(vol, torch.float16, "stochastic")
+    vb16 = to_storage(vol, torch.bfloat16, "stochastic")
+    ops["fast"].forward(vol, out=sino)
+    y = sino.clone()
+    upd = ns * nu * nv * nz   # z-driven: one sample per ray and z plane
+    ms = {
+        "fwd_fast_f32": _time_ms(lambda: ops["fast"].forward(vol, out=sino)),
+        "fwd_fast_f16": _time_ms(lambda: ops["fast"].forward(v16, out=sino)),
+        "fwd_fast_bf16": _time_ms(lambda: ops["fast"].forward(vb16, out=sino)),
+        "fwd_exact_f32": _time_ms(lambda: ops["exact"].forward(vol, out=sino)),
+        "bp_fast_f32": _time_ms(lambda: ops["fast"].adjoint(y, out=out)),
+        "bp_exact_f32": _time_ms(lambda: ops["exact"].adjoint(y, out=out)),
+    }
+    gupd = {k: round(upd / (v * 1e-3) / 1e9, 1) for k, v in ms.items()}
+    mixed = ms["fwd_fast_f16"] + ms["bp_fast_f32"]
+    fp32 = ms["fwd_exact_f32"] + ms["bp_exact_f32"]
+    line = {"metric": "Gvox-ray updates/s (tomosynthesis fwd+bp, 1024x768x64, 25 views, "
+                      "2048x1536 detector), f16 storage vs reference arithmetic",
+            "value": round(2 * upd / (mixed * 1e-3) / 1e9, 1), "unit": "Gvox-ray updates/s",
+            "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": round(mixed, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "storage": "f16 volume (forward), f32 sinogram and adjoint",
+            "data": "synthetic: breast-like phantom (200 ellipsoids, 50 calcifications), "
+                    "its own projections as the adjoint's input",
+            "config": {"workload": "C3: limited-angle tomosynthesis, z-driven projector"},
+            "kernels_ms": {k: round(v, 3) for k, v in ms.items()}, "gupd_per_s": gupd,
+            "fp32_value": round(2 * upd / (fp32 * 1e-3) / 1e9, 1),
+            "mixed_over_fp32": round(fp32 / mixed, 2)}
+    print(json.dumps(line), flush=True)
+
+
+def adam_workload(args):
+    """The differentiation workload: 128^3 random ellipsoids (seed 20260105),
+    120 views over 360 deg, 192^2 detector at 1.6 mm (R = 500, SDD = 1000);
+    1/2||Ax-b||^2 + 1e-3 TV(x) minimised by torch Adam (lr 1e-2) for 200
+    steps through ct2.autograd (fast arithmetic); one step is a forward, the
+    loss, its backward (adjoint + TV gradient) and the Adam update."""
+    import numpy as np
+    import torch
+    from ct2 import configs3D, phantoms
+    from ct2.operator import ConeBeamOperator
+    from ct2.solvers import tv_least_squares
+    n, ns, nd = 128, 120, 192
+    fv = configs3D.frame_vectors("circular_conebeam", {"radius": 500.0, "source_to_detector": 1000.0},
+                                 ns, 2 * np.pi, 0.0, 0)
+    op = ConeBeamOperator(fv, ns, nd, nd, 1.6, 1.6, -nd * 0.8, -nd * 0.8, (n, n, n),
+                          (1.0, 1.0, 1.0), (-n / 2, -n / 2, -n / 2), arith="fast")
+    truth = torch.from_numpy(np.ascontiguousarray(phantoms.random_ellipsoids(n, n / 2, 40, 20260105),
+                                                  np.float32)).cuda()
+    b = op.forward(truth)
+    tv_least_squares(op, b, lam=1e-3, steps=5, lr=1e-2)   # warm-up (tables, workspaces)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, losses = tv_least_squares(op, b, lam=1e-3, steps=200, lr=1e-2)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    tn = float(torch.linalg.vector_norm(truth))
+    err = float(torch.linalg.vector_norm(x - truth)) / tn
+    upd = 2 * op.updates_per_direction()
+    line = {"metric": "ms per Adam step (1/2||Ax-b||^2 + 1e-3 TV, 128^3, 120 views, 192^2 "
+                      "detector, through ct2.autograd)",
+            "value": round(1e3 * t / 200, 3), "unit": "ms", "n_gpus": 1, "steps": 200,
+            "warmup": 5, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: 40-ellipsoid phantom and its projections",
+            "config": {"workload": "C5: TV-regularised least squares by torch Adam"},
+            "gupd_per_s": round(200 * upd / t / 1e9, 1),
+            "loss_first_last": [losses[0], losses[-1]], "rel_l2_to_truth": round(err, 4)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -584,11 +694,14 @@ def main():
     ap.add_argument("--profile-once", action="store_true",
                     help="for counter runs: one warm-up and one mixed fwd+bp step, then a "
                          "512 MiB device copy that calibrates FETCH_SIZE/WRITE_SIZE; no JSON")
-    ap.add_argument("--workload", choices=["c2", "batch", "recon", "asd", "art"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "batch", "recon", "asd", "art", "tomo", "adam"],
+                    default="c2",
                     help="c2: the headline fwd+bp metric (default); batch: 64-problem batched "
                          "SART in a HIP graph; recon: FDK + 20 EM-TV iterations at 256^3; "
                          "asd: ASD-POCS iterations (SART + TV) in a HIP graph at 256^3; "
-                         "art: C1, 10 ART sweeps on the GPU beside the reference Fortran")
+                         "art: C1, 10 ART sweeps on the GPU beside the reference Fortran; "
+                         "tomo: C3, tomosynthesis fwd and bp per storage type; adam: C5, "
+                         "200 Adam steps through the autograd projector")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the "
@@ -610,6 +723,12 @@ def main():
         return
     if args.workload == "art":
         art_workload(args)
+        return
+    if args.workload == "tomo":
+        tomo_workload(args)
+        return
+    if args.workload == "adam":
+        adam_workload(args)
         return
     fv, geo, grid = geometry()
     log("building phantom")
