@@ -594,8 +594,10 @@ def tomo_workload(args):
     x 1 mm) of the breast-like phantom (seed 20260103), 25 views over +-25 deg
     (R = 650 mm, detector 20 mm below the centre of rotation), 2048x1536
     detector at 0.1 mm, z-driven projector and its transpose.  Times the
-    forward with f32, f16 and bf16 volume storage and the adjoint into f32,
-    in fast and in reference arithmetic (best of 5 calls each); the
+    forward with f32, f16 and bf16 volume storage, C-ordered and plane-major,
+    and the adjoint into f32, in fast and in reference arithmetic (best of 5
+    calls each); the value is the f16 plane-major forward plus the fast
+    adjoint; the
     precision of each output against FP64 is tests/test_gpu_tomo_precision's
     table."""
     import numpy as np
@@ -615,6 +617,10 @@ def tomo_workload(args):
     out = torch.empty((nx, ny, nz), dtype=torch.float32, device="cuda")
     v16 = to_storage(vol, torch.float16, "stochastic")
     vb16 = to_storage(vol, torch.bfloat16, "stochastic")
+    # plane-major copies (nz planes of C-ordered (x, y)): a wavefront's rays
+    # then gather neighbouring (x, y) of one plane from contiguous memory
+    z32 = to_storage(vol, torch.float32, zplanes=True)
+    z16 = to_storage(vol, torch.float16, "stochastic", zplanes=True)
     ops["fast"].forward(vol, out=sino)
     y = sino.clone()
     upd = ns * nu * nv * nz   # z-driven: one sample per ray and z plane
@@ -623,18 +629,22 @@ def tomo_workload(args):
         "fwd_fast_f16": _time_ms(lambda: ops["fast"].forward(v16, out=sino)),
         "fwd_fast_bf16": _time_ms(lambda: ops["fast"].forward(vb16, out=sino)),
         "fwd_exact_f32": _time_ms(lambda: ops["exact"].forward(vol, out=sino)),
+        "fwd_fast_f32_zplanes": _time_ms(lambda: ops["fast"].forward(z32, out=sino, layout="zxy")),
+        "fwd_fast_f16_zplanes": _time_ms(lambda: ops["fast"].forward(z16, out=sino, layout="zxy")),
+        "fwd_exact_f32_zplanes": _time_ms(
+            lambda: ops["exact"].forward(z32, out=sino, layout="zxy")),
         "bp_fast_f32": _time_ms(lambda: ops["fast"].adjoint(y, out=out)),
         "bp_exact_f32": _time_ms(lambda: ops["exact"].adjoint(y, out=out)),
     }
     gupd = {k: round(upd / (v * 1e-3) / 1e9, 1) for k, v in ms.items()}
-    mixed = ms["fwd_fast_f16"] + ms["bp_fast_f32"]
+    mixed = ms["fwd_fast_f16_zplanes"] + ms["bp_fast_f32"]
     fp32 = ms["fwd_exact_f32"] + ms["bp_exact_f32"]
     line = {"metric": "Gvox-ray updates/s (tomosynthesis fwd+bp, 1024x768x64, 25 views, "
                       "2048x1536 detector), f16 storage vs reference arithmetic",
             "value": round(2 * upd / (mixed * 1e-3) / 1e9, 1), "unit": "Gvox-ray updates/s",
             "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": round(mixed, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "storage": "f16 volume (forward), f32 sinogram and adjoint",
+            "storage": "f16 plane-major volume (forward), f32 sinogram and adjoint",
             "data": "synthetic: breast-like phantom (200 ellipsoids, 50 calcifications), "
                     "its own projections as the adjoint's input",
             "config": {"workload": "C3: limited-angle tomosynthesis, z-driven projector"},

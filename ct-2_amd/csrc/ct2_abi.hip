@@ -254,6 +254,12 @@ int ct2_convert_volume(const float* src, void* dst, int dtype, size_t n, int rou
     return launch_convert(src, dst, dtype, n, rounding, seed, (hipStream_t)stream);
 }
 
+int ct2_convert_volume_zplanes(const float* src, void* dst, int dtype, int nx, int ny, int nz,
+                               int rounding, unsigned seed, ct2_stream_t stream) {
+    return launch_convert_zplanes(src, dst, dtype, nx, ny, nz, rounding, seed,
+                                  (hipStream_t)stream);
+}
+
 // ---- SART ----
 int ct2_sart_step(const ct2_desc* d, float lambda, const float* sino, const int8_t* ind,
                   const float* fv, float* vol, float* work, ct2_stream_t stream) {

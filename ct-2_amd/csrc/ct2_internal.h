@@ -793,6 +793,11 @@ int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, cons
                           hipStream_t st);
 int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,
                    hipStream_t st);
+int launch_convert_zplanes(const float* src, void* dst, int dtype, int nx, int ny, int nz,
+                           int rounding, unsigned seed, hipStream_t st);
+inline bool is_zplanes(int dtype) {
+    return dtype == CT2_F32_ZPLANES || dtype == CT2_F16_ZPLANES || dtype == CT2_BF16_ZPLANES;
+}
 // device fill (graph-capture safe, unlike a memset node)
 int launch_fill(float* p, float v, size_t n, hipStream_t st);
 

@@ -24,8 +24,18 @@ namespace ct2 {
 // is uniform (kernel arguments), so each ray picks its loop once.
 template <typename VT>
 __device__ __forceinline__ bool grid_small(const Grid& g) {
-    return g.s[0] < (1 << 24) && g.n[0] < (1 << 24) &&
-           (unsigned long long)g.n[0] * (unsigned)g.s[0] * sizeof(VT) < (1ull << 32);
+    // every stride and dimension a signed 24-bit operand, every element
+    // within 32-bit byte offsets (the largest n * s spans the volume, in any
+    // axis order)
+    unsigned long long ext = 0;
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        ok &= g.s[q] < (1 << 23) && g.n[q] < (1 << 23);
+        const unsigned long long e = (unsigned long long)g.n[q] * (unsigned)g.s[q];
+        ext = e > ext ? e : ext;
+    }
+    return ok && ext * sizeof(VT) < (1ull << 32);
 }
 
 // Element offset of cell (ib, ic) of a plane (SMALL: 24-bit multiplies).
@@ -785,6 +795,10 @@ static int dispatch_dtype(const ct2_desc* d, const Grid& g, const Det& dt, const
         case CT2_F16: launch_views<INTERP, ARITH, __half>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         case CT2_BF16: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         case CT2_F16Z2: launch_views<INTERP, ARITH, h16x2>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
+        // plane-major storage: the same kernels, the grid's strides say the layout
+        case CT2_F32_ZPLANES: launch_views<INTERP, ARITH, float>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
+        case CT2_F16_ZPLANES: launch_views<INTERP, ARITH, __half>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
+        case CT2_BF16_ZPLANES: launch_views<INTERP, ARITH, hip_bfloat16>(g, dt, vs, z, tu, sino, ind, fv, vol, st); break;
         default: set_error("ct2_project: unknown vol_dtype %d", d->vol_dtype); return CT2_EINVAL;
     }
     return CT2_OK;
@@ -835,6 +849,7 @@ static std::map<TileKey, int>& tile_cache() {
 static bool fsep_supported(const ct2_desc* d, const Grid& g, const void* vol) {
     if (d->arith != CT2_ARITH_FAST || d->interp != CT2_LINEAR || d->axis != CT2_AXIS_FREE)
         return false;
+    if (is_zplanes(d->vol_dtype)) return false;   // it reads C-ordered z rows
     if (g.n[2] % 4 != 0 || g.n[2] > 256 || ((uintptr_t)vol & 15u) != 0) return false;
     const size_t esz = d->vol_dtype == CT2_F32 || d->vol_dtype == CT2_F16Z2 ? 4 : 2;
     if (g.s[0] >= (1 << 23) || g.n[0] >= (1 << 23) || g.n[1] >= (1 << 23) ||
@@ -957,6 +972,12 @@ int launch_project(const ct2_desc* d, void* sino, const int8_t* ind, const float
     Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
     int rc;
     if ((rc = check_grid(g)) || (rc = check_det(dt))) return rc;
+    if (is_zplanes(d->vol_dtype)) {
+        // element (i, j, k) at (k*nx + i)*ny + j
+        g.s[0] = d->ny;
+        g.s[1] = 1;
+        g.s[2] = d->nx * d->ny;
+    }
     ViewSet vs;
     if ((rc = view_set(d, "ct2_project", vs))) return rc;
     if (!sino || !fv || !vol) { set_error("ct2_project: null pointer"); return CT2_EINVAL; }

@@ -96,6 +96,73 @@ __global__ void k_convert_pairs(const float* __restrict__ src, uint32_t* __restr
     }
 }
 
+// Plane-major copy of a C-ordered (nx, ny, nz) float32 volume: element
+// (i, j, k) to (k*nx + i)*ny + j, as float32 or rounded to 16 bits exactly as
+// k_convert rounds element (i*ny + j)*nz + k.  One 32 x 32 (k, j) tile of
+// x-slice i per workgroup, transposed through LDS so that the reads (along k)
+// and the writes (along j) are both contiguous.
+template <typename OT>
+__global__ __launch_bounds__(256) void k_convert_zplanes(const float* __restrict__ src,
+                                                         OT* __restrict__ dst, int nx, int ny,
+                                                         int nz, int stochastic, uint32_t seed) {
+    __shared__ float t[32][33];
+    const int i = blockIdx.z, k0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int dy = ty; dy < 32; dy += 8) {
+        const int j = j0 + dy, k = k0 + tx;
+        if (j < ny && k < nz) t[dy][tx] = src[((size_t)i * ny + j) * nz + k];
+    }
+    __syncthreads();
+    for (int dy = ty; dy < 32; dy += 8) {
+        const int k = k0 + dy, j = j0 + tx;
+        if (j < ny && k < nz) {
+            const float x = t[tx][dy];
+            const size_t q = ((size_t)k * nx + i) * ny + j;
+            if constexpr (std::is_same<OT, float>::value)
+                dst[q] = x;
+            else
+                reinterpret_cast<uint16_t*>(dst)[q] =
+                    round16<OT>(x, ((size_t)i * ny + j) * nz + k, stochastic, seed);
+        }
+    }
+}
+
+int launch_convert_zplanes(const float* src, void* dst, int dtype, int nx, int ny, int nz,
+                           int rounding, unsigned seed, hipStream_t st) {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 65535 ||
+        (long long)nx * ny * nz >= (1LL << 31)) {
+        set_error("ct2_convert_volume_zplanes: bad shape %d x %d x %d", nx, ny, nz);
+        return CT2_EINVAL;
+    }
+    if (!src || !dst) { set_error("ct2_convert_volume_zplanes: null pointer"); return CT2_EINVAL; }
+    if (rounding != CT2_ROUND_NEAREST && rounding != CT2_ROUND_STOCHASTIC) {
+        set_error("ct2_convert_volume_zplanes: unknown rounding %d", rounding);
+        return CT2_EINVAL;
+    }
+    const int s = rounding == CT2_ROUND_STOCHASTIC;
+    const dim3 grid((unsigned)((nz + 31) / 32), (unsigned)((ny + 31) / 32), (unsigned)nx);
+    switch (dtype) {
+        case CT2_F32_ZPLANES:
+            hipLaunchKernelGGL(k_convert_zplanes<float>, grid, dim3(256), 0, st, src, (float*)dst,
+                               nx, ny, nz, s, (uint32_t)seed);
+            break;
+        case CT2_F16_ZPLANES:
+            hipLaunchKernelGGL(k_convert_zplanes<__half>, grid, dim3(256), 0, st, src,
+                               (__half*)dst, nx, ny, nz, s, (uint32_t)seed);
+            break;
+        case CT2_BF16_ZPLANES:
+            hipLaunchKernelGGL(k_convert_zplanes<hip_bfloat16>, grid, dim3(256), 0, st, src,
+                               (hip_bfloat16*)dst, nx, ny, nz, s, (uint32_t)seed);
+            break;
+        default:
+            set_error("ct2_convert_volume_zplanes: destination must be CT2_F32_ZPLANES, "
+                      "CT2_F16_ZPLANES or CT2_BF16_ZPLANES");
+            return CT2_EINVAL;
+    }
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
 // Per-view constants: S, D, e_u, e_v, e_w and the source-to-panel distance
 // sd_dist (view_proj3D.f:1699-1714), computed on the device in FP32.
 struct PdView {

@@ -17,6 +17,7 @@ LIBPATH = os.path.join(_HERE, "libct2.so")
 CT2_LINEAR, CT2_NEAREST = 0, 1
 CT2_AXIS_FREE, CT2_AXIS_Z = 0, 1
 CT2_F32, CT2_F16, CT2_BF16, CT2_F16Z2 = 0, 1, 2, 3
+CT2_F32_ZPLANES, CT2_F16_ZPLANES, CT2_BF16_ZPLANES = 4, 5, 6
 CT2_ARITH_EXACT, CT2_ARITH_FAST = 0, 1
 CT2_ROUND_NEAREST, CT2_ROUND_STOCHASTIC = 0, 1
 CT2_TV_WORK_DOUBLES = 1040
@@ -35,6 +36,7 @@ SIGNATURES = {
     "ct2_backproject_work_bytes": "P",
     "ct2_sart_step": "PfPPPPPS",
     "ct2_convert_volume": "PPiziuS",
+    "ct2_convert_volume_zplanes": "PPiiiiiuS",
     "ct2_batch_create": "PiPPPPPPPiiS",
     "ct2_batch_destroy": "P",
     "ct2_batch_info": "PPP",

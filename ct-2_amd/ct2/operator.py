@@ -19,22 +19,42 @@ import torch
 
 from . import _dev
 from ._lib import (CT2_ARITH_EXACT, CT2_ARITH_FAST, CT2_AXIS_FREE, CT2_AXIS_Z, CT2_BF16,
-                   CT2_F16, CT2_F16Z2, CT2_F32, CT2_LINEAR, CT2_NEAREST, CT2_ROUND_NEAREST,
+                   CT2_BF16_ZPLANES, CT2_F16, CT2_F16_ZPLANES, CT2_F16Z2, CT2_F32,
+                   CT2_F32_ZPLANES, CT2_LINEAR, CT2_NEAREST, CT2_ROUND_NEAREST,
                    CT2_ROUND_STOCHASTIC, call, ct2_desc, lib)
 
 _DT = {torch.float32: CT2_F32, torch.float16: CT2_F16, torch.bfloat16: CT2_BF16}
+_DTZ = {torch.float32: CT2_F32_ZPLANES, torch.float16: CT2_F16_ZPLANES,
+        torch.bfloat16: CT2_BF16_ZPLANES}
 
 
 def to_storage(vol32, dtype=torch.float16, rounding="stochastic", seed=20260107, out=None,
-               pairs=False):
+               pairs=False, zplanes=False):
     """float32 volume -> 16-bit storage (float16 or bfloat16) by the library's
     conversion kernel; ``rounding`` is "stochastic" (unbiased, deterministic
     in ``seed``) or "nearest".  ``pairs`` (float16 only) gives the f16 pair
     storage read by the fast forward projector: a (*shape, 2) tensor whose
     [..., 0] is the volume as float16 storage rounds it and [..., 1] the next
-    element along the last axis (zero after the last element)."""
+    element along the last axis (zero after the last element).  ``zplanes``
+    gives the plane-major copy (dtype float32, float16 or bfloat16): an (nz,
+    nx, ny) tensor, element [k, i, j] = voxel (i, j, k) rounded as the C-order
+    conversion rounds it, for ``ConeBeamOperator.forward(v, layout="zxy")``."""
     if vol32.dtype != torch.float32 or not vol32.is_contiguous():
         raise ValueError("source must be a contiguous float32 tensor")
+    if zplanes:
+        if vol32.dim() != 3 or pairs:
+            raise ValueError("plane-major storage takes a 3-D volume (no pairs)")
+        nx, ny, nz = vol32.shape
+        if dtype not in _DTZ:
+            raise ValueError("plane-major storage dtype must be float32, float16 or bfloat16")
+        shape = (nz, nx, ny)
+        out = torch.empty(shape, dtype=dtype, device=vol32.device) if out is None else out
+        if tuple(out.shape) != shape or out.dtype != dtype or not out.is_contiguous():
+            raise ValueError(f"output must be a contiguous {dtype} tensor of shape {shape}")
+        call("ct2_convert_volume_zplanes", _dev.ptr(vol32), _dev.ptr(out), _DTZ[dtype], nx, ny, nz,
+             {"stochastic": CT2_ROUND_STOCHASTIC, "nearest": CT2_ROUND_NEAREST}[rounding],
+             int(seed), _dev.stream_handle())
+        return out
     if dtype not in (torch.float16, torch.bfloat16):
         raise ValueError("storage dtype must be float16 or bfloat16")
     if pairs and dtype != torch.float16:
@@ -87,21 +107,33 @@ class ConeBeamOperator:
         d.vstride = vstride
         return d
 
-    def forward(self, vol, out=None, view0=0, nviews=0, vstride=1):
+    def forward(self, vol, out=None, view0=0, nviews=0, vstride=1, layout="xyz"):
         """A vol -> sinogram (float32).  vol: (nx,ny,nz) float32/float16/bfloat16,
         or (nx,ny,nz,2) float16 pair storage (``to_storage(..., pairs=True)``),
-        which projects exactly as its float16 values do.  With a view subset
-        (views view0 + j*vstride, j < nviews; nviews 0 = all such views) only
-        those views of ``out`` are written."""
-        pairs = tuple(vol.shape) == self.shape + (2,) and vol.dtype == torch.float16
-        if not pairs and (vol.shape != self.shape or vol.dtype not in _DT):
-            raise ValueError(f"volume must be a {self.shape} float32/16/bf16 tensor or its "
-                             f"{self.shape + (2,)} float16 pair storage")
+        which projects exactly as its float16 values do; with layout="zxy" the
+        plane-major (nz,nx,ny) storage of ``to_storage(..., zplanes=True)``,
+        which projects exactly as the same values in (nx,ny,nz) order do.
+        With a view subset (views view0 + j*vstride, j < nviews; nviews 0 = all
+        such views) only those views of ``out`` are written."""
+        nx, ny, nz = self.shape
+        if layout == "zxy":
+            if tuple(vol.shape) != (nz, nx, ny) or vol.dtype not in _DTZ:
+                raise ValueError(f"plane-major volume must be a {(nz, nx, ny)} "
+                                 "float32/16/bf16 tensor")
+            code = _DTZ[vol.dtype]
+        elif layout == "xyz":
+            pairs = tuple(vol.shape) == self.shape + (2,) and vol.dtype == torch.float16
+            if not pairs and (vol.shape != self.shape or vol.dtype not in _DT):
+                raise ValueError(f"volume must be a {self.shape} float32/16/bf16 tensor or its "
+                                 f"{self.shape + (2,)} float16 pair storage")
+            code = CT2_F16Z2 if pairs else vol.dtype
+        else:
+            raise ValueError('layout must be "xyz" or "zxy"')
         if not vol.is_contiguous():
             raise ValueError("volume must be contiguous")
         if out is None:
             out = torch.empty(self.sino_shape, dtype=torch.float32, device=vol.device)
-        d = self._desc(CT2_F16Z2 if pairs else vol.dtype, view0, nviews, vstride)
+        d = self._desc(code, view0, nviews, vstride)
         call("ct2_project", C.byref(d), _dev.ptr(out), _dev.ptr(self.ind), _dev.ptr(self.fv),
              _dev.ptr(vol), _dev.stream_handle())
         return out

@@ -37,6 +37,10 @@ enum { CT2_OK = 0, CT2_EINVAL = -1, CT2_EHIP = -2, CT2_ENOMEM = -3 };
 enum { CT2_LINEAR = 0, CT2_NEAREST = 1 };
 enum { CT2_AXIS_FREE = 0, CT2_AXIS_Z = 1 };
 enum { CT2_F32 = 0, CT2_F16 = 1, CT2_BF16 = 2, CT2_F16Z2 = 3 };
+/* Plane-major volume storage (forward projection only): element (i,j,k) at
+ * (k*nx + i)*ny + j, i.e. the volume as nz planes of C-ordered (x, y); made
+ * by ct2_convert_volume_zplanes. */
+enum { CT2_F32_ZPLANES = 4, CT2_F16_ZPLANES = 5, CT2_BF16_ZPLANES = 6 };
 /* CT2_ARITH_EXACT: the reference's FP32 operation order, correctly rounded
  * divisions, no FMA contraction -- bit-identical to the reference Fortran
  * for the forward projector.  CT2_ARITH_FAST: per-ray setup in FP64, FMA
@@ -114,6 +118,16 @@ size_t ct2_backproject_work_bytes(const ct2_desc* desc);
  * reference (all REAL*4). */
 int ct2_convert_volume(const float* src, void* dst, int dtype, size_t n, int rounding,
                        unsigned seed, ct2_stream_t stream);
+/* float32 (nx, ny, nz) volume -> plane-major storage (dtype CT2_F32_ZPLANES,
+ * CT2_F16_ZPLANES or CT2_BF16_ZPLANES: element (i,j,k) at (k*nx + i)*ny + j),
+ * 16-bit values rounded exactly as ct2_convert_volume rounds the same
+ * element.  ct2_project reads it with desc->vol_dtype set to that dtype and
+ * returns the same projections as from the C-ordered volume of the same
+ * values.  For z-driven projection (tomosynthesis), where a wavefront's rays
+ * sample neighbouring (x, y) of one z plane, the plane-major copy turns their
+ * gathers into contiguous reads.  No counterpart in the reference. */
+int ct2_convert_volume_zplanes(const float* src, void* dst, int dtype, int nx, int ny, int nz,
+                               int rounding, unsigned seed, ct2_stream_t stream);
 /* SART update over the view subset of desc (views view0 + j*vstride; no
  * counterpart in the reference, which has only sequential ART):
  * x += lambda * A^T((b-Ax)/A1) / A^T1 restricted to those views, with
