@@ -861,10 +861,10 @@ static int tuned_backproject(const ct2_desc* d, const Grid& g, const Det& dt,
         return launch_with(&e, g, dt, vs, vol, sn, ind, fv, st);
     const auto& cand = kBrickCandidates;
     const int nc = (int)(sizeof cand / sizeof cand[0]);
-    hipEvent_t ev[2 * 8];
+    hipEvent_t ev[2 * kBrickCount];
     for (int c = 0; c < 2 * nc; ++c) CT2_HIP(hipEventCreate(&ev[c]));
     int rc = CT2_OK;
-    int3 tried[8];
+    int3 tried[kBrickCount];
     int ntried = 0;
     for (int c = 0; c < nc && rc == CT2_OK; ++c) {
         int3 b = make_int3(min(cand[c][0], g.n[0]), min(cand[c][1], g.n[1]), min(cand[c][2], g.n[2]));

@@ -724,10 +724,16 @@ int batch_backproject_occupancy(int interp, int arith, int mode, int3 bdim, int*
 size_t brick_lds_bytes(int3 bdim);
 // adjoint brick shapes the autotuners time (single problems and batched plans)
 // (at 256^3 x 360 views with dense data: 12x12x24 63.8 ms, 10x10x24 64.9,
-// 14x14x14 66.0, 12x12x14 66.4, 16x16x12 71, 16x16x8 94)
-inline constexpr int kBrickCandidates[8][3] = {{12, 12, 14}, {12, 12, 24}, {10, 10, 24},
-                                               {14, 14, 14}, {16, 12, 14}, {12, 16, 14},
-                                               {16, 16, 12}, {16, 16, 16}};
+// 14x14x14 66.0, 12x12x14 66.4, 16x16x12 71, 16x16x8 94).  The last four are
+// flat and long across the direction a source arc moves the rays: z-driven
+// tomosynthesis rays (1024x768x64 at 0.1 x 0.1 x 1 mm, +-25 deg in y-z) drift
+// 4.7 voxels in y per plane, so a brick long in y keeps each ray's set-up for
+// more samples (16x32x8 6.8 ms, 8x64x8 6.5 ms against 10.2 ms for the best
+// of the first eight).
+constexpr int kBrickCount = 12;
+inline constexpr int kBrickCandidates[kBrickCount][3] = {
+    {12, 12, 14}, {12, 12, 24}, {10, 10, 24}, {14, 14, 14}, {16, 12, 14}, {12, 16, 14},
+    {16, 16, 12}, {16, 16, 16}, {16, 32, 8},  {32, 16, 8},  {8, 64, 8},   {64, 8, 8}};
 
 // View subset of a descriptor: views view0 + j*vstride for j < nviews.
 struct ViewSet {
