@@ -34,6 +34,11 @@ struct Brick {
 // compile-time stride keeps the candidate walk in scalar registers.
 constexpr int BRICK_THREADS = 256;
 
+// Views a brick pass leaves to another route (fast arithmetic only): none,
+// those view_separable accepts (column-separable adjoint), or those view_zsep
+// accepts under the z-forced projector (plane-separable adjoint).
+constexpr int BRICK_SKIP_NONE = 0, BRICK_SKIP_SEP = 1, BRICK_SKIP_ZSEP = 2;
+
 // LDS accumulation in FP64: on gfx950 ds_add_f64 runs at ~9 cycles per
 // wavefront instruction per CU while ds_add_f32 takes ~190 (measured), and
 // the wider sum is also more accurate.
@@ -372,15 +377,16 @@ __device__ __forceinline__ void lane_march(const LaneRay& L, float dtr, double* 
 // Accumulate A^T sino over views view0 + pv*vstride (pv < nviews) for the
 // voxels of brick br into the FP64 LDS accumulator acc (zeroed here).  Called
 // by the whole workgroup; ends with a barrier, after which acc holds the sums.
-// With skip_sep (fast arithmetic only), views that view_separable accepts are
-// left out: the column-separable kernel takes them.  Returns whether any view
-// was taken (the same in every thread).
+// With skip (fast arithmetic only), the views another route takes are left
+// out (BRICK_SKIP_SEP: those view_separable accepts; BRICK_SKIP_ZSEP: those
+// view_zsep accepts).  Returns whether any view was taken (the same in every
+// thread).
 template <int INTERP, int ARITH>
 __device__ bool brick_accumulate(const Grid& g, const Det& dt, int view0, int nviews,
                                  int vstride, bool zonly, const Brick& br, double* acc,
                                  const float* __restrict__ sino,
                                  const int8_t* __restrict__ ind,
-                                 const float* __restrict__ fv, bool skip_sep = false) {
+                                 const float* __restrict__ fv, int skip = 0) {
     __shared__ int s_range[4];
     const BrickCtx bc = brick_ctx(g, br);
     const int* ls = bc.ls;
@@ -400,9 +406,10 @@ __device__ bool brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
             // views the separable kernel takes: lane b classifies view b; every
             // wavefront finds the same mask, so skipping a batch is uniform
             unsigned long long sepm = 0ull;
-            if (skip_sep) {
-                const bool sb = lane < nb &&
-                                view_separable(fv + (size_t)(view0 + (pv0 + lane) * vstride) * 18, g, dt);
+            if (skip) {
+                const float* fb = fv + (size_t)(view0 + (pv0 + lane) * vstride) * 18;
+                const bool sb = lane < nb && (skip == BRICK_SKIP_ZSEP ? view_zsep(fb, g, dt)
+                                                                      : view_separable(fb, g, dt));
                 sepm = __ballot(sb);
                 if (__popcll(sepm) == nb) continue;
             }
@@ -532,14 +539,15 @@ __device__ bool brick_accumulate(const Grid& g, const Det& dt, int view0, int nv
 // LDS bytes of a brick accumulator (FP64 slots, one-voxel halo)
 size_t brick_lds_bytes(int3 b) { return (size_t)(b.x + 2) * (b.y + 2) * (b.z + 2) * sizeof(double); }
 
-// skip_sep: leave out the views view_separable accepts and ADD the brick's
-// sums to vol (after the column-separable kernels wrote it), if any view
-// was taken; requires nsplit = 1 and an f32 output.
+// skip: leave out the views another route takes (BRICK_SKIP_SEP,
+// BRICK_SKIP_ZSEP) and ADD the brick's sums to vol (after that route's
+// kernels wrote it), if any view was taken; requires nsplit = 1 and an f32
+// output.
 template <int INTERP, int ARITH, typename OT>
 __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det dt, int view0,
                                                            int nviews, int vstride, int zonly,
                                                            int3 bdim, int3 nbr, int nsplit,
-                                                           int skip_sep, OT* __restrict__ vol,
+                                                           int skip, OT* __restrict__ vol,
                                                            const float* __restrict__ sino,
                                                            const int8_t* __restrict__ ind,
                                                            const float* __restrict__ fv,
@@ -554,8 +562,8 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det
     const int pv_lo = part * nviews / nsplit, pv_hi = (part + 1) * nviews / nsplit;
     const bool any = brick_accumulate<INTERP, ARITH>(g, dt, view0 + pv_lo * vstride,
                                                      pv_hi - pv_lo, vstride, zonly != 0, br, acc,
-                                                     sino, ind, fv, skip_sep != 0);
-    if (skip_sep && !any) return;   // uniform: vol already holds these voxels' sums
+                                                     sino, ind, fv, skip);
+    if (skip && !any) return;   // uniform: vol already holds these voxels' sums
     // write the brick: z-rows are contiguous in both LDS and HBM
     const int nloc = br.ext[0] * br.ext[1] * br.ext[2];
     for (int q = threadIdx.x; q < nloc; q += BRICK_THREADS) {
@@ -569,7 +577,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_backproject_brick(Grid g, Det
                 atomicAdd(vol + gi, (float)acc[brick_slot(br, lx, ly, lz)]);
                 continue;
             }
-            if (skip_sep) {   // one workgroup per brick: a plain add
+            if (skip) {   // one workgroup per brick: a plain add
                 vol[gi] += (float)acc[brick_slot(br, lx, ly, lz)];
                 continue;
             }
@@ -771,6 +779,18 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
     // view (stream order fixes the order of the adds).
     if constexpr (ARITH == CT2_ARITH_FAST && INTERP == CT2_LINEAR &&
                   std::is_same<OT, float>::value) {
+        // z-forced views with the detector parallel to the planes: the
+        // plane-separable kernel stores every voxel, then one workgroup per
+        // brick adds any other view
+        if (zsep_route(d, g)) {
+            int rc = launch_backproject_zsep(g, dt, vs, (float*)vol, sino, ind, fv, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>),
+                               dim3(nb.x * nb.y * nb.z), dim3(BRICK_THREADS), lds, st, g, dt,
+                               vs.view0, vs.nviews, vs.vstride, zonly, b, nb, 1,
+                               BRICK_SKIP_ZSEP, (OT*)vol, sino, ind, fv, (const int*)nullptr);
+            return CT2_OK;
+        }
         if (sep_route(d, g)) {
             void* own = nullptr;
             int rc = sep_work_acquire(dt, vs, w, &own, st);
@@ -779,7 +799,8 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
             if (rc == CT2_OK)
                 hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>),
                                    dim3(nb.x * nb.y * nb.z), dim3(BRICK_THREADS), lds, st, g, dt,
-                                   vs.view0, vs.nviews, vs.vstride, zonly, b, nb, 1, 1, (OT*)vol,
+                                   vs.view0, vs.nviews, vs.vstride, zonly, b, nb, 1,
+                                   BRICK_SKIP_SEP, (OT*)vol,
                                    sino, ind, fv, sep_nonsep_count(w));
             const int rf = sep_work_release(own, st);
             return rc ? rc : rf;
@@ -790,7 +811,8 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
         hipLaunchKernelGGL(k_zero_strided, dim3(2048), dim3(256), 0, st, g, (float*)vol);
     dim3 grid(nb.x * nb.y * nb.z * nsplit);
     hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>), grid, dim3(BRICK_THREADS), lds, st, g, dt,
-                       vs.view0, vs.nviews, vs.vstride, zonly, b, nb, nsplit, 0, (OT*)vol, sino,
+                       vs.view0, vs.nviews, vs.vstride, zonly, b, nb, nsplit, BRICK_SKIP_NONE,
+                       (OT*)vol, sino,
                        ind, fv, (const int*)nullptr);
     return CT2_OK;
 }
@@ -912,9 +934,10 @@ int launch_backproject(const ct2_desc* d, void* vol, const void* sino, const int
         set_error("ct2_backproject: unknown arith %d", d->arith);
         return CT2_EINVAL;
     }
-    // (the separable route leaves the brick kernel only the other views:
-    // no brick is tuned for it)
-    if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 && !sep_route(d, g))
+    // (the separable routes leave the brick kernel only the other views: no
+    // brick is tuned for them)
+    if (d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 && !sep_route(d, g) &&
+        !zsep_route(d, g))
         return tuned_backproject(d, g, dt, vs, vol, (const float*)sino, ind, fv, st);
     return launch_with(d, g, dt, vs, vol, (const float*)sino, ind, fv, st, w);
 }

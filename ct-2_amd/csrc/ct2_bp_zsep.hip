@@ -1,0 +1,278 @@
+// ct2_bp_zsep.hip -- the adjoint of the fast z-driven Joseph projector for
+// views whose flat detector is parallel to the z planes, e_u along x and e_v
+// along y: the tomosynthesis frames of emil_code/configs3D.py:615-654 under
+// the z-forced projector of emil_code/view_proj3D_tomo.f:4-148.
+//
+// For such a view (view_zsep) the fast set-up of ray (j, k) gives its x cell
+// coordinate on plane l as X_l(j) = fma(l, dub_j, ub0_j), a function of the
+// detector column j alone, and its y coordinate Y_l(k) = fma(l, duc_k,
+// uc0_k) of the row k alone (both the forward's own FP32 values); only the
+// ray weight trav = |d_z| sqrt((1 + r1_j^2) + r2_k^2) couples them.  The
+// forward samples plane l of ray (j, k) when 0 <= floor(X) <= nx-2 and 0 <=
+// floor(Y) <= ny-2, with bilinear weights (1-wb, wb) x (1-wc, wc).  Its
+// transpose on plane l is therefore separable: voxel (i, c, l) receives
+//     sum_j hb_l(j, i) sum_k hc_l(k, c) trav(j, k) datum(j, k),
+// hb_l(j, i) = 1-wb if floor(X_l(j)) = i, wb if it is i-1 (0 otherwise), and
+// hc_l likewise in y.  The inner sum (a resampling of detector row data onto
+// y cells) is taken per (column j, voxel column c); the outer one adds it to
+// the two x rows column j reaches.
+//
+// Tiles.  A workgroup owns 8 x rows, 64 y cells and 8 z planes; wavefront w
+// owns plane k0 + w, lane c one y cell, and sums its 8 voxels in FP32
+// registers over every view, so each voxel is stored once, with no atomics
+// (an LDS transpose at the end makes each lane's store 8 contiguous z
+// values).  Per (view, plane) a wavefront
+//   1. evaluates the exact-arithmetic maps X_l(j) = ax + bx j and Y_l(k) =
+//      ay + by k (FP64), which bound the detector columns reaching its rows
+//      (fewer than 9/ZSEP_SPACING + 5, so at most 17) and each lane's
+//      candidate rows (4: a window [c-1, c+1) holds at most 2/ZSEP_SPACING
+//      + 1 < 4 rows, with a margin of 0.01 row for the FP32 rounding of Y);
+//   2. lane e sets up column e of that range with the forward's FP64 set-up
+//      expressions (ub0, dub, 1 + r1^2) and its sample's cell and weight;
+//   3. each lane sets up its 4 candidate rows (uc0, duc, r2^2), their cells
+//      and their weights hc for its own y cell;
+//   4. for each column, whose cell, weight and trav term it reads from lane
+//      e (v_readlane, scalar), gathers G = sum hc trav datum over its
+//      candidate rows (coalesced: neighbouring lanes read neighbouring
+//      detector rows) into the lane's own LDS slot of that column; then each
+//      x row adds (1-wb) G of the columns whose cell it is and wb G of those
+//      whose cell is the row below (ballot masks; the slots stand in for a
+//      register array indexed by the column, which the compiler would put
+//      in scratch memory).
+// The matrix entries are the brick kernel's (the same FP32 cell
+// coordinates, weights and trav); only the grouping of the sums differs, so
+// the result is the fast forward's transpose up to FP32 rounding.
+#include "ct2_internal.h"
+
+namespace ct2 {
+
+constexpr int ZS_ROWS = 8;                   // x rows per workgroup (all in each wavefront)
+constexpr int ZS_TY = 64;                    // y cells per workgroup (one per lane)
+constexpr int ZS_KZ = 8;                     // z planes per workgroup (one per wavefront)
+constexpr int ZS_NC = 4;                     // candidate detector rows per y cell
+constexpr int ZS_THREADS = 64 * ZS_KZ;
+constexpr int ZS_MAXCOL = 20;                // detector columns per (wavefront, plane), >= 17
+
+// Per-view constants of the fast set-up for a = z, b = x, c = y
+// (setup_fast_ax), FP64 and wavefront-uniform.
+struct ZsView {
+    double Sx, Sy, ia, ta, eux, Dx, evy, Dy;
+};
+
+// Plane k of the tile for view V: steps 1-4 above.
+template <bool IND>
+__device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
+                                         const Det& dt, const ZsView& V, int k, int iw, int c,
+                                         const float* __restrict__ sp,
+                                         const int8_t* __restrict__ ip, double idx, double idy,
+                                         double dzidx, double dzidy, double ox, double oy,
+                                         float adz, float* sg) {
+    const int lane = threadIdx.x & 63;
+    const int nx = g.n[0], ny = g.n[1];
+    const float kf = (float)k;
+    // 1. exact maps: X(j) = ax + bx j, Y(q) = ay + by q on this plane
+    const double T = fma((double)k, (double)g.d[2], V.ta);
+    const double sx = T * V.ia * idx, sy = T * V.ia * idy;
+    const double bx = sx * V.eux * (double)dt.du, by = sy * V.evy * (double)dt.dv;
+    const double ax = fma(V.Sx - ox, idx, -0.5) +
+                      sx * (fma(V.eux, fma(0.5, (double)dt.du, (double)dt.u0), V.Dx) - V.Sx);
+    const double ay = fma(V.Sy - oy, idy, -0.5) +
+                      sy * (fma(V.evy, fma(0.5, (double)dt.dv, (double)dt.v0), V.Dy) - V.Sy);
+    // columns whose sample falls in cells [iw-1, iw+ZS_ROWS): X in [iw-1, iw+ZS_ROWS)
+    const double rbx = rcp_f64(bx), rby = rcp_f64(by);
+    const double e0 = ((double)(iw - 1) - ax) * rbx, e1 = ((double)(iw + ZS_ROWS) - ax) * rbx;
+    const double elo = fmin(fmax(fmin(e0, e1) - 1.0, -1.0), (double)dt.nu);
+    const double ehi = fmin(fmax(fmax(e0, e1) + 1.0, -1.0), (double)dt.nu);
+    const int jlo = max(0, (int)floor(elo)), jhi = min(dt.nu - 1, (int)ceil(ehi));
+    // fewer than 9/ZSEP_SPACING + 5 columns (17), so every one has a lane and
+    // an LDS slot
+    const int ncol = __builtin_amdgcn_readfirstlane(min(ZS_MAXCOL, jhi - jlo + 1));
+    if (ncol <= 0) return;
+    // 2. lane e: column jlo + e, the forward's set-up and its sample
+    int ibl = -2;
+    float wbl = 0.0f, q1l = 0.0f;
+    if (lane < ncol) {
+        const double u = fma((double)(jlo + lane) + 0.5, (double)dt.du, (double)dt.u0);
+        const double r1 = (fma(V.eux, u, V.Dx) - V.Sx) * V.ia;
+        const float ub0 = (float)fma(fma(r1, V.ta, V.Sx) - ox, idx, -0.5);
+        const float dub = (float)(r1 * dzidx);
+        const float r1f = (float)r1;
+        q1l = 1.0f + r1f * r1f;
+        const float X = __builtin_fmaf(kf, dub, ub0);
+        const int ib = floor_i(X);
+        wbl = frac_pos(X);
+        // rows the wavefront owns, and the forward's validity window
+        ibl = (ib >= 0 && ib <= nx - 2) ? ib - iw : -2;
+    }
+    // 3. this lane's candidate rows and their weights for y cell c
+    int rowc[ZS_NC];
+    float hc[ZS_NC], s2c[ZS_NC];
+    {
+        const double f0 = ((double)(c - 1) - ay) * rby, f1 = ((double)(c + 1) - ay) * rby;
+        const double flo = fmin(fmax(fmin(f0, f1) - 0.01, -8.0), (double)dt.nv + 8.0);
+        const int q0 = (int)ceil(flo);
+#pragma unroll
+        for (int n = 0; n < ZS_NC; ++n) {
+            const int q = q0 + n;
+            hc[n] = 0.0f;
+            s2c[n] = 0.0f;
+            rowc[n] = min(max(q, 0), dt.nv - 1);
+            if (q >= 0 && q < dt.nv && c < ny) {
+                const double v = fma((double)q + 0.5, (double)dt.dv, (double)dt.v0);
+                const double r2 = (fma(V.evy, v, V.Dy) - V.Sy) * V.ia;
+                const float uc0 = (float)fma(fma(r2, V.ta, V.Sy) - oy, idy, -0.5);
+                const float duc = (float)(r2 * dzidy);
+                const float r2f = (float)r2;
+                s2c[n] = r2f * r2f;
+                const float Y = __builtin_fmaf(kf, duc, uc0);
+                const int ic = floor_i(Y);
+                const float wc = frac_pos(Y);
+                if (ic >= 0 && ic <= ny - 2)
+                    hc[n] = ic == c ? 1.0f - wc : (ic == c - 1 ? wc : 0.0f);
+            }
+        }
+    }
+    // 4a. per column: G = sum hc trav datum over the lane's candidate rows
+    // (loads unconditional, in bounds; a row with hc = 0 adds nothing), into
+    // the lane's own LDS slot of that column
+    for (int e = 0; e < ncol; ++e) {
+        const int r = __builtin_amdgcn_readlane(ibl, e);
+        if (r < -1 || r >= ZS_ROWS) continue;   // uniform: no row of the tile takes it
+        const float q1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q1l), e));
+        const size_t rowoff = (size_t)(jlo + e) * dt.nv;
+        const float* __restrict__ sr = sp + rowoff;
+        float datum[ZS_NC];
+#pragma unroll
+        for (int n = 0; n < ZS_NC; ++n) {
+            datum[n] = sr[rowc[n]];
+            if (IND && ip[rowoff + rowc[n]] != 1) datum[n] = 0.0f;
+        }
+        float G = 0.0f;
+#pragma unroll
+        for (int n = 0; n < ZS_NC; ++n) {
+            const float t = datum[n] * (adz * fast_sqrt(q1 + s2c[n]));
+            const float Gn = __builtin_fmaf(hc[n], t, G);
+            G = hc[n] != 0.0f ? Gn : G;
+        }
+        sg[e * 64 + lane] = G;
+    }
+    // 4b. row q takes (1-wb) G of the columns whose cell is q and wb G of
+    // those whose cell is q-1 (the lane reads back only its own slots)
+#pragma unroll
+    for (int q = 0; q < ZS_ROWS; ++q) {
+        unsigned long long m0 = __ballot(lane < ncol && ibl == q);
+        unsigned long long m1 = __ballot(lane < ncol && ibl == q - 1);
+        while (m0) {
+            const int e = __builtin_ctzll(m0);
+            m0 &= m0 - 1;
+            const float wb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wbl), e));
+            acc[q] = __builtin_fmaf(1.0f - wb, sg[e * 64 + lane], acc[q]);
+        }
+        while (m1) {
+            const int e = __builtin_ctzll(m1);
+            m1 &= m1 - 1;
+            const float wb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wbl), e));
+            acc[q] = __builtin_fmaf(wb, sg[e * 64 + lane], acc[q]);
+        }
+    }
+}
+
+template <bool IND>
+__global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view0, int nviews,
+                                                        int vstride, float* __restrict__ vol,
+                                                        const float* __restrict__ sino,
+                                                        const int8_t* __restrict__ ind,
+                                                        const float* __restrict__ fv) {
+    // per wavefront, one slot per (column, lane) for step 4; then the tile's
+    // sums for the write-back
+    constexpr int SG = ZS_KZ * ZS_MAXCOL * 64, SO = ZS_ROWS * ZS_TY * (ZS_KZ + 1);
+    __shared__ float s_lds[SG > SO ? SG : SO];
+    float (*s_out)[ZS_TY][ZS_KZ + 1] = reinterpret_cast<float (*)[ZS_TY][ZS_KZ + 1]>(s_lds);
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int ntx = (nx + ZS_ROWS - 1) / ZS_ROWS, nty = (ny + ZS_TY - 1) / ZS_TY;
+    int b = blockIdx.x;
+    const int tx = b % ntx;
+    b /= ntx;
+    const int ty = b % nty;
+    const int tz = b / nty;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int iw = tx * ZS_ROWS;            // the tile's first x row
+    const int c0 = ty * ZS_TY, c = c0 + lane;   // the lane's y cell
+    const int k0 = tz * ZS_KZ, k = k0 + wave;   // the wavefront's plane
+    float acc[ZS_ROWS];
+#pragma unroll
+    for (int r = 0; r < ZS_ROWS; ++r) acc[r] = 0.0f;
+    if (k < nz) {   // uniform
+        // the fast set-up's grid constants (setup_fast_ax with a = z, b = x, c = y)
+        const double idx = rcp_f64((double)g.d[0]), idy = rcp_f64((double)g.d[1]);
+        const double dzidx = (double)g.d[2] * idx, dzidy = (double)g.d[2] * idy;
+        const double ox = g.o[0], oy = g.o[1];
+        const float adz = fabsf(g.d[2]);
+        const size_t vsz = (size_t)dt.nu * dt.nv;
+        for (int pv = 0; pv < nviews; ++pv) {
+            const int p = view0 + pv * vstride;
+            const float* f = fv + (size_t)p * 18;
+            if (!view_zsep(f, g, dt)) continue;   // the brick pass takes it
+            ZsView V;
+            V.Sx = uniform_f(f[0]);
+            V.Sy = uniform_f(f[1]);
+            const double Sz = uniform_f(f[2]);
+            // B_z - S_z: e_u and e_v have no z component, so every bin has z = D_z
+            V.ia = rcp_f64((double)uniform_f(f[8]) - Sz);
+            V.ta = fma(0.5, (double)g.d[2], (double)g.o[2]) - Sz;
+            V.eux = uniform_f(f[9]);
+            V.Dx = uniform_f(f[6]);
+            V.evy = uniform_f(f[13]);
+            V.Dy = uniform_f(f[7]);
+            zs_plane<IND>(acc, g, dt, V, k, iw, c, sino + (size_t)p * vsz,
+                          IND ? ind + (size_t)p * vsz : nullptr, idx, idy, dzidx, dzidy, ox, oy,
+                          adz, s_lds + wave * (ZS_MAXCOL * 64));
+        }
+    }
+    // write-back through LDS: each thread stores 4 consecutive z values of
+    // one (x row, y cell)
+    __syncthreads();   // every wavefront is done with its column slots
+#pragma unroll
+    for (int r = 0; r < ZS_ROWS; ++r) s_out[r][lane][wave] = acc[r];
+    __syncthreads();
+    const bool quads = (nz & 3) == 0 && ((uintptr_t)vol & 15) == 0;
+    for (int q = threadIdx.x; q < ZS_ROWS * ZS_TY * 2; q += ZS_THREADS) {
+        const int h = q & 1, cc = (q >> 1) % ZS_TY, r = q / (2 * ZS_TY);
+        const int i = iw + r, cy = c0 + cc, kq = k0 + 4 * h;
+        if (i >= nx || cy >= ny || kq >= nz) continue;
+        const float* sv = &s_out[r][cc][4 * h];
+        const size_t gi = ((size_t)i * ny + cy) * nz + kq;
+        if (quads)   // kq + 4 <= nz: nz and kq are multiples of 4
+            *reinterpret_cast<float4*>(vol + gi) = make_float4(sv[0], sv[1], sv[2], sv[3]);
+        else
+            for (int e = 0; e < 4 && kq + e < nz; ++e) vol[gi + e] = sv[e];
+    }
+}
+
+bool zsep_route(const ct2_desc* d, const Grid& g) {
+    return d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 &&
+           d->arith == CT2_ARITH_FAST && d->interp == CT2_LINEAR && d->vol_dtype == CT2_F32 &&
+           d->axis == CT2_AXIS_Z && g.s[2] == 1 && g.s[1] == g.n[2] &&
+           g.s[0] == g.n[1] * g.n[2];
+}
+
+int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
+                            const float* sino, const int8_t* ind, const float* fv,
+                            hipStream_t st) {
+    const long long ntile = (long long)((g.n[0] + ZS_ROWS - 1) / ZS_ROWS) *
+                            ((g.n[1] + ZS_TY - 1) / ZS_TY) * ((g.n[2] + ZS_KZ - 1) / ZS_KZ);
+    if (ntile > 0x7fffffffLL) {
+        set_error("ct2_backproject: volume too large for the plane-separable route");
+        return CT2_EINVAL;
+    }
+    if (ind != nullptr)
+        hipLaunchKernelGGL(k_bp_zsep<true>, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g, dt,
+                           vs.view0, vs.nviews, vs.vstride, vol, sino, ind, fv);
+    else
+        hipLaunchKernelGGL(k_bp_zsep<false>, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g,
+                           dt, vs.view0, vs.nviews, vs.vstride, vol, sino, ind, fv);
+    CT2_HIP(hipGetLastError());
+    return CT2_OK;
+}
+
+}  // namespace ct2

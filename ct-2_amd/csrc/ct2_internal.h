@@ -492,6 +492,34 @@ __device__ inline bool view_separable(const float* __restrict__ f, const Grid& g
     return fabs((double)f[14] * (double)dt.dv / (double)g.d[2]) * dist >= maxlen / 48.0;
 }
 
+// ---- plane-separable views (the z-driven adjoint's fast route) ----
+// With e_u exactly along x, e_v exactly along y and the projector forced to
+// z, the bin of ray (j, k) lies at B = (fma(eu_x, u_j, D_x), fma(ev_y, v_k,
+// D_y), D_z) (the zero products of view_ray_f64 vanish exactly), so the fast
+// set-up's x coordinate of a ray (ub0, dub) depends on its column j alone,
+// its y coordinate (uc0, duc) on its row k alone, and on plane l the samples
+// of a view form the detector grid, scaled.  view_zsep says whether view F
+// qualifies: those zero entries, D_z != S_z, and on every plane the ray
+// spacing in voxels, |t_l e_u,x du / d_x| and |t_l e_v,y dv / d_y| with t_l
+// the plane's fraction of the ray (of one sign over the planes), at least
+// ZSEP_SPACING, so that a voxel's [c-1, c+1) window holds at most four
+// detector rows (k_bp_zsep).
+constexpr double ZSEP_SPACING = 0.7;
+
+__device__ inline bool view_zsep(const float* __restrict__ f, const Grid& g, const Det& dt) {
+    if (f[10] != 0.0f || f[11] != 0.0f || f[12] != 0.0f || f[14] != 0.0f || f[9] == 0.0f ||
+        f[13] == 0.0f)
+        return false;
+    const double dd2 = (double)f[8] - (double)f[2];
+    if (dd2 == 0.0) return false;
+    const double ta = fma(0.5, (double)g.d[2], (double)g.o[2]) - (double)f[2];
+    const double t0 = ta / dd2, t1 = fma((double)(g.n[2] - 1), (double)g.d[2], ta) / dd2;
+    if (!(t0 * t1 > 0.0)) return false;
+    const double tmin = fmin(fabs(t0), fabs(t1));
+    return tmin * fabs((double)f[9] * (double)dt.du / (double)g.d[0]) >= ZSEP_SPACING &&
+           tmin * fabs((double)f[13] * (double)dt.dv / (double)g.d[1]) >= ZSEP_SPACING;
+}
+
 // Workgroup b of a grid of nblk runs on XCD b % 8 (the dispatcher deals
 // workgroups to the 8 XCDs in turn): the tile it takes, such that each XCD
 // gets a contiguous range of tiles, whose data then share its L2.  A
@@ -795,6 +823,13 @@ int sep_work_release(void* own, hipStream_t st);
 // The device count of views of the last separable launch that
 // view_separable rejected (in the workspace header).
 inline const int* sep_nonsep_count(const SepWork& w) { return static_cast<const int*>(w.ptr); }
+// Plane-separable adjoint of the z-driven projector (ct2_bp_zsep.hip): fast
+// linear arithmetic into an f32 volume in C order, axis forced to z, no
+// brick named.  It stores every voxel from the views view_zsep accepts; a
+// brick pass then adds the others.
+bool zsep_route(const ct2_desc* d, const Grid& g);
+int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
+                            const float* sino, const int8_t* ind, const float* fv, hipStream_t st);
 int launch_pd_backproject(const ct2_desc* d, float* vol, const float* sino, const float* fv,
                           hipStream_t st);
 int launch_convert(const float* src, void* dst, int dtype, size_t n, int rounding, unsigned seed,
