@@ -783,7 +783,7 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
         // plane-separable kernel stores every voxel, then one workgroup per
         // brick adds any other view
         if (zsep_route(d, g)) {
-            int rc = launch_backproject_zsep(g, dt, vs, (float*)vol, sino, ind, fv, st);
+            int rc = launch_backproject_zsep(g, dt, vs, (float*)vol, sino, ind, fv, st, w);
             if (rc) return rc;
             hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>),
                                dim3(nb.x * nb.y * nb.z), dim3(BRICK_THREADS), lds, st, g, dt,

@@ -638,6 +638,7 @@ size_t sep_work_bytes(const ct2_desc* d) {
     Det dt = make_det(d->ns, d->nu, d->nv, d->du, d->dv, d->u0, d->v0);
     ViewSet vs;
     if (check_grid(g) || check_det(dt) || view_set(d, "ct2_backproject_work_bytes", vs)) return 0;
+    if (zsep_route(d, g)) return zsep_work_bytes(d, dt, vs);
     if (!sep_route(d, g)) return 0;
     return SEP_WS_HEAD + (size_t)vs.nviews * sep_view_bytes(dt);
 }
@@ -681,7 +682,7 @@ static int launch_sep_cpl(const Grid& g, const Det& dt, const ViewSet& vs, float
 // next allocation instead of returning it at each synchronisation (its
 // default is 0), so that repeated adjoints without a caller workspace do not
 // map and unmap their tables every call.  Set once per device.
-static void keep_pool_memory(int dev) {
+void keep_pool_memory(int dev) {
     static std::mutex mu;
     static bool done[64] = {};
     std::lock_guard<std::mutex> lk(mu);

@@ -28,20 +28,23 @@
 //      candidate rows (4: a window [c-1, c+1) holds at most 2/ZSEP_SPACING
 //      + 1 < 4 rows, with a margin of 0.01 row for the FP32 rounding of Y);
 //   2. lane e sets up column e of that range with the forward's FP64 set-up
-//      expressions (ub0, dub, 1 + r1^2) and its sample's cell and weight;
-//   3. each lane sets up its 4 candidate rows (uc0, duc, r2^2), their cells
-//      and their weights hc for its own y cell;
-//   4. for each column, whose cell, weight and trav term it reads from lane
-//      e (v_readlane, scalar), gathers G = sum hc trav datum over its
+//      expressions (ub0, dub) and its sample's cell and weight;
+//   3. each lane sets up its 4 candidate rows (uc0, duc), their cells and
+//      their weights hc for its own y cell;
+//   4. for each column, whose cell and weight it reads from lane e
+//      (v_readlane, scalar), gathers G = sum hc td over its
 //      candidate rows (coalesced: neighbouring lanes read neighbouring
 //      detector rows) into the lane's own LDS slot of that column; then each
 //      x row adds (1-wb) G of the columns whose cell it is and wb G of those
 //      whose cell is the row below (ballot masks; the slots stand in for a
 //      register array indexed by the column, which the compiler would put
 //      in scratch memory).
-// The matrix entries are the brick kernel's (the same FP32 cell
-// coordinates, weights and trav); only the grouping of the sums differs, so
-// the result is the fast forward's transpose up to FP32 rounding.
+// td = trav datum (0 for rays the indicator leaves out) is formed once per
+// ray by k_zsep_td into a workspace (trav as the forward's set-up forms it,
+// one v_sqrt_f32), not once per (ray, plane) in the tiles.  The matrix
+// entries are the brick kernel's (the same FP32 cell coordinates, weights
+// and trav); only the grouping of the sums differs, so the result is the
+// fast forward's transpose up to FP32 rounding.
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -60,13 +63,11 @@ struct ZsView {
 };
 
 // Plane k of the tile for view V: steps 1-4 above.
-template <bool IND>
 __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
                                          const Det& dt, const ZsView& V, int k, int iw, int c,
-                                         const float* __restrict__ sp,
-                                         const int8_t* __restrict__ ip, double idx, double idy,
+                                         const float* __restrict__ sp, double idx, double idy,
                                          double dzidx, double dzidy, double ox, double oy,
-                                         float adz, float* sg) {
+                                         float* sg) {
     const int lane = threadIdx.x & 63;
     const int nx = g.n[0], ny = g.n[1];
     const float kf = (float)k;
@@ -90,14 +91,12 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     if (ncol <= 0) return;
     // 2. lane e: column jlo + e, the forward's set-up and its sample
     int ibl = -2;
-    float wbl = 0.0f, q1l = 0.0f;
+    float wbl = 0.0f;
     if (lane < ncol) {
         const double u = fma((double)(jlo + lane) + 0.5, (double)dt.du, (double)dt.u0);
         const double r1 = (fma(V.eux, u, V.Dx) - V.Sx) * V.ia;
         const float ub0 = (float)fma(fma(r1, V.ta, V.Sx) - ox, idx, -0.5);
         const float dub = (float)(r1 * dzidx);
-        const float r1f = (float)r1;
-        q1l = 1.0f + r1f * r1f;
         const float X = __builtin_fmaf(kf, dub, ub0);
         const int ib = floor_i(X);
         wbl = frac_pos(X);
@@ -106,7 +105,7 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     }
     // 3. this lane's candidate rows and their weights for y cell c
     int rowc[ZS_NC];
-    float hc[ZS_NC], s2c[ZS_NC];
+    float hc[ZS_NC];
     {
         const double f0 = ((double)(c - 1) - ay) * rby, f1 = ((double)(c + 1) - ay) * rby;
         const double flo = fmin(fmax(fmin(f0, f1) - 0.01, -8.0), (double)dt.nv + 8.0);
@@ -115,15 +114,12 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
         for (int n = 0; n < ZS_NC; ++n) {
             const int q = q0 + n;
             hc[n] = 0.0f;
-            s2c[n] = 0.0f;
             rowc[n] = min(max(q, 0), dt.nv - 1);
             if (q >= 0 && q < dt.nv && c < ny) {
                 const double v = fma((double)q + 0.5, (double)dt.dv, (double)dt.v0);
                 const double r2 = (fma(V.evy, v, V.Dy) - V.Sy) * V.ia;
                 const float uc0 = (float)fma(fma(r2, V.ta, V.Sy) - oy, idy, -0.5);
                 const float duc = (float)(r2 * dzidy);
-                const float r2f = (float)r2;
-                s2c[n] = r2f * r2f;
                 const float Y = __builtin_fmaf(kf, duc, uc0);
                 const int ic = floor_i(Y);
                 const float wc = frac_pos(Y);
@@ -132,26 +128,20 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
             }
         }
     }
-    // 4a. per column: G = sum hc trav datum over the lane's candidate rows
-    // (loads unconditional, in bounds; a row with hc = 0 adds nothing), into
-    // the lane's own LDS slot of that column
+    // 4a. per column: G = sum hc td over the lane's candidate rows (loads
+    // unconditional, in bounds; a row with hc = 0 adds nothing), into the
+    // lane's own LDS slot of that column
     for (int e = 0; e < ncol; ++e) {
         const int r = __builtin_amdgcn_readlane(ibl, e);
         if (r < -1 || r >= ZS_ROWS) continue;   // uniform: no row of the tile takes it
-        const float q1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(q1l), e));
-        const size_t rowoff = (size_t)(jlo + e) * dt.nv;
-        const float* __restrict__ sr = sp + rowoff;
-        float datum[ZS_NC];
+        const float* __restrict__ sr = sp + (size_t)(jlo + e) * dt.nv;
+        float td[ZS_NC];
 #pragma unroll
-        for (int n = 0; n < ZS_NC; ++n) {
-            datum[n] = sr[rowc[n]];
-            if (IND && ip[rowoff + rowc[n]] != 1) datum[n] = 0.0f;
-        }
+        for (int n = 0; n < ZS_NC; ++n) td[n] = sr[rowc[n]];
         float G = 0.0f;
 #pragma unroll
         for (int n = 0; n < ZS_NC; ++n) {
-            const float t = datum[n] * (adz * fast_sqrt(q1 + s2c[n]));
-            const float Gn = __builtin_fmaf(hc[n], t, G);
+            const float Gn = __builtin_fmaf(hc[n], td[n], G);
             G = hc[n] != 0.0f ? Gn : G;
         }
         sg[e * 64 + lane] = G;
@@ -177,11 +167,12 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     }
 }
 
-template <bool IND>
+// td: k_zsep_td's values of views view0 + pv*vstride (pv < nviews); add:
+// add the sums to vol instead of storing them.
 __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view0, int nviews,
-                                                        int vstride, float* __restrict__ vol,
-                                                        const float* __restrict__ sino,
-                                                        const int8_t* __restrict__ ind,
+                                                        int vstride, int add,
+                                                        float* __restrict__ vol,
+                                                        const float* __restrict__ td,
                                                         const float* __restrict__ fv) {
     // per wavefront, one slot per (column, lane) for step 4; then the tile's
     // sums for the write-back
@@ -207,7 +198,6 @@ __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view
         const double idx = rcp_f64((double)g.d[0]), idy = rcp_f64((double)g.d[1]);
         const double dzidx = (double)g.d[2] * idx, dzidy = (double)g.d[2] * idy;
         const double ox = g.o[0], oy = g.o[1];
-        const float adz = fabsf(g.d[2]);
         const size_t vsz = (size_t)dt.nu * dt.nv;
         for (int pv = 0; pv < nviews; ++pv) {
             const int p = view0 + pv * vstride;
@@ -224,9 +214,8 @@ __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view
             V.Dx = uniform_f(f[6]);
             V.evy = uniform_f(f[13]);
             V.Dy = uniform_f(f[7]);
-            zs_plane<IND>(acc, g, dt, V, k, iw, c, sino + (size_t)p * vsz,
-                          IND ? ind + (size_t)p * vsz : nullptr, idx, idy, dzidx, dzidy, ox, oy,
-                          adz, s_lds + wave * (ZS_MAXCOL * 64));
+            zs_plane(acc, g, dt, V, k, iw, c, td + (size_t)pv * vsz, idx, idy, dzidx, dzidy, ox,
+                     oy, s_lds + wave * (ZS_MAXCOL * 64));
         }
     }
     // write-back through LDS: each thread stores 4 consecutive z values of
@@ -242,10 +231,18 @@ __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view
         if (i >= nx || cy >= ny || kq >= nz) continue;
         const float* sv = &s_out[r][cc][4 * h];
         const size_t gi = ((size_t)i * ny + cy) * nz + kq;
-        if (quads)   // kq + 4 <= nz: nz and kq are multiples of 4
-            *reinterpret_cast<float4*>(vol + gi) = make_float4(sv[0], sv[1], sv[2], sv[3]);
-        else
-            for (int e = 0; e < 4 && kq + e < nz; ++e) vol[gi + e] = sv[e];
+        if (quads) {   // kq + 4 <= nz: nz and kq are multiples of 4
+            float4* o = reinterpret_cast<float4*>(vol + gi);
+            float4 v = make_float4(sv[0], sv[1], sv[2], sv[3]);
+            if (add) {
+                const float4 w = *o;
+                v = make_float4(w.x + v.x, w.y + v.y, w.z + v.z, w.w + v.w);
+            }
+            *o = v;
+        } else {
+            for (int e = 0; e < 4 && kq + e < nz; ++e)
+                vol[gi + e] = add ? vol[gi + e] + sv[e] : sv[e];
+        }
     }
 }
 
@@ -256,23 +253,101 @@ bool zsep_route(const ct2_desc* d, const Grid& g) {
            g.s[0] == g.n[1] * g.n[2];
 }
 
+// td = trav datum of every ray of views view0 + pv*vstride (pv < nviews)
+// that view_zsep accepts, 0 where the indicator leaves a ray out: one
+// workgroup per (view, detector column), the column's set-up (r1) shared,
+// trav = |d_z| fast_sqrt((1 + r1f^2) + r2f^2) as setup_fast_ax forms it.
+__global__ __launch_bounds__(256) void k_zsep_td(Grid g, Det dt, int view0, int nviews,
+                                                 int vstride, float* __restrict__ td,
+                                                 const float* __restrict__ sino,
+                                                 const int8_t* __restrict__ ind,
+                                                 const float* __restrict__ fv) {
+    const int pv = blockIdx.x / dt.nu, j = blockIdx.x % dt.nu;
+    if (pv >= nviews) return;
+    const int p = view0 + pv * vstride;
+    const float* f = fv + (size_t)p * 18;
+    if (!view_zsep(f, g, dt)) return;   // uniform; the tiles skip the view too
+    const double Sx = f[0], Sy = f[1], Sz = f[2];
+    const double ia = rcp_f64((double)f[8] - Sz);
+    const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
+    const float r1f = (float)((fma((double)f[9], u, (double)f[6]) - Sx) * ia);
+    const float q1 = 1.0f + r1f * r1f;
+    const float adz = fabsf(g.d[2]);
+    const size_t row = ((size_t)p * dt.nu + j) * dt.nv;
+    float* __restrict__ out = td + ((size_t)pv * dt.nu + j) * dt.nv;
+    for (int k = threadIdx.x; k < dt.nv; k += 256) {
+        const double v = fma((double)k + 0.5, (double)dt.dv, (double)dt.v0);
+        const float r2f = (float)((fma((double)f[13], v, (double)f[7]) - Sy) * ia);
+        const float trav = adz * fast_sqrt(q1 + r2f * r2f);
+        const bool live = ind == nullptr || ind[row + k] == 1;
+        out[k] = live ? sino[row + k] * trav : 0.0f;
+    }
+}
+
+// Bytes of one view's td values.
+static size_t zsep_view_bytes(const Det& dt) { return (size_t)dt.nu * dt.nv * sizeof(float); }
+
+// An internally allocated workspace holds the td values of at most this
+// many bytes of views (the tomosynthesis scan of 25 views of 2048 x 1536
+// needs 315 MB).
+constexpr size_t ZSEP_WS_CAP = (size_t)4 << 30;
+
+size_t zsep_work_bytes(const ct2_desc* d, const Det& dt, const ViewSet& vs) {
+    (void)d;
+    return (size_t)vs.nviews * zsep_view_bytes(dt);
+}
+
 int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
-                            const float* sino, const int8_t* ind, const float* fv,
-                            hipStream_t st) {
+                            const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
+                            SepWork w) {
     const long long ntile = (long long)((g.n[0] + ZS_ROWS - 1) / ZS_ROWS) *
                             ((g.n[1] + ZS_TY - 1) / ZS_TY) * ((g.n[2] + ZS_KZ - 1) / ZS_KZ);
     if (ntile > 0x7fffffffLL) {
         set_error("ct2_backproject: volume too large for the plane-separable route");
         return CT2_EINVAL;
     }
-    if (ind != nullptr)
-        hipLaunchKernelGGL(k_bp_zsep<true>, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g, dt,
-                           vs.view0, vs.nviews, vs.vstride, vol, sino, ind, fv);
-    else
-        hipLaunchKernelGGL(k_bp_zsep<false>, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g,
-                           dt, vs.view0, vs.nviews, vs.vstride, vol, sino, ind, fv);
-    CT2_HIP(hipGetLastError());
-    return CT2_OK;
+    const size_t per = zsep_view_bytes(dt);
+    void* own = nullptr;
+    size_t gv;
+    if (w.ptr) {
+        if (((uintptr_t)w.ptr & 15u) != 0) {
+            set_error("ct2_backproject: workspace must be 16-byte aligned");
+            return CT2_EINVAL;
+        }
+        if (w.bytes < per) {
+            set_error("ct2_backproject: workspace of %zu B holds no view's values (%zu B)",
+                      w.bytes, per);
+            return CT2_EINVAL;
+        }
+        gv = w.bytes / per;
+    } else {
+        gv = max((size_t)1, ZSEP_WS_CAP / per);
+        gv = min(gv, (size_t)max(vs.nviews, 1));
+        int dev = 0;
+        CT2_HIP(hipGetDevice(&dev));
+        keep_pool_memory(dev);
+        CT2_HIP(hipMallocAsync(&own, per * gv, st));
+    }
+    float* td = static_cast<float*>(own ? own : w.ptr);
+    int rc = CT2_OK;
+    // groups of gv views: the first stores every voxel, later ones add, in
+    // stream order (so the result does not depend on the schedule)
+    for (int v = 0; v == 0 || v < vs.nviews; v += (int)gv) {
+        const int nvg = min((int)gv, vs.nviews - v), v0 = vs.view0 + v * vs.vstride;
+        const long long items = (long long)nvg * dt.nu;
+        if (items > 0)
+            hipLaunchKernelGGL(k_zsep_td, dim3((unsigned)items), dim3(256), 0, st, g, dt, v0, nvg,
+                               vs.vstride, td, sino, ind, fv);
+        hipLaunchKernelGGL(k_bp_zsep, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g, dt, v0,
+                           nvg, vs.vstride, v > 0 ? 1 : 0, vol, td, fv);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { rc = hip_fail(e, "ct2_backproject"); break; }
+    }
+    if (own) {
+        hipError_t e = hipFreeAsync(own, st);
+        if (rc == CT2_OK && e != hipSuccess) rc = hip_fail(e, "ct2_backproject");
+    }
+    return rc;
 }
 
 }  // namespace ct2

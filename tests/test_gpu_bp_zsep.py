@@ -173,3 +173,32 @@ def test_zsep_dot_product(seed):
     gap = abs(lhs - rhs) / scale
     print(f"seed {seed}: <Ax,y> {lhs:.6e}, <x,ATy> {rhs:.6e}, gap {gap:.2e} of <A|x|,|y|>")
     assert gap <= 1e-6, gap
+
+
+def test_zsep_workspace_groups():
+    """ct2_backproject_work_bytes gives the route's workspace (trav * datum of
+    every view); through ct2_backproject_ws a workspace of one view's values
+    runs the views one group at a time (the first stores, the rest add) and
+    matches the allocating entry; a smaller one is rejected."""
+    from ct2 import _dev
+    from ct2._lib import CT2Error, call, lib
+    fv, geo, g2, ind, rng = _case(ns=6)
+    ns, nu, nv = geo[:3]
+    nx, ny, nz = g2[6:]
+    y = rng.standard_normal(geo[:3]).astype(np.float32)
+    d = _desc(geo, g2)
+    assert lib().ct2_backproject_work_bytes(C.byref(d)) == ns * nu * nv * 4
+    want = _adjoint(fv, geo, g2, ind, y)
+    yd, fvd, indd = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (y, fv, ind))
+    out = torch.full((nx, ny, nz), float("nan"), dtype=torch.float32, device="cuda")
+    work = torch.empty(nu * nv, dtype=torch.float32, device="cuda")
+    call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(yd), _dev.ptr(indd),
+         _dev.ptr(fvd), _dev.ptr(work), nu * nv * 4, _dev.stream_handle())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    e = rel_l2(got, want)
+    print(f"one-view groups vs one group: rel L2 {e:.2e}")
+    assert e <= 1e-6, e
+    with pytest.raises(CT2Error):
+        call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(yd), _dev.ptr(indd),
+             _dev.ptr(fvd), _dev.ptr(work), nu * nv * 4 - 16, _dev.stream_handle())
