@@ -56,76 +56,87 @@ constexpr int ZS_NC = 4;                     // candidate detector rows per y ce
 constexpr int ZS_THREADS = 64 * ZS_KZ;
 constexpr int ZS_MAXCOL = 20;                // detector columns per (wavefront, plane), >= 17
 
-// Per-view constants of the fast set-up for a = z, b = x, c = y
-// (setup_fast_ax), FP64 and wavefront-uniform.
-struct ZsView {
-    double Sx, Sy, ia, ta, eux, Dx, evy, Dy;
+// A view's record in the workspace (k_zsep_td writes it), 16-byte aligned
+// parts: per plane k the exact-arithmetic maps X_k(j) = ax + bx j, Y_k(q) =
+// ay + by q rounded to FP32 (ax, bx, ay, by), which only bound the columns
+// and rows searched; per detector column j the forward's (ub0, dub) of its x
+// coordinate, per row q its (uc0, duc) of y (FP32, as setup_fast_ax rounds
+// them); then td [nu][nv].
+struct ZsRec {
+    const float4* pm;
+    const float2* cx;
+    const float2* cy;
+    const float* td;
 };
 
-// Plane k of the tile for view V: steps 1-4 above.
+__host__ __device__ inline size_t zsep_td_offset(int nu, int nv, int nz) {
+    return ((4 * (size_t)nz + 2 * (size_t)nu + 2 * (size_t)nv) + 3) & ~(size_t)3;
+}
+
+__host__ __device__ inline size_t zsep_view_floats(int nu, int nv, int nz) {
+    return (zsep_td_offset(nu, nv, nz) + (size_t)nu * nv + 3) & ~(size_t)3;
+}
+
+__device__ __forceinline__ ZsRec zs_rec(const float* base, const Det& dt, int nz) {
+    ZsRec R;
+    R.pm = reinterpret_cast<const float4*>(base);
+    R.cx = reinterpret_cast<const float2*>(base + 4 * (size_t)nz);
+    R.cy = R.cx + dt.nu;
+    R.td = base + zsep_td_offset(dt.nu, dt.nv, nz);
+    return R;
+}
+
+// Plane k of the tile for view record V: steps 1-4 above.
 __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
-                                         const Det& dt, const ZsView& V, int k, int iw, int c,
-                                         const float* __restrict__ sp, double idx, double idy,
-                                         double dzidx, double dzidy, double ox, double oy,
+                                         const Det& dt, const ZsRec& V, int k, int iw, int c,
                                          float* sg) {
     const int lane = threadIdx.x & 63;
     const int nx = g.n[0], ny = g.n[1];
     const float kf = (float)k;
-    // 1. exact maps: X(j) = ax + bx j, Y(q) = ay + by q on this plane
-    const double T = fma((double)k, (double)g.d[2], V.ta);
-    const double sx = T * V.ia * idx, sy = T * V.ia * idy;
-    const double bx = sx * V.eux * (double)dt.du, by = sy * V.evy * (double)dt.dv;
-    const double ax = fma(V.Sx - ox, idx, -0.5) +
-                      sx * (fma(V.eux, fma(0.5, (double)dt.du, (double)dt.u0), V.Dx) - V.Sx);
-    const double ay = fma(V.Sy - oy, idy, -0.5) +
-                      sy * (fma(V.evy, fma(0.5, (double)dt.dv, (double)dt.v0), V.Dy) - V.Sy);
-    // columns whose sample falls in cells [iw-1, iw+ZS_ROWS): X in [iw-1, iw+ZS_ROWS)
-    const double rbx = rcp_f64(bx), rby = rcp_f64(by);
-    const double e0 = ((double)(iw - 1) - ax) * rbx, e1 = ((double)(iw + ZS_ROWS) - ax) * rbx;
-    const double elo = fmin(fmax(fmin(e0, e1) - 1.0, -1.0), (double)dt.nu);
-    const double ehi = fmin(fmax(fmax(e0, e1) + 1.0, -1.0), (double)dt.nu);
-    const int jlo = max(0, (int)floor(elo)), jhi = min(dt.nu - 1, (int)ceil(ehi));
+    // 1. the plane's maps (uniform): columns whose sample falls in cells
+    // [iw-1, iw+ZS_ROWS), with a margin of one column on each side (the FP32
+    // maps are within 1e-3 column of the exact ones)
+    const float4 pm = V.pm[k];
+    const float rbx = __builtin_amdgcn_rcpf(pm.y), rby = __builtin_amdgcn_rcpf(pm.w);
+    const float e0 = ((float)(iw - 1) - pm.x) * rbx, e1 = ((float)(iw + ZS_ROWS) - pm.x) * rbx;
+    const float elo = fminf(fmaxf(fminf(e0, e1) - 1.0f, -1.0f), (float)dt.nu);
+    const float ehi = fminf(fmaxf(fmaxf(e0, e1) + 1.0f, -1.0f), (float)dt.nu);
+    const int jlo = max(0, (int)floorf(elo)), jhi = min(dt.nu - 1, (int)ceilf(ehi));
     // fewer than 9/ZSEP_SPACING + 5 columns (17), so every one has a lane and
     // an LDS slot
     const int ncol = __builtin_amdgcn_readfirstlane(min(ZS_MAXCOL, jhi - jlo + 1));
     if (ncol <= 0) return;
-    // 2. lane e: column jlo + e, the forward's set-up and its sample
+    // 2. lane e: column jlo + e, its sample on this plane (the forward's)
     int ibl = -2;
     float wbl = 0.0f;
     if (lane < ncol) {
-        const double u = fma((double)(jlo + lane) + 0.5, (double)dt.du, (double)dt.u0);
-        const double r1 = (fma(V.eux, u, V.Dx) - V.Sx) * V.ia;
-        const float ub0 = (float)fma(fma(r1, V.ta, V.Sx) - ox, idx, -0.5);
-        const float dub = (float)(r1 * dzidx);
-        const float X = __builtin_fmaf(kf, dub, ub0);
+        const float2 cx = V.cx[jlo + lane];
+        const float X = __builtin_fmaf(kf, cx.y, cx.x);
         const int ib = floor_i(X);
         wbl = frac_pos(X);
         // rows the wavefront owns, and the forward's validity window
         ibl = (ib >= 0 && ib <= nx - 2) ? ib - iw : -2;
     }
-    // 3. this lane's candidate rows and their weights for y cell c
+    // 3. this lane's candidate rows and their weights for y cell c: the rows
+    // from just below the window [c-1, c+1) (a margin of 0.02 row for the
+    // FP32 maps)
     int rowc[ZS_NC];
     float hc[ZS_NC];
     {
-        const double f0 = ((double)(c - 1) - ay) * rby, f1 = ((double)(c + 1) - ay) * rby;
-        const double flo = fmin(fmax(fmin(f0, f1) - 0.01, -8.0), (double)dt.nv + 8.0);
-        const int q0 = (int)ceil(flo);
+        const float f0 = ((float)(c - 1) - pm.z) * rby, f1 = ((float)(c + 1) - pm.z) * rby;
+        const float flo = fminf(fmaxf(fminf(f0, f1) - 0.02f, -8.0f), (float)dt.nv + 8.0f);
+        const int q0 = (int)ceilf(flo);
 #pragma unroll
         for (int n = 0; n < ZS_NC; ++n) {
             const int q = q0 + n;
             hc[n] = 0.0f;
             rowc[n] = min(max(q, 0), dt.nv - 1);
-            if (q >= 0 && q < dt.nv && c < ny) {
-                const double v = fma((double)q + 0.5, (double)dt.dv, (double)dt.v0);
-                const double r2 = (fma(V.evy, v, V.Dy) - V.Sy) * V.ia;
-                const float uc0 = (float)fma(fma(r2, V.ta, V.Sy) - oy, idy, -0.5);
-                const float duc = (float)(r2 * dzidy);
-                const float Y = __builtin_fmaf(kf, duc, uc0);
-                const int ic = floor_i(Y);
-                const float wc = frac_pos(Y);
-                if (ic >= 0 && ic <= ny - 2)
-                    hc[n] = ic == c ? 1.0f - wc : (ic == c - 1 ? wc : 0.0f);
-            }
+            const float2 cy = V.cy[rowc[n]];
+            const float Y = __builtin_fmaf(kf, cy.y, cy.x);
+            const int ic = floor_i(Y);
+            const float wc = frac_pos(Y);
+            if (q >= 0 && q < dt.nv && c < ny && ic >= 0 && ic <= ny - 2)
+                hc[n] = ic == c ? 1.0f - wc : (ic == c - 1 ? wc : 0.0f);
         }
     }
     // 4a. per column: G = sum hc td over the lane's candidate rows (loads
@@ -134,7 +145,7 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     for (int e = 0; e < ncol; ++e) {
         const int r = __builtin_amdgcn_readlane(ibl, e);
         if (r < -1 || r >= ZS_ROWS) continue;   // uniform: no row of the tile takes it
-        const float* __restrict__ sr = sp + (size_t)(jlo + e) * dt.nv;
+        const float* __restrict__ sr = V.td + (size_t)(jlo + e) * dt.nv;
         float td[ZS_NC];
 #pragma unroll
         for (int n = 0; n < ZS_NC; ++n) td[n] = sr[rowc[n]];
@@ -167,12 +178,12 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     }
 }
 
-// td: k_zsep_td's values of views view0 + pv*vstride (pv < nviews); add:
-// add the sums to vol instead of storing them.
+// work: k_zsep_td's records of views view0 + pv*vstride (pv < nviews);
+// add: add the sums to vol instead of storing them.
 __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view0, int nviews,
                                                         int vstride, int add,
                                                         float* __restrict__ vol,
-                                                        const float* __restrict__ td,
+                                                        const float* __restrict__ work,
                                                         const float* __restrict__ fv) {
     // per wavefront, one slot per (column, lane) for step 4; then the tile's
     // sums for the write-back
@@ -194,28 +205,12 @@ __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view
 #pragma unroll
     for (int r = 0; r < ZS_ROWS; ++r) acc[r] = 0.0f;
     if (k < nz) {   // uniform
-        // the fast set-up's grid constants (setup_fast_ax with a = z, b = x, c = y)
-        const double idx = rcp_f64((double)g.d[0]), idy = rcp_f64((double)g.d[1]);
-        const double dzidx = (double)g.d[2] * idx, dzidy = (double)g.d[2] * idy;
-        const double ox = g.o[0], oy = g.o[1];
-        const size_t vsz = (size_t)dt.nu * dt.nv;
+        const size_t rec = zsep_view_floats(dt.nu, dt.nv, nz);
         for (int pv = 0; pv < nviews; ++pv) {
             const int p = view0 + pv * vstride;
-            const float* f = fv + (size_t)p * 18;
-            if (!view_zsep(f, g, dt)) continue;   // the brick pass takes it
-            ZsView V;
-            V.Sx = uniform_f(f[0]);
-            V.Sy = uniform_f(f[1]);
-            const double Sz = uniform_f(f[2]);
-            // B_z - S_z: e_u and e_v have no z component, so every bin has z = D_z
-            V.ia = rcp_f64((double)uniform_f(f[8]) - Sz);
-            V.ta = fma(0.5, (double)g.d[2], (double)g.o[2]) - Sz;
-            V.eux = uniform_f(f[9]);
-            V.Dx = uniform_f(f[6]);
-            V.evy = uniform_f(f[13]);
-            V.Dy = uniform_f(f[7]);
-            zs_plane(acc, g, dt, V, k, iw, c, td + (size_t)pv * vsz, idx, idy, dzidx, dzidy, ox,
-                     oy, s_lds + wave * (ZS_MAXCOL * 64));
+            if (!view_zsep(fv + (size_t)p * 18, g, dt)) continue;   // the brick pass takes it
+            zs_plane(acc, g, dt, zs_rec(work + (size_t)pv * rec, dt, nz), k, iw, c,
+                     s_lds + wave * (ZS_MAXCOL * 64));
         }
     }
     // write-back through LDS: each thread stores 4 consecutive z values of
@@ -253,12 +248,14 @@ bool zsep_route(const ct2_desc* d, const Grid& g) {
            g.s[0] == g.n[1] * g.n[2];
 }
 
-// td = trav datum of every ray of views view0 + pv*vstride (pv < nviews)
-// that view_zsep accepts, 0 where the indicator leaves a ray out: one
-// workgroup per (view, detector column), the column's set-up (r1) shared,
-// trav = |d_z| fast_sqrt((1 + r1f^2) + r2f^2) as setup_fast_ax forms it.
+// The records of views view0 + pv*vstride (pv < nviews) that view_zsep
+// accepts (ZsRec): the planes' maps; the columns' and rows' set-ups; td =
+// trav datum of every ray, 0 where the indicator leaves it out, with trav =
+// |d_z| fast_sqrt((1 + r1f^2) + r2f^2) as setup_fast_ax forms it.  One
+// workgroup per (view, detector column); the workgroups of column 0 also
+// write the rows and the planes.
 __global__ __launch_bounds__(256) void k_zsep_td(Grid g, Det dt, int view0, int nviews,
-                                                 int vstride, float* __restrict__ td,
+                                                 int vstride, float* __restrict__ work,
                                                  const float* __restrict__ sino,
                                                  const int8_t* __restrict__ ind,
                                                  const float* __restrict__ fv) {
@@ -267,25 +264,58 @@ __global__ __launch_bounds__(256) void k_zsep_td(Grid g, Det dt, int view0, int 
     const int p = view0 + pv * vstride;
     const float* f = fv + (size_t)p * 18;
     if (!view_zsep(f, g, dt)) return;   // uniform; the tiles skip the view too
+    const int nz = g.n[2];
+    float* rec = work + (size_t)pv * zsep_view_floats(dt.nu, dt.nv, nz);
+    float4* pm = reinterpret_cast<float4*>(rec);
+    float2* cx = reinterpret_cast<float2*>(rec + 4 * (size_t)nz);
+    float2* cy = cx + dt.nu;
+    // the fast set-up (setup_fast_ax with a = z, b = x, c = y): e_u, e_v have
+    // no z component, so every bin has z = D_z
     const double Sx = f[0], Sy = f[1], Sz = f[2];
     const double ia = rcp_f64((double)f[8] - Sz);
+    const double ta = fma(0.5, (double)g.d[2], (double)g.o[2]) - Sz;
+    const double idx = rcp_f64((double)g.d[0]), idy = rcp_f64((double)g.d[1]);
+    const double dzidx = (double)g.d[2] * idx, dzidy = (double)g.d[2] * idy;
+    const double ox = g.o[0], oy = g.o[1];
     const double u = fma((double)j + 0.5, (double)dt.du, (double)dt.u0);
-    const float r1f = (float)((fma((double)f[9], u, (double)f[6]) - Sx) * ia);
+    const double r1 = (fma((double)f[9], u, (double)f[6]) - Sx) * ia;
+    if (threadIdx.x == 0)
+        cx[j] = make_float2((float)fma(fma(r1, ta, Sx) - ox, idx, -0.5), (float)(r1 * dzidx));
+    const float r1f = (float)r1;
     const float q1 = 1.0f + r1f * r1f;
     const float adz = fabsf(g.d[2]);
     const size_t row = ((size_t)p * dt.nu + j) * dt.nv;
-    float* __restrict__ out = td + ((size_t)pv * dt.nu + j) * dt.nv;
+    float* __restrict__ out = rec + zsep_td_offset(dt.nu, dt.nv, nz) + (size_t)j * dt.nv;
     for (int k = threadIdx.x; k < dt.nv; k += 256) {
         const double v = fma((double)k + 0.5, (double)dt.dv, (double)dt.v0);
-        const float r2f = (float)((fma((double)f[13], v, (double)f[7]) - Sy) * ia);
+        const double r2 = (fma((double)f[13], v, (double)f[7]) - Sy) * ia;
+        const float r2f = (float)r2;
         const float trav = adz * fast_sqrt(q1 + r2f * r2f);
         const bool live = ind == nullptr || ind[row + k] == 1;
         out[k] = live ? sino[row + k] * trav : 0.0f;
+        if (j == 0)
+            cy[k] = make_float2((float)fma(fma(r2, ta, Sy) - oy, idy, -0.5), (float)(r2 * dzidy));
+    }
+    if (j == 0) {
+        const double eux = f[9], evy = f[13];
+        for (int k = threadIdx.x; k < nz; k += 256) {
+            // X_k(j) = (S_x + r1(j) T_k - o_x)/d_x - 0.5 with r1 affine in j
+            const double T = fma((double)k, (double)g.d[2], ta);
+            const double sx = T * ia * idx, sy = T * ia * idy;
+            const double ax = fma(Sx - ox, idx, -0.5) +
+                              sx * (fma(eux, fma(0.5, (double)dt.du, (double)dt.u0), (double)f[6]) - Sx);
+            const double ay = fma(Sy - oy, idy, -0.5) +
+                              sy * (fma(evy, fma(0.5, (double)dt.dv, (double)dt.v0), (double)f[7]) - Sy);
+            pm[k] = make_float4((float)ax, (float)(sx * eux * (double)dt.du), (float)ay,
+                                (float)(sy * evy * (double)dt.dv));
+        }
     }
 }
 
-// Bytes of one view's td values.
-static size_t zsep_view_bytes(const Det& dt) { return (size_t)dt.nu * dt.nv * sizeof(float); }
+// Bytes of one view's record.
+static size_t zsep_view_bytes(const Det& dt, int nz) {
+    return zsep_view_floats(dt.nu, dt.nv, nz) * sizeof(float);
+}
 
 // An internally allocated workspace holds the td values of at most this
 // many bytes of views (the tomosynthesis scan of 25 views of 2048 x 1536
@@ -293,8 +323,7 @@ static size_t zsep_view_bytes(const Det& dt) { return (size_t)dt.nu * dt.nv * si
 constexpr size_t ZSEP_WS_CAP = (size_t)4 << 30;
 
 size_t zsep_work_bytes(const ct2_desc* d, const Det& dt, const ViewSet& vs) {
-    (void)d;
-    return (size_t)vs.nviews * zsep_view_bytes(dt);
+    return (size_t)vs.nviews * zsep_view_bytes(dt, d->nz);
 }
 
 int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
@@ -306,7 +335,7 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
         set_error("ct2_backproject: volume too large for the plane-separable route");
         return CT2_EINVAL;
     }
-    const size_t per = zsep_view_bytes(dt);
+    const size_t per = zsep_view_bytes(dt, g.n[2]);
     void* own = nullptr;
     size_t gv;
     if (w.ptr) {
@@ -315,7 +344,7 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
             return CT2_EINVAL;
         }
         if (w.bytes < per) {
-            set_error("ct2_backproject: workspace of %zu B holds no view's values (%zu B)",
+            set_error("ct2_backproject: workspace of %zu B holds no view's record (%zu B)",
                       w.bytes, per);
             return CT2_EINVAL;
         }

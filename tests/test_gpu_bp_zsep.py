@@ -176,10 +176,11 @@ def test_zsep_dot_product(seed):
 
 
 def test_zsep_workspace_groups():
-    """ct2_backproject_work_bytes gives the route's workspace (trav * datum of
-    every view); through ct2_backproject_ws a workspace of one view's values
-    runs the views one group at a time (the first stores, the rest add) and
-    matches the allocating entry; a smaller one is rejected."""
+    """ct2_backproject_work_bytes gives the route's workspace (per view: the
+    planes' maps, the columns' and rows' set-ups, trav * datum of every ray,
+    in 16-byte aligned parts); through ct2_backproject_ws a workspace of one
+    view's record runs the views one group at a time (the first stores, the
+    rest add) and matches the allocating entry; a smaller one is rejected."""
     from ct2 import _dev
     from ct2._lib import CT2Error, call, lib
     fv, geo, g2, ind, rng = _case(ns=6)
@@ -187,13 +188,15 @@ def test_zsep_workspace_groups():
     nx, ny, nz = g2[6:]
     y = rng.standard_normal(geo[:3]).astype(np.float32)
     d = _desc(geo, g2)
-    assert lib().ct2_backproject_work_bytes(C.byref(d)) == ns * nu * nv * 4
+    r4 = lambda n: (n + 3) // 4 * 4   # noqa: E731
+    rec = r4(r4(4 * nz + 2 * nu + 2 * nv) + nu * nv) * 4
+    assert lib().ct2_backproject_work_bytes(C.byref(d)) == ns * rec
     want = _adjoint(fv, geo, g2, ind, y)
     yd, fvd, indd = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (y, fv, ind))
     out = torch.full((nx, ny, nz), float("nan"), dtype=torch.float32, device="cuda")
-    work = torch.empty(nu * nv, dtype=torch.float32, device="cuda")
+    work = torch.empty(rec // 4, dtype=torch.float32, device="cuda")
     call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(yd), _dev.ptr(indd),
-         _dev.ptr(fvd), _dev.ptr(work), nu * nv * 4, _dev.stream_handle())
+         _dev.ptr(fvd), _dev.ptr(work), rec, _dev.stream_handle())
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     e = rel_l2(got, want)
@@ -201,4 +204,4 @@ def test_zsep_workspace_groups():
     assert e <= 1e-6, e
     with pytest.raises(CT2Error):
         call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(yd), _dev.ptr(indd),
-             _dev.ptr(fvd), _dev.ptr(work), nu * nv * 4 - 16, _dev.stream_handle())
+             _dev.ptr(fvd), _dev.ptr(work), rec - 16, _dev.stream_handle())
