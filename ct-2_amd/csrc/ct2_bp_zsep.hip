@@ -25,11 +25,12 @@
 //   1. evaluates the exact-arithmetic maps X_l(j) = ax + bx j and Y_l(k) =
 //      ay + by k (FP64), which bound the detector columns reaching its rows
 //      (fewer than 9/ZSEP_SPACING + 5, so at most 17) and each lane's
-//      candidate rows (4: a window [c-1, c+1) holds at most 2/ZSEP_SPACING
-//      + 1 < 4 rows, with a margin of 0.01 row for the FP32 rounding of Y);
+//      candidate rows: from the first row within 0.02 row of the window
+//      [c-1, c+1) (a margin for the FP32 maps), the rows up to 2/ZSEP_SPACING
+//      + 0.04 < 3 rows further, so 3 candidates;
 //   2. lane e sets up column e of that range with the forward's FP64 set-up
 //      expressions (ub0, dub) and its sample's cell and weight;
-//   3. each lane sets up its 4 candidate rows (uc0, duc), their cells and
+//   3. each lane sets up its 3 candidate rows (uc0, duc), their cells and
 //      their weights hc for its own y cell;
 //   4. for each column, whose cell and weight it reads from lane e
 //      (v_readlane, scalar), gathers G = sum hc td over its
@@ -52,7 +53,7 @@ namespace ct2 {
 constexpr int ZS_ROWS = 8;                   // x rows per workgroup (all in each wavefront)
 constexpr int ZS_TY = 64;                    // y cells per workgroup (one per lane)
 constexpr int ZS_KZ = 8;                     // z planes per workgroup (one per wavefront)
-constexpr int ZS_NC = 4;                     // candidate detector rows per y cell
+constexpr int ZS_NC = 3;                     // candidate detector rows per y cell
 constexpr int ZS_THREADS = 64 * ZS_KZ;
 constexpr int ZS_MAXCOL = 20;                // detector columns per (wavefront, plane), >= 17
 

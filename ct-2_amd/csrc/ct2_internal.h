@@ -502,8 +502,8 @@ __device__ inline bool view_separable(const float* __restrict__ f, const Grid& g
 // qualifies: those zero entries, D_z != S_z, and on every plane the ray
 // spacing in voxels, |t_l e_u,x du / d_x| and |t_l e_v,y dv / d_y| with t_l
 // the plane's fraction of the ray (of one sign over the planes), at least
-// ZSEP_SPACING, so that a voxel's [c-1, c+1) window holds at most four
-// detector rows (k_bp_zsep).
+// ZSEP_SPACING, so that the detector rows within 0.02 row of a voxel's
+// [c-1, c+1) window span less than 3 rows (k_bp_zsep takes 3 candidates).
 constexpr double ZSEP_SPACING = 0.7;
 
 __device__ inline bool view_zsep(const float* __restrict__ f, const Grid& g, const Det& dt) {
