@@ -81,29 +81,34 @@ int ct2_desc_init(ct2_desc* desc, int ns, int nu, int nv, float du, float dv, fl
 int ct2_project(const ct2_desc* desc, float* sino, const int8_t* ind, const float* fv,
                 const void* vol, ct2_stream_t stream);
 /* Exact adjoint of ct2_project (same matrix entries), written over vol
- * (storage desc->vol_dtype).  The column-separable route (fast linear
- * arithmetic into float32, views whose detector rows run along z) builds
- * per-(view, column) tables in device memory: this entry allocates them on
- * the stream (hipMallocAsync, at most 4 GiB, freed on the stream; the
- * device's default stream-ordered pool is set to keep up to 4 GiB of freed
- * memory for the next call).  ct2_backproject_ws takes them from the caller
- * instead. */
+ * (storage desc->vol_dtype).  Two routes build tables in device memory: the
+ * column-separable route (fast linear arithmetic into float32, views whose
+ * detector rows run along z) per-(view, column) prefix sums, and the
+ * plane-separable route (the same arithmetic with desc->axis CT2_AXIS_Z,
+ * views whose detector is parallel to the z planes with e_u along x and e_v
+ * along y) per-view set-ups and trav*datum of every ray.  This entry
+ * allocates them on the stream (hipMallocAsync, at most 4 GiB, freed on the
+ * stream; the device's default stream-ordered pool is set to keep up to
+ * 4 GiB of freed memory for the next call).  ct2_backproject_ws takes them
+ * from the caller instead. */
 int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
                     const float* fv, ct2_stream_t stream);
 /* ct2_backproject with a caller-owned device workspace (16-byte aligned) for
- * the column-separable route: a 256-byte header and the column tables,
- * work_bytes >= 256 + one view's tables (nu * (nv+1) * 16 bytes);
- * ct2_backproject_work_bytes(desc) bytes hold the tables of every view of
- * the subset, and a smaller workspace processes the views in groups.  The call allocates nothing; it must not overlap another
- * call that uses the same workspace.  work == NULL behaves as
+ * the separable routes.  Column-separable: a 256-byte header and the column
+ * tables, work_bytes >= 256 + one view's tables (nu * (nv+1) * 16 bytes).
+ * Plane-separable: one record per view of 4 * (r4(4 nz + 2 nu + 2 nv) +
+ * nu nv) bytes rounded up to 16 (r4: up to a multiple of 4), work_bytes >=
+ * one record.  ct2_backproject_work_bytes(desc) bytes hold every view of the
+ * subset, and a smaller workspace processes the views in groups.  The call
+ * allocates nothing; it must not overlap another call that uses the same
+ * workspace.  work == NULL behaves as
  * ct2_backproject.  No counterpart in the reference (view_proj3D.f:1096-1370
  * has no workspace); for callers that keep a workspace across calls (graph
  * capture, iteration loops). */
 int ct2_backproject_ws(const ct2_desc* desc, void* vol, const float* sino, const int8_t* ind,
                        const float* fv, void* work, size_t work_bytes, ct2_stream_t stream);
 /* Bytes of ct2_backproject_ws's workspace for every view of desc's subset; 0
- * when the column-separable route does not apply (then no workspace is
- * used). */
+ * when neither separable route applies (then no workspace is used). */
 size_t ct2_backproject_work_bytes(const ct2_desc* desc);
 /* float32 volume -> 16-bit storage (dtype CT2_F16 or CT2_BF16) for the
  * mixed-precision forward projection.  CT2_ROUND_STOCHASTIC picks the upper
