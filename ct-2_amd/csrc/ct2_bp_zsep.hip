@@ -352,7 +352,9 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
         gv = w.bytes / per;
     } else {
         gv = max((size_t)1, ZSEP_WS_CAP / per);
-        gv = min(gv, (size_t)max(vs.nviews, 1));
+    }
+    gv = min(gv, (size_t)max(vs.nviews, 1));   // so the int view counts below hold it
+    if (!w.ptr) {
         int dev = 0;
         CT2_HIP(hipGetDevice(&dev));
         keep_pool_memory(dev);
