@@ -17,35 +17,40 @@
 // y cells) is taken per (column j, voxel column c); the outer one adds it to
 // the two x rows column j reaches.
 //
+// Records.  k_zsep_td writes one record per view into a workspace: the
+// columns' (ub0, dub) and the rows' (uc0, duc), the forward's FP32 values
+// from its own FP64 set-up expressions; per plane the exact-arithmetic maps
+// X_l(j) = ax + bx j and Y_l(k) = ay + by k rounded to FP32, which only
+// bound searches; and td = trav datum of every ray (0 where the indicator
+// leaves it out), trav formed as the forward's set-up forms it (one
+// v_sqrt_f32 per ray, not one per (ray, plane)).
+//
 // Tiles.  A workgroup owns 8 x rows, 64 y cells and 8 z planes; wavefront w
 // owns plane k0 + w, lane c one y cell, and sums its 8 voxels in FP32
 // registers over every view, so each voxel is stored once, with no atomics
-// (an LDS transpose at the end makes each lane's store 8 contiguous z
+// (an LDS transpose at the end makes each thread's store 4 contiguous z
 // values).  Per (view, plane) a wavefront
-//   1. evaluates the exact-arithmetic maps X_l(j) = ax + bx j and Y_l(k) =
-//      ay + by k (FP64), which bound the detector columns reaching its rows
-//      (fewer than 9/ZSEP_SPACING + 5, so at most 17) and each lane's
-//      candidate rows: from the first row within 0.02 row of the window
-//      [c-1, c+1) (a margin for the FP32 maps), the rows up to 2/ZSEP_SPACING
-//      + 0.04 < 3 rows further, so 3 candidates;
-//   2. lane e sets up column e of that range with the forward's FP64 set-up
-//      expressions (ub0, dub) and its sample's cell and weight;
-//   3. each lane sets up its 3 candidate rows (uc0, duc), their cells and
-//      their weights hc for its own y cell;
+//   1. bounds, from the plane's maps, the detector columns whose sample can
+//      fall in its cells (a margin of one column; fewer than 9/ZSEP_SPACING
+//      + 5, so at most 17) and each lane's candidate rows: from the first row
+//      within 0.02 row of the window [c-1, c+1) (a margin for the FP32
+//      maps), the rows up to 2/ZSEP_SPACING + 0.04 < 3 rows further, so 3
+//      candidates;
+//   2. lane e evaluates column e's sample on the plane, X = fma(l, dub,
+//      ub0), its cell and weight;
+//   3. each lane evaluates its 3 candidate rows' Y = fma(l, duc, uc0), their
+//      cells and their weights hc for its own y cell;
 //   4. for each column, whose cell and weight it reads from lane e
-//      (v_readlane, scalar), gathers G = sum hc td over its
-//      candidate rows (coalesced: neighbouring lanes read neighbouring
-//      detector rows) into the lane's own LDS slot of that column; then each
-//      x row adds (1-wb) G of the columns whose cell it is and wb G of those
-//      whose cell is the row below (ballot masks; the slots stand in for a
-//      register array indexed by the column, which the compiler would put
-//      in scratch memory).
-// td = trav datum (0 for rays the indicator leaves out) is formed once per
-// ray by k_zsep_td into a workspace (trav as the forward's set-up forms it,
-// one v_sqrt_f32), not once per (ray, plane) in the tiles.  The matrix
-// entries are the brick kernel's (the same FP32 cell coordinates, weights
-// and trav); only the grouping of the sums differs, so the result is the
-// fast forward's transpose up to FP32 rounding.
+//      (v_readlane, scalar), gathers G = sum hc td over its candidate rows
+//      (coalesced: neighbouring lanes read neighbouring detector rows) into
+//      the lane's own LDS slot of that column; then each x row adds (1-wb) G
+//      of the columns whose cell it is and wb G of those whose cell is the
+//      row below (ballot masks; the slots stand in for a register array
+//      indexed by the column, which the compiler would put in scratch
+//      memory).
+// The matrix entries are the brick kernel's (the same FP32 cell
+// coordinates, weights and trav); only the grouping of the sums differs, so
+// the result is the fast forward's transpose up to FP32 rounding.
 #include "ct2_internal.h"
 
 namespace ct2 {
@@ -98,6 +103,7 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     // [iw-1, iw+ZS_ROWS), with a margin of one column on each side (the FP32
     // maps are within 1e-3 column of the exact ones)
     const float4 pm = V.pm[k];
+    if (pm.y == 0.0f) return;   // uniform: a view the brick pass takes
     const float rbx = __builtin_amdgcn_rcpf(pm.y), rby = __builtin_amdgcn_rcpf(pm.w);
     const float e0 = ((float)(iw - 1) - pm.x) * rbx, e1 = ((float)(iw + ZS_ROWS) - pm.x) * rbx;
     const float elo = fminf(fmaxf(fminf(e0, e1) - 1.0f, -1.0f), (float)dt.nu);
@@ -179,13 +185,12 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     }
 }
 
-// work: k_zsep_td's records of views view0 + pv*vstride (pv < nviews);
-// add: add the sums to vol instead of storing them.
-__global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view0, int nviews,
-                                                        int vstride, int add,
+// work: k_zsep_td's records of the nviews views of the group (those the
+// brick pass takes are marked); add: add the sums to vol instead of storing
+// them.
+__global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int nviews, int add,
                                                         float* __restrict__ vol,
-                                                        const float* __restrict__ work,
-                                                        const float* __restrict__ fv) {
+                                                        const float* __restrict__ work) {
     // per wavefront, one slot per (column, lane) for step 4; then the tile's
     // sums for the write-back
     constexpr int SG = ZS_KZ * ZS_MAXCOL * 64, SO = ZS_ROWS * ZS_TY * (ZS_KZ + 1);
@@ -208,8 +213,6 @@ __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int view
     if (k < nz) {   // uniform
         const size_t rec = zsep_view_floats(dt.nu, dt.nv, nz);
         for (int pv = 0; pv < nviews; ++pv) {
-            const int p = view0 + pv * vstride;
-            if (!view_zsep(fv + (size_t)p * 18, g, dt)) continue;   // the brick pass takes it
             zs_plane(acc, g, dt, zs_rec(work + (size_t)pv * rec, dt, nz), k, iw, c,
                      s_lds + wave * (ZS_MAXCOL * 64));
         }
@@ -264,10 +267,16 @@ __global__ __launch_bounds__(256) void k_zsep_td(Grid g, Det dt, int view0, int 
     if (pv >= nviews) return;
     const int p = view0 + pv * vstride;
     const float* f = fv + (size_t)p * 18;
-    if (!view_zsep(f, g, dt)) return;   // uniform; the tiles skip the view too
     const int nz = g.n[2];
     float* rec = work + (size_t)pv * zsep_view_floats(dt.nu, dt.nv, nz);
     float4* pm = reinterpret_cast<float4*>(rec);
+    if (!view_zsep(f, g, dt)) {   // uniform
+        // the brick pass takes the view; its maps marked with bx = 0 (a view
+        // the route takes has |bx| >= ZSEP_SPACING) tell the tiles to skip it
+        if (j == 0)
+            for (int k = threadIdx.x; k < nz; k += 256) pm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
     float2* cx = reinterpret_cast<float2*>(rec + 4 * (size_t)nz);
     float2* cy = cx + dt.nu;
     // the fast set-up (setup_fast_ax with a = z, b = x, c = y): e_u, e_v have
@@ -370,8 +379,8 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
         if (items > 0)
             hipLaunchKernelGGL(k_zsep_td, dim3((unsigned)items), dim3(256), 0, st, g, dt, v0, nvg,
                                vs.vstride, td, sino, ind, fv);
-        hipLaunchKernelGGL(k_bp_zsep, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g, dt, v0,
-                           nvg, vs.vstride, v > 0 ? 1 : 0, vol, td, fv);
+        hipLaunchKernelGGL(k_bp_zsep, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g, dt, nvg,
+                           v > 0 ? 1 : 0, vol, td);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { rc = hip_fail(e, "ct2_backproject"); break; }
     }
