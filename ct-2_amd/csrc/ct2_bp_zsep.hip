@@ -39,7 +39,8 @@
 //   2. lane e evaluates column e's sample on the plane, X = fma(l, dub,
 //      ub0), its cell and weight;
 //   3. each lane evaluates its 3 candidate rows' Y = fma(l, duc, uc0), their
-//      cells and their weights hc for its own y cell;
+//      cells and their weights hc for its own y cell (the window moved
+//      inside the detector at its edges, so the 3 rows are one load);
 //   4. for each column, whose cell and weight it reads from lane e
 //      (v_readlane, scalar), gathers G = sum hc td over its candidate rows
 //      (coalesced: neighbouring lanes read neighbouring detector rows) into
@@ -127,22 +128,22 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     // 3. this lane's candidate rows and their weights for y cell c: the rows
     // from just below the window [c-1, c+1) (a margin of 0.02 row for the
     // FP32 maps)
-    int rowc[ZS_NC];
+    int q0;
     float hc[ZS_NC];
     {
         const float f0 = ((float)(c - 1) - pm.z) * rby, f1 = ((float)(c + 1) - pm.z) * rby;
         const float flo = fminf(fmaxf(fminf(f0, f1) - 0.02f, -8.0f), (float)dt.nv + 8.0f);
-        const int q0 = (int)ceilf(flo);
+        // the window moved inside the detector (nv >= ZS_NC) still holds
+        // every row of [q0, q0+ZS_NC) that exists
+        q0 = min(max((int)ceilf(flo), 0), dt.nv - ZS_NC);
 #pragma unroll
         for (int n = 0; n < ZS_NC; ++n) {
-            const int q = q0 + n;
             hc[n] = 0.0f;
-            rowc[n] = min(max(q, 0), dt.nv - 1);
-            const float2 cy = V.cy[rowc[n]];
+            const float2 cy = V.cy[q0 + n];
             const float Y = __builtin_fmaf(kf, cy.y, cy.x);
             const int ic = floor_i(Y);
             const float wc = frac_pos(Y);
-            if (q >= 0 && q < dt.nv && c < ny && ic >= 0 && ic <= ny - 2)
+            if (c < ny && ic >= 0 && ic <= ny - 2)
                 hc[n] = ic == c ? 1.0f - wc : (ic == c - 1 ? wc : 0.0f);
         }
     }
@@ -152,10 +153,10 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     for (int e = 0; e < ncol; ++e) {
         const int r = __builtin_amdgcn_readlane(ibl, e);
         if (r < -1 || r >= ZS_ROWS) continue;   // uniform: no row of the tile takes it
-        const float* __restrict__ sr = V.td + (size_t)(jlo + e) * dt.nv;
-        float td[ZS_NC];
-#pragma unroll
-        for (int n = 0; n < ZS_NC; ++n) td[n] = sr[rowc[n]];
+        // the lane's ZS_NC consecutive rows in one load
+        struct Rows { float v[ZS_NC]; };
+        const Rows rw = *reinterpret_cast<const Rows*>(V.td + (size_t)(jlo + e) * dt.nv + q0);
+        const float* td = rw.v;
         float G = 0.0f;
 #pragma unroll
         for (int n = 0; n < ZS_NC; ++n) {
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(ZS_THREADS) void k_bp_zsep(Grid g, Det dt, int nvie
 bool zsep_route(const ct2_desc* d, const Grid& g) {
     return d->brick[0] <= 0 && d->brick[1] <= 0 && d->brick[2] <= 0 &&
            d->arith == CT2_ARITH_FAST && d->interp == CT2_LINEAR && d->vol_dtype == CT2_F32 &&
-           d->axis == CT2_AXIS_Z && g.s[2] == 1 && g.s[1] == g.n[2] &&
+           d->axis == CT2_AXIS_Z && d->nv >= ZS_NC && g.s[2] == 1 && g.s[1] == g.n[2] &&
            g.s[0] == g.n[1] * g.n[2];
 }
 
