@@ -783,13 +783,17 @@ static int launch_brick(const ct2_desc* d, const Grid& g, const Det& dt, const V
         // plane-separable kernel stores every voxel, then one workgroup per
         // brick adds any other view
         if (zsep_route(d, g)) {
-            int rc = launch_backproject_zsep(g, dt, vs, (float*)vol, sino, ind, fv, st, w);
-            if (rc) return rc;
-            hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>),
-                               dim3(nb.x * nb.y * nb.z), dim3(BRICK_THREADS), lds, st, g, dt,
-                               vs.view0, vs.nviews, vs.vstride, zonly, b, nb, 1,
-                               BRICK_SKIP_ZSEP, (OT*)vol, sino, ind, fv, (const int*)nullptr);
-            return CT2_OK;
+            void* own = nullptr;
+            const int* nonsep = nullptr;
+            int rc = launch_backproject_zsep(g, dt, vs, (float*)vol, sino, ind, fv, st, w, &own,
+                                             &nonsep);
+            if (rc == CT2_OK)
+                hipLaunchKernelGGL((k_backproject_brick<INTERP, ARITH, OT>),
+                                   dim3(nb.x * nb.y * nb.z), dim3(BRICK_THREADS), lds, st, g, dt,
+                                   vs.view0, vs.nviews, vs.vstride, zonly, b, nb, 1,
+                                   BRICK_SKIP_ZSEP, (OT*)vol, sino, ind, fv, nonsep);
+            const int rf = sep_work_release(own, st);
+            return rc ? rc : rf;
         }
         if (sep_route(d, g)) {
             void* own = nullptr;

@@ -153,10 +153,10 @@ __device__ __forceinline__ void zs_plane(float (&acc)[ZS_ROWS], const Grid& g,
     for (int e = 0; e < ncol; ++e) {
         const int r = __builtin_amdgcn_readlane(ibl, e);
         if (r < -1 || r >= ZS_ROWS) continue;   // uniform: no row of the tile takes it
-        // the lane's ZS_NC consecutive rows in one load
-        struct Rows { float v[ZS_NC]; };
-        const Rows rw = *reinterpret_cast<const Rows*>(V.td + (size_t)(jlo + e) * dt.nv + q0);
-        const float* td = rw.v;
+        // the lane's ZS_NC consecutive rows in one load (a 4-byte aligned
+        // copy, which the compiler issues as one global_load_dwordx3)
+        float td[ZS_NC];
+        __builtin_memcpy(td, V.td + (size_t)(jlo + e) * dt.nv + q0, sizeof td);
         float G = 0.0f;
 #pragma unroll
         for (int n = 0; n < ZS_NC; ++n) {
@@ -258,9 +258,11 @@ bool zsep_route(const ct2_desc* d, const Grid& g) {
 // trav datum of every ray, 0 where the indicator leaves it out, with trav =
 // |d_z| fast_sqrt((1 + r1f^2) + r2f^2) as setup_fast_ax forms it.  One
 // workgroup per (view, detector column); the workgroups of column 0 also
-// write the rows and the planes.
+// write the rows and the planes, and count the views the route leaves
+// into *nonsep (the brick pass after the tiles exits at once when it is 0).
 __global__ __launch_bounds__(256) void k_zsep_td(Grid g, Det dt, int view0, int nviews,
                                                  int vstride, float* __restrict__ work,
+                                                 int* __restrict__ nonsep,
                                                  const float* __restrict__ sino,
                                                  const int8_t* __restrict__ ind,
                                                  const float* __restrict__ fv) {
@@ -274,8 +276,10 @@ __global__ __launch_bounds__(256) void k_zsep_td(Grid g, Det dt, int view0, int 
     if (!view_zsep(f, g, dt)) {   // uniform
         // the brick pass takes the view; its maps marked with bx = 0 (a view
         // the route takes has |bx| >= ZSEP_SPACING) tell the tiles to skip it
-        if (j == 0)
+        if (j == 0) {
             for (int k = threadIdx.x; k < nz; k += 256) pm[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (threadIdx.x == 0) atomicAdd(nonsep, 1);
+        }
         return;
     }
     float2* cx = reinterpret_cast<float2*>(rec + 4 * (size_t)nz);
@@ -333,13 +337,23 @@ static size_t zsep_view_bytes(const Det& dt, int nz) {
 // needs 315 MB).
 constexpr size_t ZSEP_WS_CAP = (size_t)4 << 30;
 
+// The workspace: a header (the count of views the route leaves to the
+// brick pass), then one record per view of a group.
+constexpr size_t ZSEP_WS_HEAD = 256;
+
 size_t zsep_work_bytes(const ct2_desc* d, const Det& dt, const ViewSet& vs) {
-    return (size_t)vs.nviews * zsep_view_bytes(dt, d->nz);
+    return ZSEP_WS_HEAD + (size_t)vs.nviews * zsep_view_bytes(dt, d->nz);
+}
+
+__global__ void k_zsep_clear(int* nonsep) {
+    if (threadIdx.x == 0) *nonsep = 0;
 }
 
 int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
                             const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
-                            SepWork w) {
+                            SepWork w, void** own_out, const int** nonsep) {
+    *own_out = nullptr;
+    *nonsep = nullptr;
     const long long ntile = (long long)((g.n[0] + ZS_ROWS - 1) / ZS_ROWS) *
                             ((g.n[1] + ZS_TY - 1) / ZS_TY) * ((g.n[2] + ZS_KZ - 1) / ZS_KZ);
     if (ntile > 0x7fffffffLL) {
@@ -354,12 +368,12 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
             set_error("ct2_backproject: workspace must be 16-byte aligned");
             return CT2_EINVAL;
         }
-        if (w.bytes < per) {
+        if (w.bytes < ZSEP_WS_HEAD + per) {
             set_error("ct2_backproject: workspace of %zu B holds no view's record (%zu B)",
-                      w.bytes, per);
+                      w.bytes, ZSEP_WS_HEAD + per);
             return CT2_EINVAL;
         }
-        gv = w.bytes / per;
+        gv = (w.bytes - ZSEP_WS_HEAD) / per;
     } else {
         gv = max((size_t)1, ZSEP_WS_CAP / per);
     }
@@ -368,9 +382,14 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
         int dev = 0;
         CT2_HIP(hipGetDevice(&dev));
         keep_pool_memory(dev);
-        CT2_HIP(hipMallocAsync(&own, per * gv, st));
+        CT2_HIP(hipMallocAsync(&own, ZSEP_WS_HEAD + per * gv, st));
+        *own_out = own;
     }
-    float* td = static_cast<float*>(own ? own : w.ptr);
+    char* base = static_cast<char*>(own ? own : w.ptr);
+    int* count = reinterpret_cast<int*>(base);
+    float* td = reinterpret_cast<float*>(base + ZSEP_WS_HEAD);
+    // a kernel, not hipMemsetAsync, so that a captured graph replays it
+    hipLaunchKernelGGL(k_zsep_clear, dim3(1), dim3(64), 0, st, count);
     int rc = CT2_OK;
     // groups of gv views: the first stores every voxel, later ones add, in
     // stream order (so the result does not depend on the schedule)
@@ -379,16 +398,16 @@ int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, flo
         const long long items = (long long)nvg * dt.nu;
         if (items > 0)
             hipLaunchKernelGGL(k_zsep_td, dim3((unsigned)items), dim3(256), 0, st, g, dt, v0, nvg,
-                               vs.vstride, td, sino, ind, fv);
+                               vs.vstride, td, count, sino, ind, fv);
         hipLaunchKernelGGL(k_bp_zsep, dim3((unsigned)ntile), dim3(ZS_THREADS), 0, st, g, dt, nvg,
                            v > 0 ? 1 : 0, vol, td);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { rc = hip_fail(e, "ct2_backproject"); break; }
     }
-    if (own) {
-        hipError_t e = hipFreeAsync(own, st);
-        if (rc == CT2_OK && e != hipSuccess) rc = hip_fail(e, "ct2_backproject");
-    }
+    // the caller frees an allocated workspace (sep_work_release) after the
+    // brick pass, which reads the count
+    *own_out = own;
+    *nonsep = count;
     return rc;
 }
 

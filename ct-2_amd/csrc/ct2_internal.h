@@ -827,13 +827,16 @@ inline const int* sep_nonsep_count(const SepWork& w) { return static_cast<const 
 // linear arithmetic into an f32 volume in C order, axis forced to z, no
 // brick named.  It stores every voxel from the views view_zsep accepts; a
 // brick pass then adds the others.
-// Its workspace holds trav*datum of a group of views (w, or allocated on
-// the stream when w is empty); zsep_work_bytes is the size for every view.
+// Its workspace (w, or allocated on the stream when w is empty and
+// returned in *own, for sep_work_release after the brick pass) holds a
+// header, the device count of views the route leaves (*nonsep, for the
+// brick pass), and the records of a group of views; zsep_work_bytes is the
+// size for every view.
 bool zsep_route(const ct2_desc* d, const Grid& g);
 size_t zsep_work_bytes(const ct2_desc* d, const Det& dt, const ViewSet& vs);
 int launch_backproject_zsep(const Grid& g, const Det& dt, const ViewSet& vs, float* vol,
                             const float* sino, const int8_t* ind, const float* fv, hipStream_t st,
-                            SepWork w);
+                            SepWork w, void** own, const int** nonsep);
 // Lets the device's stream-ordered pool keep freed workspaces for the next
 // call instead of unmapping them at each synchronisation (set once per
 // device).

@@ -96,9 +96,9 @@ int ct2_backproject(const ct2_desc* desc, void* vol, const float* sino, const in
 /* ct2_backproject with a caller-owned device workspace (16-byte aligned) for
  * the separable routes.  Column-separable: a 256-byte header and the column
  * tables, work_bytes >= 256 + one view's tables (nu * (nv+1) * 16 bytes).
- * Plane-separable: one record per view of 4 * (r4(4 nz + 2 nu + 2 nv) +
- * nu nv) bytes rounded up to 16 (r4: up to a multiple of 4), work_bytes >=
- * one record.  ct2_backproject_work_bytes(desc) bytes hold every view of the
+ * Plane-separable: a 256-byte header and one record per view of
+ * 4 * (r4(4 nz + 2 nu + 2 nv) + nu nv) bytes rounded up to 16 (r4: up to a
+ * multiple of 4), work_bytes >= 256 + one record.  ct2_backproject_work_bytes(desc) bytes hold every view of the
  * subset, and a smaller workspace processes the views in groups.  The call
  * allocates nothing; it must not overlap another call that uses the same
  * workspace.  work == NULL behaves as

@@ -176,11 +176,12 @@ def test_zsep_dot_product(seed):
 
 
 def test_zsep_workspace_groups():
-    """ct2_backproject_work_bytes gives the route's workspace (per view: the
-    planes' maps, the columns' and rows' set-ups, trav * datum of every ray,
-    in 16-byte aligned parts); through ct2_backproject_ws a workspace of one
-    view's record runs the views one group at a time (the first stores, the
-    rest add) and matches the allocating entry; a smaller one is rejected."""
+    """ct2_backproject_work_bytes gives the route's workspace (a 256-byte
+    header, then per view the planes' maps, the columns' and rows' set-ups
+    and trav * datum of every ray, in 16-byte aligned parts); through
+    ct2_backproject_ws a workspace of one view's record runs the views one
+    group at a time (the first stores, the rest add) and matches the
+    allocating entry; a smaller one is rejected."""
     from ct2 import _dev
     from ct2._lib import CT2Error, call, lib
     fv, geo, g2, ind, rng = _case(ns=6)
@@ -190,13 +191,13 @@ def test_zsep_workspace_groups():
     d = _desc(geo, g2)
     r4 = lambda n: (n + 3) // 4 * 4   # noqa: E731
     rec = r4(r4(4 * nz + 2 * nu + 2 * nv) + nu * nv) * 4
-    assert lib().ct2_backproject_work_bytes(C.byref(d)) == ns * rec
+    assert lib().ct2_backproject_work_bytes(C.byref(d)) == 256 + ns * rec
     want = _adjoint(fv, geo, g2, ind, y)
     yd, fvd, indd = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (y, fv, ind))
     out = torch.full((nx, ny, nz), float("nan"), dtype=torch.float32, device="cuda")
-    work = torch.empty(rec // 4, dtype=torch.float32, device="cuda")
+    work = torch.empty((256 + rec) // 4, dtype=torch.float32, device="cuda")
     call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(yd), _dev.ptr(indd),
-         _dev.ptr(fvd), _dev.ptr(work), rec, _dev.stream_handle())
+         _dev.ptr(fvd), _dev.ptr(work), 256 + rec, _dev.stream_handle())
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     e = rel_l2(got, want)
@@ -204,4 +205,4 @@ def test_zsep_workspace_groups():
     assert e <= 1e-6, e
     with pytest.raises(CT2Error):
         call("ct2_backproject_ws", C.byref(d), _dev.ptr(out), _dev.ptr(yd), _dev.ptr(indd),
-             _dev.ptr(fvd), _dev.ptr(work), rec - 16, _dev.stream_handle())
+             _dev.ptr(fvd), _dev.ptr(work), 256 + rec - 16, _dev.stream_handle())
